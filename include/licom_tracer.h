@@ -1,0 +1,172 @@
+/*
+ * licom_tracer.h -- C ABI of the MI355X (gfx950) LICOM3 tracer time step.
+ *
+ * Drop-in for the reference's tracer path:
+ *   extern "C" void tracer_(int ii)      tracer.cpp:1701-1903
+ *   extern "C" void convadj_()           convadj.cpp:290-298
+ * called from stepon_ (stepon.cpp:69,77).  The reference reaches every array
+ * through file-scope device globals (cuda_data.h:35-345) and Fortran module
+ * symbols (pconst_mod_mp_dts_ ...); this ABI passes the same arrays
+ * explicitly, in the same layout:
+ *
+ *   Fortran (imt,jmt,km[,ntra]), i fastest == C [ntra][km][jmt][imt],
+ *   imt = nx + 4, jmt = ny + 4 (two ghost cells, param_mod.h:18-20),
+ *   atb is (imt,jmt,0:km,ntra): level k lives at plane k+1 (tracer_mod.h:9),
+ *   ws is (imt,jmt,km+1) (dyn_mod.h:29), 2-D fields are [jmt][imt].
+ *
+ * Grid and coefficient inputs are host pointers (read once by create);
+ * state, forcing and diagnostics are device pointers.  Every function returns
+ * LICOM_OK (0) or a negative error code and never exits the process (the
+ * reference exits through CHECK/iCheck, cuda_data.h:4-11, common.h:6-16).
+ * One context serves one stream; contexts are not thread safe.
+ */
+#ifndef LICOM_TRACER_H
+#define LICOM_TRACER_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LICOM_MAX_KM 64
+
+enum licom_status {
+    LICOM_OK = 0,
+    LICOM_EINVAL = -1,   /* bad argument or grid (kmt > km, imt < 8, ...) */
+    LICOM_ENOMEM = -2,   /* device allocation failed                      */
+    LICOM_EHIP = -3,     /* HIP runtime error (message: licom_last_error)  */
+    LICOM_ENODEV = -4,   /* no usable gfx950 device                        */
+    LICOM_ENOTSUP = -5   /* option not built (scheme, precision)           */
+};
+
+enum licom_hmix { LICOM_HMIX_DEL2 = 2, LICOM_HMIX_DEL4 = 4 };
+enum licom_adv { LICOM_ADV_TSPAS = 2 };
+
+/* licom_tracer_step flags */
+#define LICOM_STEP_CONVADJ 0x1      /* run convadj after the tracer update    */
+#define LICOM_STEP_SIDE_EFFECTS 0x2 /* reproduce the reference's in-place
+                                       mutations: utf,vtf *= oncc; h0f *= onbc;
+                                       AKT surface floor written back
+                                       (tracer.cpp:1088,1107-1108,1187,1315) */
+#define LICOM_STEP_VTL_SCRATCH 0x4  /* leave the new S in vtl, as the reference
+                                       does with its scratch reuse
+                                       (tracer.cpp:1638,1891)                 */
+
+/* Static grid (host pointers, copied by licom_tracer_create).  Replaces the
+ * grid/pconst module arrays read through cuda_init.cpp:708-864. */
+typedef struct licom_grid {
+    int nx, ny, km;           /* physical extents; imt = nx+4, jmt = ny+4      */
+    int imt_global;           /* zonal count in the AHF normalisation
+                                 (hmix_del2.F90:346, hmix_del4.F90:357)        */
+    const int32_t *kmt;       /* [jmt][imt] levels per T column; ghost cells
+                                 are refilled (cyclic i, closed j)             */
+    const double *zkp;        /* [km+1] interface depths: 0, -dz1, ...         */
+    const double *pen;        /* [km-1] shortwave fraction times OD0CP, or NULL*/
+    const double *htw, *hts;  /* [jmt][imt] west/south face lengths of T cells */
+    const double *hun, *hue;  /* [jmt][imt] north/east face lengths of U cells */
+    const double *tarea;      /* [jmt][imt] T-cell area                        */
+    const double *au0, *aus, *auw, *ausw; /* [jmt][imt] T->U weights, or NULL
+                                 for the lat-lon 0.25 (grid.F90:1654-1657)     */
+    const double *to, *so;    /* [km] EOS reference T and S                    */
+    const double *eosc;       /* [9][km] = Fortran C(km,9), kg/m^3             */
+} licom_grid;
+
+typedef struct licom_params {
+    double dts;               /* tracer step (s), pconst_mod dts              */
+    double onbc, oncc;        /* 1/(NBB*NCC+1), 1/(NCC+1)                      */
+    double dwndmix;           /* AKT surface floor (m^2/s)                    */
+    double gamma;             /* restoring rate (1/s)                         */
+    double od0, od0cp;        /* 1/D0, 1/(D0*CP)                              */
+    double ah;                /* AH_HOR (del2) or AH_BIHAR (del4)             */
+    int hmix;                 /* LICOM_HMIX_DEL2 / LICOM_HMIX_DEL4            */
+    int adv;                  /* LICOM_ADV_TSPAS                              */
+    int boundary_restore;     /* 1: restoring term of tracer_7                 */
+    int sssnorm;              /* 1: global freshwater normalisation of S       */
+} licom_params;
+
+/* Device arrays of one step (the d_* globals of cuda_data.h). */
+typedef struct licom_fields {
+    double *at;               /* [2][km][jmt][imt]        in/out               */
+    double *atb;              /* [2][km+1][jmt][imt]      in/out               */
+    double *akt;              /* [2][km][jmt][imt]        in (k=0 floor out)   */
+    double *utf, *vtf;        /* [km][jmt][imt]           in (scaled out)      */
+    const double *utl;        /* [km][jmt][imt]           in                   */
+    double *vtl;              /* [km][jmt][imt]           in (S out if flag)   */
+    double *ws;               /* [km+1][jmt][imt]         out (plane 0 in)     */
+    double *h0f;              /* [jmt][imt]               in (scaled out)      */
+    const double *h0l;        /* [jmt][imt]               in                   */
+    const double *swv, *nswv, *sst, *seaice, *sss, *fresh; /* [jmt][imt] or NULL */
+    const double *restore;    /* [2][km][jmt][imt] or NULL                     */
+} licom_fields;
+
+/* Device diagnostics; any pointer may be NULL (not produced).  Values are
+ * written on the physical cells [2,jmt-2) x [2,imt-2); ghost cells are left
+ * untouched. */
+typedef struct licom_diag {
+    double *ax, *ay, *az;     /* [2][km][jmt][imt] advective tendencies        */
+    double *dz;               /* [2][km][jmt][imt] explicit vertical diffusion */
+    double *tend;             /* [2][km][jmt][imt] total tendency              */
+    double *dt_diff;          /* [2][km][jmt][imt] implicit diffusion          */
+    double *dt_conv;          /* [2][km][jmt][imt] convective adjustment       */
+    double *penetrate;        /* [km][jmt][imt]    shortwave penetration       */
+    double *net;              /* [2][jmt][imt]     surface flux (every column) */
+    double *fw_norm2;         /* [1] SSSNORM correction (device scalar)        */
+} licom_diag;
+
+/* Derived static fields, for inspection (host buffers sized by the caller). */
+typedef struct licom_grid_derived {
+    int32_t *kmt, *kmu;                      /* [jmt][imt]                     */
+    double *tarea_r, *ohbt, *ohbu;           /* [jmt][imt]                     */
+    double *dtn, *dts, *dte, *dtw, *ahf;     /* [jmt][imt]                     */
+    double *dzp, *odzp, *odzt;               /* [km]                           */
+    double area_t;                           /* out                            */
+} licom_grid_derived;
+
+typedef struct licom_ctx licom_ctx;
+
+/* Creates a context on the current HIP device: validates and copies the grid,
+ * derives the masks and coefficients of boundary.F90:29-52,
+ * grid.F90:1096-1106/1343-1346/2029-2050 and hmix_del{2,4}.F90 init_del*t,
+ * and allocates scratch for one step.  Replaces cudainit()/turbinit()
+ * (stepon.cpp:50-51, cuda_init.cpp:372-864). */
+int licom_tracer_create(const licom_grid *grid, const licom_params *params,
+                        licom_ctx **out);
+
+/* Binds the context to a hipStream_t (NULL = default stream). */
+int licom_tracer_set_stream(licom_ctx *ctx, void *hip_stream);
+
+/* One tracer step for T and S: tracer_(ii) of tracer.cpp:1701-1903, followed
+ * by convadj_() (convadj.cpp:290) when LICOM_STEP_CONVADJ is set.  Enqueued
+ * on the context's stream; returns after launch. */
+int licom_tracer_step(licom_ctx *ctx, const licom_fields *f,
+                      const licom_diag *d, int flags);
+
+/* Convective adjustment alone: convadj_() of convadj.cpp:290-298 over every
+ * column, ghost columns included. */
+int licom_convadj(licom_ctx *ctx, const licom_fields *f, const licom_diag *d);
+
+/* Waits for the context's stream. */
+int licom_tracer_sync(licom_ctx *ctx);
+
+/* Copies the derived masks and coefficients to host buffers. */
+int licom_grid_export(const licom_ctx *ctx, licom_grid_derived *out);
+
+/* Per-kernel timing with HIP events on the context stream.  When enabled,
+ * every launch of a step is bracketed by events; licom_kernel_times returns
+ * accumulated milliseconds and launch counts per kernel name. */
+int licom_set_profiling(licom_ctx *ctx, int enable);
+int licom_kernel_times(licom_ctx *ctx, int max, const char **names,
+                       double *ms, int *counts, int *n_out);
+
+/* Bytes of device scratch held by the context. */
+size_t licom_scratch_bytes(const licom_ctx *ctx);
+
+void licom_tracer_destroy(licom_ctx *ctx);
+const char *licom_strerror(int status);
+const char *licom_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,84 @@
+// tracer_internal.h -- context and kernel interfaces shared by the HIP
+// sources of liblicom_tracer.so.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "licom_tracer.h"
+
+namespace licom {
+
+// Static grid and coefficients as seen by the kernels (device pointers).
+struct KGrid {
+    int imt, jmt, km;
+    const int32_t *kmt, *kmu;
+    const double *tarea, *tarea_r, *htw, *hts, *hun, *hue;
+    const double *ohbt, *ohbu, *dtn, *dts, *dte, *dtw, *ahf;
+    const double *au0, *aus, *auw, *ausw;
+    const double *dzp, *odzp, *odzt, *pen, *to, *so, *eosc;
+};
+
+struct KPar {
+    double dts, onbc, oncc, dwndmix, gamma, od0, od0cp, ah;
+    int hmix, boundary_restore, sssnorm;
+};
+
+struct Scratch {
+    double *stf_pre = nullptr;   // [jmt][imt] 0.5*h0f*onbc + 0.5*h0l
+    double *work_u = nullptr;    // [jmt][imt] T->U average of stf_pre
+    double *stf_n = nullptr;     // [2][jmt][imt] surface flux seen by tracer n
+    double *temp11 = nullptr;    // [jmt][imt] SSSNORM integrand
+    double *partials = nullptr;  // [2*nblk] block partial sums (hi, lo)
+    double *err_norm2 = nullptr; // [1]
+    double *uw = nullptr, *vs = nullptr;   // [km][jmt][imt] face velocities
+    uint8_t *flags = nullptr;    // [2][km][jmt][imt] tspas limiter flags
+    double *d2tk = nullptr;      // [2][km][jmt][imt] del4 first pass
+    double *ori = nullptr;       // [2][km][jmt][imt] explicit update
+    int nblk = 0;
+};
+
+struct TimedEvent {
+    const char *name;
+    hipEvent_t start, stop;
+};
+
+}  // namespace licom
+
+struct licom_ctx {
+    int nx, ny, km, imt, jmt;
+    size_t n2, n3;
+    licom_params p;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    licom::KGrid g{};
+    licom::KPar kp{};
+    double area_t = 0.0;
+    // device allocations owned by the context
+    std::vector<void *> allocs;
+    size_t scratch_bytes = 0;
+    licom::Scratch s;
+    // host copies of the derived fields (for licom_grid_export)
+    std::vector<int32_t> h_kmt, h_kmu;
+    std::vector<double> h_tarea_r, h_ohbt, h_ohbu, h_dtn, h_dts, h_dte, h_dtw, h_ahf;
+    std::vector<double> h_dzp, h_odzp, h_odzt;
+    // profiling
+    bool profiling = false;
+    std::vector<licom::TimedEvent> pending;
+    std::vector<std::string> prof_names;
+    std::vector<double> prof_ms;
+    std::vector<int> prof_counts;
+};
+
+namespace licom {
+
+void set_error(const char *fmt, ...);
+int hip_status(hipError_t e, const char *what);
+
+// Kernel launchers (tracer_kernels.hip).  All enqueue on ctx->stream.
+int launch_step(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags);
+int launch_convadj(licom_ctx *ctx, const licom_fields *f, const licom_diag *d);
+
+}  // namespace licom

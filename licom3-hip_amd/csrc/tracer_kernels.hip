@@ -1,0 +1,854 @@
+// tracer_kernels.hip -- gfx950 kernels of the LICOM3 tracer step (FP64).
+//
+// Semantics follow the reference's active HIP path (tracer.cpp, convadj.cpp)
+// expression by expression; the library is built with -ffp-contract=off so
+// that every product and sum rounds exactly where the reference's does.
+//
+// Launch sequence of one step (both tracers advance in every launch):
+//   k_surface   2-D   stf, T->U average, surface fluxes, SSSNORM integrand
+//   k_sss_*     2-D   deterministic compensated global sum (SSSNORM)
+//   k_faces     3-D   face velocities u_wface, v_sface
+//   k_ws        col   divergence, column integral, continuity w
+//   k_predict   3-D   tspas predictor and local bounds -> limiter flags
+//   k_d2tk      3-D   first del2 pass of the biharmonic operator
+//   k_correct   3-D   limited fluxes, mixing, explicit vertical terms
+//   k_column    col   implicit vertical diffusion, tendencies, convadj
+//   k_halo      ghost cells of at/atb (cyclic i, closed j)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+
+#include "tracer_internal.h"
+
+namespace licom {
+namespace {
+
+constexpr int KMAX = LICOM_MAX_KM;
+
+__device__ __forceinline__ size_t i2(const KGrid &g, int j, int i) {
+    return (size_t)j * g.imt + i;
+}
+__device__ __forceinline__ size_t i3(const KGrid &g, int k, int j, int i) {
+    return ((size_t)k * g.jmt + j) * g.imt + i;
+}
+__device__ __forceinline__ double vitf(const KGrid &g, int k, int j, int i) {
+    return (k < g.kmt[i2(g, j, i)]) ? 1.0 : 0.0;
+}
+
+// ---------------------------------------------------------------- surface
+// tracer_1_0 (tracer.cpp:1081-1093), T->U average of tracer_1
+// (:1131-1149), surface fluxes of tracer_5 n=1 (:1479-1524, FRC_CORE) and
+// tracer_6 n=0 (:1543-1566, FRC_CORE), SSSNORM integrand (:1517-1522).
+__device__ __forceinline__ double stf0(const double *h0f, const double *h0l, double onbc, size_t q) {
+    return 0.5 * (h0f[q] * onbc) + 0.5 * h0l[q];
+}
+
+__global__ void k_surface(KGrid g, KPar p, const double *__restrict__ h0f,
+                          const double *__restrict__ h0l, const double *__restrict__ atb,
+                          const double *swv, const double *nswv, const double *sst,
+                          const double *seaice, const double *sss, const double *fresh,
+                          double *stf_pre, double *work_u, double *stf_n, double *temp11,
+                          double *net) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.imt || j >= g.jmt) return;
+    const size_t q = i2(g, j, i), n2 = (size_t)g.imt * g.jmt;
+    const double s0 = stf0(h0f, h0l, p.onbc, q);
+    stf_pre[q] = s0;
+    double wu = 0.0;
+    if (j < g.jmt - 1 && i >= 1) {
+        wu = g.au0[q] * s0 + g.aus[q] * stf0(h0f, h0l, p.onbc, i2(g, j + 1, i)) +
+             g.auw[q] * stf0(h0f, h0l, p.onbc, i2(g, j, i - 1)) +
+             g.ausw[q] * stf0(h0f, h0l, p.onbc, i2(g, j + 1, i - 1));
+    }
+    work_u[q] = wu;
+    double sT = s0, sS = s0, t11 = 0.0;
+    if (g.kmt[q] > 0) {
+        const double odzp0 = g.odzp[0];
+        const double tb = atb[n2 + q];                       // atb(:,:,1,1)
+        const double sb = atb[(size_t)(g.km + 1) * n2 + n2 + q];  // atb(:,:,1,2)
+        const double fw = fresh ? fresh[q] : 0.0, si = seaice ? seaice[q] : 0.0;
+        const double ss = sss ? sss[q] : 0.0, st = sst ? sst[q] : 0.0;
+        const double sw = swv ? swv[q] : 0.0, nsw = nswv ? nswv[q] : 0.0;
+        sS = fw * 34.7 * p.od0 / 1000.0 + p.gamma * (ss - sb) * si / odzp0 +
+             p.gamma * 30.0 / 365.0 / 4.0 * 50.0 * (ss - sb) / odzp0 * (1.0 - si);
+        sT = (sw + nsw) * p.od0cp + si * p.gamma * (st - tb) / odzp0;
+        const double n1 = sS * odzp0;
+        t11 = g.tarea[q] * n1 * 1.0;
+        if (net) {
+            net[q] = sT * odzp0;
+            net[n2 + q] = n1;
+        }
+    }
+    stf_n[q] = sT;
+    stf_n[n2 + q] = sS;
+    temp11[q] = t11;
+}
+
+// Compensated (TwoSum) accumulation of the SSSNORM integrand over the
+// physical cells; the reference sums on the host in j-major order and then
+// MPI_ALLREDUCEs (tracer.cpp:1849-1868, mpi3.F90:26).  The order here is
+// fixed by the launch shape, so repeated runs give identical bits.
+__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+    s = a + b;
+    double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+constexpr int kRedThreads = 256;
+
+__global__ void k_sss_partial(KGrid g, const double *__restrict__ temp11, double *partials) {
+    __shared__ double sh[kRedThreads], sl[kRedThreads];
+    const int nx = g.imt - 4, ny = g.jmt - 4;
+    const size_t total = (size_t)nx * ny;
+    double hi = 0.0, lo = 0.0;
+    for (size_t t = (size_t)blockIdx.x * kRedThreads + threadIdx.x; t < total;
+         t += (size_t)gridDim.x * kRedThreads) {
+        const int jj = (int)(t / nx), ii = (int)(t % nx);
+        double s, e;
+        two_sum(hi, temp11[i2(g, jj + 2, ii + 2)], s, e);
+        hi = s;
+        lo += e;
+    }
+    sh[threadIdx.x] = hi;
+    sl[threadIdx.x] = lo;
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            double s, e;
+            two_sum(sh[threadIdx.x], sh[threadIdx.x + w], s, e);
+            sh[threadIdx.x] = s;
+            sl[threadIdx.x] = sl[threadIdx.x] + sl[threadIdx.x + w] + e;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = sh[0];
+        partials[2 * blockIdx.x + 1] = sl[0];
+    }
+}
+
+__global__ void k_sss_final(const double *__restrict__ partials, int nblk, double area_t,
+                            int enabled, double *err_norm2, double *out_copy) {
+    __shared__ double sh[kRedThreads], sl[kRedThreads];
+    double hi = 0.0, lo = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kRedThreads) {
+        double s, e;
+        two_sum(hi, partials[2 * b], s, e);
+        hi = s;
+        lo += e + partials[2 * b + 1];
+    }
+    sh[threadIdx.x] = hi;
+    sl[threadIdx.x] = lo;
+    __syncthreads();
+    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            double s, e;
+            two_sum(sh[threadIdx.x], sh[threadIdx.x + w], s, e);
+            sh[threadIdx.x] = s;
+            sl[threadIdx.x] = sl[threadIdx.x] + sl[threadIdx.x + w] + e;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double total = sh[0] + sl[0];
+        const double v = enabled ? -total / area_t : 0.0;
+        *err_norm2 = v;
+        if (out_copy) *out_copy = v;
+    }
+}
+
+// ------------------------------------------------------------ velocities
+__device__ __forceinline__ double wkd_at(const double *utf, const double *utl, double oncc,
+                                         size_t c) {
+    return 0.5 * (utf[c] * oncc) + 0.5 * utl[c];   // tracer_1 (:1107-1119)
+}
+
+// advection_tracer_1_0, tspas branch (tracer.cpp:83-104)
+__global__ void k_faces(KGrid g, KPar p, const double *__restrict__ utf,
+                        const double *__restrict__ utl, const double *__restrict__ vtf,
+                        const double *__restrict__ vtl, double *uw, double *vs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.imt || j >= g.jmt) return;
+    const size_t c = i3(g, k, j, i), q = i2(g, j, i);
+    double v = 0.0, u = 0.0;
+    if (i >= 1 && i < g.imt - 1)
+        v = (wkd_at(vtf, vtl, p.oncc, c) + wkd_at(vtf, vtl, p.oncc, c + 1)) * g.hts[q] * 0.25;
+    if (j >= 1 && j < g.jmt - 1)
+        u = (wkd_at(utf, utl, p.oncc, c - g.imt) + wkd_at(utf, utl, p.oncc, c)) * g.htw[q] * 0.25;
+    vs[c] = v;
+    uw[c] = u;
+}
+
+// upwell_2, operators_div, upwell_3, upwell_4 (tracer.cpp:980-1080):
+// barotropic-corrected continuity w, one column per lane.
+__global__ void k_ws(KGrid g, KPar p, const double *__restrict__ utf,
+                     const double *__restrict__ utl, const double *__restrict__ vtf,
+                     const double *__restrict__ vtl, const double *__restrict__ work_u,
+                     const double *__restrict__ stf_pre, double *ws) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i < 1 || i >= g.imt - 1 || j < 1 || j >= g.jmt - 1) return;
+    const int kmt = g.kmt[i2(g, j, i)];
+    const size_t qc = i2(g, j, i), qe = i2(g, j, i + 1), qn = i2(g, j - 1, i),
+                 qne = i2(g, j - 1, i + 1);
+    const double fc = 1.0 + work_u[qc] * g.ohbu[qc], fe = 1.0 + work_u[qe] * g.ohbu[qe];
+    const double fn = 1.0 + work_u[qn] * g.ohbu[qn], fne = 1.0 + work_u[qne] * g.ohbu[qne];
+    const double htw_c = g.htw[qc], htw_e = g.htw[qe], hts_c = g.hts[qc], hts_n = g.hts[qn];
+    const double tr = g.tarea_r[qc];
+    double wka[KMAX];
+    double w = 0.0;
+    for (int k = 0; k < g.km; ++k) {
+        double d = 0.0;
+        if (k <= kmt - 1) {
+            const size_t o = (size_t)k * g.imt * g.jmt;
+            const double ukc = fc * wkd_at(utf, utl, p.oncc, o + qc);
+            const double uke = fe * wkd_at(utf, utl, p.oncc, o + qe);
+            const double ukn = fn * wkd_at(utf, utl, p.oncc, o + qn);
+            const double ukne = fne * wkd_at(utf, utl, p.oncc, o + qne);
+            const double vkc = fc * wkd_at(vtf, vtl, p.oncc, o + qc);
+            const double vke = fe * wkd_at(vtf, vtl, p.oncc, o + qe);
+            const double vkn = fn * wkd_at(vtf, vtl, p.oncc, o + qn);
+            const double vkne = fne * wkd_at(vtf, vtl, p.oncc, o + qne);
+            d = 0.5 * ((uke + ukne) * htw_e - (ukc + ukn) * htw_c + (vke + vkc) * hts_c -
+                       (vkne + vkn) * hts_n) * tr;
+        }
+        wka[k] = d;
+        w = w - g.dzp[k] * d * (k < kmt ? 1.0 : 0.0);
+    }
+    const double ohbt = g.ohbt[qc];
+    const double sc = 1.0 / (1.0 + stf_pre[qc] * ohbt);
+    double prev = ws[qc];
+    for (int k = 1; k < g.km; ++k) {
+        const double cur = (k < kmt ? 1.0 : 0.0) * (prev + g.dzp[k - 1] * (w * ohbt + wka[k - 1]));
+        ws[i3(g, k, j, i)] = cur * sc;
+        prev = cur;
+    }
+}
+
+// ------------------------------------------------------------------ tspas
+struct PTerms {
+    double x0, y0, xy1, xy2, xy3, xy4, c1, c2, za, zb1, zb2, zc;
+};
+
+// advection_tracer_2 predictor terms (tracer.cpp:311-414)
+__device__ __forceinline__ void predictor_terms(const KGrid &g, double dts, const double *T,
+                                                const double *U, const double *V,
+                                                const double *W, int k, int j, int i,
+                                                PTerms &t) {
+    const size_t c = i3(g, k, j, i), q = i2(g, j, i);
+    const size_t sj = g.imt, sk = (size_t)g.imt * g.jmt;
+    const double tr = g.tarea_r[q];
+    const double Tc = T[c], Te = T[c + 1], Tw = T[c - 1], Ts = T[c + sj], Tn = T[c - sj];
+    const double Ue = U[c + 1], Uw = U[c], Vs = V[c], Vn = V[c - sj];
+    t.x0 = (Te + Tc) * Ue * tr - (Tc + Tw) * Uw * tr;
+    t.y0 = ((Ts + Tc) * Vs - (Tc + Tn) * Vn) * tr;
+    t.xy1 = -dts * (Te - Tc) * 2.0 * tr * Ue * Ue / (g.htw[q + 1] * g.hun[q + 1]);
+    t.xy2 = dts * (Tc - Tw) * 2.0 * tr * Uw * Uw / (g.htw[q] * g.hun[q]);
+    t.xy3 = -dts * (Ts - Tc) * 2.0 * tr * Vs * Vs / (g.hts[q] * g.hue[q]);
+    t.xy4 = dts * (Tc - Tn) * 2.0 * tr * Vn * Vn / (g.hts[q - sj] * g.hue[q - sj]);
+    t.c1 = -Tc * (Ue - Uw) * tr * 2.0;
+    t.c2 = -Tc * (Vs - Vn) * tr * 2.0;
+    const double *odzp = g.odzp, *odzt = g.odzt;
+    if (k == 0) {
+        const double W1 = W[c + sk], T1 = T[c + sk];
+        t.za = -0.5 * odzp[0] * W1 * (T1 + Tc);
+        t.zb1 = 0.0;
+        t.zb2 = 0.5 * odzp[0] * W1 * W1 * odzt[1] * (Tc - T1) * dts;
+        t.zc = odzp[0] * Tc * W1;
+    } else if (k == g.km - 1) {
+        const double Wk = W[c], Tu = T[c - sk];
+        t.za = 0.5 * odzp[k] * Wk * (Tc + Tu);
+        t.zb1 = -0.5 * odzp[k] * Wk * Wk * odzt[k] * (Tu - Tc) * dts;
+        t.zb2 = 0.0;
+        t.zc = -odzp[k] * Tc * Wk;
+    } else {
+        const double Wk = W[c], Wb = W[c + sk], Tu = T[c - sk], Td = T[c + sk];
+        t.za = 0.5 * odzp[k] * Wk * (Tc + Tu) - 0.5 * odzp[k] * Wb * (Tc + Td);
+        t.zb1 = -0.5 * odzp[k] * Wk * Wk * odzt[k] * (Tu - Tc) * dts;
+        t.zb2 = 0.5 * odzp[k] * Wb * Wb * odzt[k + 1] * (Tc - Td) * dts;
+        t.zc = -odzp[k] * Tc * (Wk - Wb);
+    }
+}
+
+// Value entering the 7-point bounds: at on ocean cells, the sentinel on land
+// (at*vit + (1-vit)*wt of tracer.cpp:418-545, exact for vit in {0,1}).
+__device__ __forceinline__ double bval(const KGrid &g, const double *T, int k, int j, int i,
+                                       double sentinel) {
+    return vitf(g, k, j, i) != 0.0 ? T[i3(g, k, j, i)] : sentinel;
+}
+
+// advection_tracer_2 (tracer.cpp:291-547): predictor and the limiter flag
+// at00 > atmax || at00 < atmin, for both tracers.
+__global__ void k_predict(KGrid g, KPar p, const double *__restrict__ at,
+                          const double *__restrict__ uw, const double *__restrict__ vs,
+                          const double *__restrict__ ws, uint8_t *flags) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.imt || j >= g.jmt) return;
+    const size_t n3 = (size_t)g.imt * g.jmt * g.km, c = i3(g, k, j, i);
+    const bool inside = i >= 1 && i < g.imt - 1 && j >= 1 && j < g.jmt - 1;
+    for (int n = 0; n < 2; ++n) {
+        uint8_t fl = 0;
+        if (inside) {
+            const double *T = at + n * n3;
+            PTerms t;
+            predictor_terms(g, p.dts, T, uw, vs, ws, k, j, i, t);
+            const double xx = -(t.x0 + t.xy1 + t.xy2 + t.c1);
+            const double yy = -(t.y0 + t.xy3 + t.xy4 + t.c2);
+            const double zz = -(t.zb1 + t.zb2 + t.za + t.zc);
+            const double a00 = T[c] + (xx + yy + zz) * p.dts;
+            const double lo = -1.0e10, hi = 1.0e10;
+            double mx = bval(g, T, k, j, i, lo), mn = bval(g, T, k, j, i, hi);
+            const int nb[4][2] = {{j - 1, i}, {j + 1, i}, {j, i - 1}, {j, i + 1}};
+            for (int m = 0; m < 4; ++m) {
+                mx = fmax(mx, bval(g, T, k, nb[m][0], nb[m][1], lo));
+                mn = fmin(mn, bval(g, T, k, nb[m][0], nb[m][1], hi));
+            }
+            if (k > 0) {
+                mx = fmax(mx, bval(g, T, k - 1, j, i, lo));
+                mn = fmin(mn, bval(g, T, k - 1, j, i, hi));
+            }
+            if (k < g.km - 1) {
+                mx = fmax(mx, bval(g, T, k + 1, j, i, lo));
+                mn = fmin(mn, bval(g, T, k + 1, j, i, hi));
+            }
+            fl = (a00 > mx || a00 < mn) ? 1 : 0;
+        }
+        flags[n * n3 + c] = fl;
+    }
+}
+
+// tracer_del4_1 (tracer.cpp:1238-1293): d2tk = AHF * del2(atb), on
+// [1,jmt-1) x [1,imt-1); zero elsewhere.
+__device__ __forceinline__ void del2_sums(const KGrid &g, const double *X, int k, int j, int i,
+                                          double &hdtk, double &cc) {
+    const size_t q = i2(g, j, i), sj = g.imt;
+    hdtk = 0.0;
+    cc = 0.0;
+    if (k <= g.kmt[q] - 1) {
+        double c;
+        if (k <= g.kmt[q - sj] - 1) { c = g.dtn[q]; hdtk += c * X[q - sj]; cc -= c; }
+        if (k <= g.kmt[q + sj] - 1) { c = g.dts[q]; hdtk += c * X[q + sj]; cc -= c; }
+        if (k <= g.kmt[q + 1] - 1) { c = g.dte[q]; hdtk += c * X[q + 1]; cc -= c; }
+        if (k <= g.kmt[q - 1] - 1) { c = g.dtw[q]; hdtk += c * X[q - 1]; cc -= c; }
+    }
+}
+
+__global__ void k_d2tk(KGrid g, const double *__restrict__ atb, double *d2tk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= g.imt || j >= g.jmt) return;
+    const size_t n2 = (size_t)g.imt * g.jmt, n3 = n2 * g.km;
+    const bool inside = i >= 1 && i < g.imt - 1 && j >= 1 && j < g.jmt - 1;
+    for (int n = 0; n < 2; ++n) {
+        double v = 0.0;
+        if (inside) {
+            const double *X = atb + n * (n3 + n2) + (size_t)(k + 1) * n2;
+            double h, cc;
+            del2_sums(g, X, k, j, i, h, cc);
+            v = g.ahf[i2(g, j, i)] * (cc * X[i2(g, j, i)] + h);
+        }
+        d2tk[n * n3 + i3(g, k, j, i)] = v;
+    }
+}
+
+__device__ __forceinline__ double akt_floored(const KPar &p, const double *akt, int n, int k,
+                                              size_t c) {
+    double a = akt[c];
+    // AKT surface floor: del4 applies it to T only (tracer.cpp:1315), del2
+    // to every tracer (:1187)
+    if (k == 0 && (n == 0 || p.hmix == LICOM_HMIX_DEL2) && a < p.dwndmix) a = p.dwndmix;
+    return a;
+}
+
+// advection_tracer_3 (tracer.cpp:549-921), tracer_del4_2 / tracer_4
+// (:1167-1362), tracer_5 (:1395-1525), tracer_6 (:1527-1569),
+// tracer_7 (:1571-1627), tracer_8 (:1629-1653) on the physical cells.
+__global__ void k_correct(KGrid g, KPar p, const double *__restrict__ at,
+                          const double *__restrict__ atb, const double *__restrict__ akt,
+                          const double *__restrict__ uw, const double *__restrict__ vs,
+                          const double *__restrict__ ws, const uint8_t *__restrict__ flags,
+                          const double *__restrict__ d2tk, const double *__restrict__ stf_n,
+                          const double *__restrict__ err_norm2, const double *swv,
+                          const double *restore, double *ori, double *ax, double *ay, double *az,
+                          double *dzd, double *pen_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y + 2;
+    const int k = blockIdx.z;
+    if (i >= g.imt - 2 || j >= g.jmt - 2) return;
+    const int km = g.km;
+    const size_t n2 = (size_t)g.imt * g.jmt, n3 = n2 * km;
+    const size_t c = i3(g, k, j, i), q = i2(g, j, i), sj = g.imt;
+    const double *odzp = g.odzp, *odzt = g.odzt;
+    const double vitc = vitf(g, k, j, i);
+    const double aidif = 0.5;
+    for (int n = 0; n < 2; ++n) {
+        const double *T = at + n * n3;
+        const uint8_t *F = flags + n * n3;
+        PTerms t;
+        predictor_terms(g, p.dts, T, uw, vs, ws, k, j, i, t);
+        const double tr = g.tarea_r[q];
+        const double Tc = T[c];
+        const bool fc = F[c], fe = F[c + 1], fw = F[c - 1], fs = F[c + sj], fn = F[c - sj];
+        bool up1, up2, up3, up4;
+        if (k == 0 || k == km - 1) {
+            up1 = fc || (fe && i <= g.imt - 3);
+            up2 = fc || (fw && i >= 2);
+            up3 = fc || (fs && j <= g.jmt - 3);
+            up4 = fc || (fn && j >= 2);
+        } else {
+            up1 = (fc || fe) && i <= g.imt - 3;
+            up2 = (fc || fw) && i >= 2;
+            up3 = (fc || fs) && j <= g.jmt - 3;
+            up4 = (fc || fn) && j >= 2;
+        }
+        const double xy1 = up1 ? -(T[c + 1] - Tc) * fabs(uw[c + 1]) * tr : t.xy1;
+        const double xy2 = up2 ? (Tc - T[c - 1]) * fabs(uw[c]) * tr : t.xy2;
+        const double xy3 = up3 ? -(T[c + sj] - Tc) * fabs(vs[c]) * tr : t.xy3;
+        const double xy4 = up4 ? (Tc - T[c - sj]) * fabs(vs[c - sj]) * tr : t.xy4;
+        double zb1, zb2;
+        if (k == 0) {
+            zb1 = 0.0;
+            zb2 = (fc || F[c + n2]) ? 0.5 * fabs(ws[c + n2]) * odzp[0] * (Tc - T[c + n2]) : t.zb2;
+        } else if (k == km - 1) {
+            zb1 = (fc || F[c - n2]) ? -0.5 * fabs(ws[c]) * odzp[k] * (T[c - n2] - Tc) : t.zb1;
+            zb2 = 0.0;
+        } else {
+            zb1 = (fc || F[c - n2]) ? -0.5 * fabs(ws[c]) * odzp[k] * (T[c - n2] - Tc) : t.zb1;
+            zb2 = (fc || F[c + n2]) ? 0.5 * fabs(ws[c + n2]) * odzp[k] * (Tc - T[c + n2]) : t.zb2;
+        }
+        const double xx = -(t.x0 + xy1 + xy2 + t.c1);
+        const double yy = -(t.y0 + xy3 + xy4 + t.c2);
+        const double zz = -(t.za + zb1 + zb2 + t.zc);
+        const double adv = xx + yy + zz;
+        if (ax) ax[n * n3 + c] = xx;
+        if (ay) ay[n * n3 + c] = yy;
+        if (az) az[n * n3 + c] = zz;
+
+        double tf = adv * vitc;
+        // horizontal mixing
+        {
+            double h, cc;
+            if (p.hmix == LICOM_HMIX_DEL4) {
+                const double *X = d2tk + n * n3 + (size_t)k * n2;
+                del2_sums(g, X, k, j, i, h, cc);
+                h = p.ah * (cc * X[q] + h);
+            } else {
+                const double *X = atb + n * (n3 + n2) + (size_t)(k + 1) * n2;
+                del2_sums(g, X, k, j, i, h, cc);
+                h = p.ah * (cc * X[q] + h);
+            }
+            tf += h;
+        }
+        // shortwave penetration (T only) and the explicit half of vertical
+        // diffusion; level 0 and km-1 receive their boundary terms exactly as
+        // the reference's k==0 thread applies them (tracer.cpp:1421-1475).
+        const double *B = atb + n * (n3 + n2) + n2;   // level k at B[k*n2]
+        const double *A = akt + n * n3;
+        if (n == 0) {
+            const double sw = swv ? swv[q] : 0.0;
+            double pv;
+            if (k == 0) {
+                const double wt1 = sw * g.pen[0] * vitf(g, 1, j, i);
+                tf = tf - odzp[0] * wt1;
+                pv = -wt1 * odzp[0];
+            } else if (k == km - 1) {
+                const double wt2 = sw * g.pen[km - 2] * vitf(g, km - 1, j, i);
+                tf = tf + odzp[km - 1] * wt2;
+                pv = wt2 * odzp[km - 1];
+            } else {
+                const double wt1 = sw * g.pen[k - 1] * vitc;
+                const double wt2 = sw * g.pen[k] * vitf(g, k + 1, j, i);
+                tf = tf + (wt1 - wt2) * odzp[k];
+                pv = (wt1 - wt2) * odzp[k];
+            }
+            if (pen_out) pen_out[c] = pv;
+        }
+        double dzv;
+        if (k == 0) {
+            const double wt1 = akt_floored(p, A, n, 0, q) * (B[q] - B[n2 + q]) * odzt[1] *
+                               vitf(g, 1, j, i);
+            tf = tf - odzp[0] * wt1 * 0.5;
+            dzv = -odzp[0] * wt1 * 0.5;
+        } else if (k == km - 1) {
+            const double wt2 = akt_floored(p, A, n, km - 2, (size_t)(km - 2) * n2 + q) *
+                               (B[(size_t)(km - 2) * n2 + q] - B[(size_t)(km - 1) * n2 + q]) *
+                               odzt[km - 1] * vitf(g, km - 1, j, i);
+            tf = tf + odzp[km - 1] * wt2 * (1.0 - aidif);
+            dzv = odzp[km - 1] * wt2 * 0.5;
+        } else {
+            const double wt1 = akt_floored(p, A, n, k - 1, c - n2) * (B[c - n2] - B[c]) *
+                               odzt[k] * vitc;
+            const double wt2 = akt_floored(p, A, n, k, c) * (B[c] - B[c + n2]) * odzt[k + 1] *
+                               vitf(g, k + 1, j, i);
+            tf = tf + odzp[k] * (wt1 - wt2) * 0.5;
+            dzv = odzp[k] * (wt1 - wt2) * 0.5;
+        }
+        if (dzd) dzd[n * n3 + c] = dzv;
+        if (k == 0) {
+            if (n == 1) {
+                if (g.kmt[q] > 0) tf = tf + stf_n[n2 + q] * (1.0 - aidif) * odzp[0];
+                if (p.sssnorm) tf = tf + err_norm2[0] * vitc;
+            } else if (g.kmt[q] > 0) {
+                tf = tf + stf_n[q] * odzp[0] * 0.5;
+            }
+        }
+        if (p.boundary_restore == 1 && restore && k >= 1) {
+            const double lamda = 1.0 / (15.0 * 86400.0);
+            tf = tf + vitc * (restore[n * n3 + c] - B[c]) * lamda;
+        }
+        ori[n * n3 + c] = T[c] + p.dts * tf;
+    }
+}
+
+// --------------------------------------------------------------- columns
+// invtrit (tracer.cpp:923-978) on one column held in x[] (in: explicit
+// update, out: solution for k < kz).
+__device__ void thomas(const KGrid &g, const KPar &p, const double *A, int n, size_t q,
+                       size_t n2, int kz, double stf, double *x, double *e, double *f) {
+    const double *odzt = g.odzt, *odzp = g.odzp;
+    const double dts = p.dts;
+    // bottom row (kz-1) and the top row, as the reference sets them last
+    auto wkc = [&](int k) { return akt_floored(p, A, n, k, (size_t)k * n2 + q); };
+    e[kz] = 0.0;
+    f[kz] = 0.0;
+    for (int k = kz - 1; k >= 0; --k) {
+        double a, b, c;
+        if (k == kz - 1) {
+            a = (k >= 1) ? wkc(k - 1) * odzt[k] * odzp[k] * dts * 0.5 : odzp[0] * dts * 0.5;
+            b = 1.0 + a;
+            c = odzp[kz - 1] * dts * 0.5;
+        } else if (k == 0) {
+            a = odzp[0] * dts * 0.5;
+            c = wkc(0) * odzt[1] * odzp[0] * dts * 0.5;
+            b = 1.0 + c;
+        } else {
+            a = wkc(k - 1) * odzt[k] * odzp[k] * dts * 0.5;
+            c = wkc(k) * odzt[k + 1] * odzp[k] * dts * 0.5;
+            b = 1.0 + a + c;
+        }
+        const double g0 = 1.0 / (b - c * e[k + 1]);
+        e[k] = a * g0;
+        f[k] = (x[k] + c * f[k + 1]) * g0;
+    }
+    x[0] = (e[0] * stf + f[0]) * 1.0;
+    for (int k = 1; k < kz; ++k) x[k] = (e[k] * x[k - 1] + f[k]) * 1.0;
+}
+
+__device__ __forceinline__ double dens(const double *C, int km, int k, double t, double s) {
+    // dens.F90:12-14, C(k,m) at C[(m-1)*km+k]
+    return (C[k] + (C[3 * km + k] + C[6 * km + k] * s) * s +
+            (C[2 * km + k] + C[7 * km + k] * s + C[5 * km + k] * t) * t) * t +
+           (C[1 * km + k] + (C[4 * km + k] + C[8 * km + k] * s) * s) * s;
+}
+
+// convadj_cu mixing (convadj.cpp:52-256) on columns T[], S[] (length km).
+__device__ void convect(const KGrid &g, int kcon, double *T, double *S, double *rhoup,
+                        double *rholo) {
+    const int km = g.km;
+    const double *C = g.eosc, *to = g.to, *so = g.so, *dzp = g.dzp;
+    if (kcon == 0) return;
+    for (int l = 0; l < km; ++l) rhoup[l] = rholo[l] = 0.0;
+    for (int l = 0; l < km - 1; ++l) {
+        const int l1 = l + 1;
+        rhoup[l1] = dens(C, km, l1, T[l1] - to[l1], S[l1] - so[l1]);
+        rholo[l] = dens(C, km, l1, T[l] - to[l1], S[l] - so[l1]);
+    }
+    int lcon = 0;
+    for (int k = kcon - 1; k >= 1; --k)
+        if (rholo[k - 1] > rhoup[k]) lcon = k;
+    if (lcon == 0) return;
+    double *X[2] = {T, S};
+    for (;;) {   // CONV_1
+        int lcona = lcon, lconb = lcon + 1;
+        double dztsum = dzp[lcona - 1] + dzp[lconb - 1];
+        double trasum[2];
+        for (int n = 0; n < 2; ++n) {
+            trasum[n] = X[n][lcona - 1] * dzp[lcona - 1] + X[n][lconb - 1] * dzp[lconb - 1];
+            const double tramix = trasum[n] / dztsum;
+            X[n][lcona - 1] = tramix;
+            X[n][lconb - 1] = tramix;
+        }
+        for (;;) {   // CONV_2
+            if (lconb != kcon) {
+                const int l1 = lconb + 1;
+                rholo[lconb - 1] = dens(C, km, l1 - 1, T[lconb - 1] - to[l1 - 1], S[lconb - 1] - so[l1 - 1]);
+                if (rholo[lconb - 1] > rhoup[l1 - 1]) {
+                    lconb = lconb + 1;
+                    dztsum = dztsum + dzp[lconb - 1];
+                    for (int n = 0; n < 2; ++n) {
+                        trasum[n] = trasum[n] + X[n][lconb - 1] * dzp[lconb - 1];
+                        const double tramix = trasum[n] / dztsum;
+                        for (int lm = lcona - 1; lm < lconb; ++lm) X[n][lm] = tramix;
+                    }
+                    continue;
+                }
+            }
+            if (lcona > 1) {
+                const int l1 = lcona - 1;
+                rholo[l1 - 1] = dens(C, km, lcona - 1, T[l1 - 1] - to[lcona - 1], S[l1 - 1] - so[lcona - 1]);
+                rhoup[lcona - 1] = dens(C, km, lcona - 1, T[lcona - 1] - to[lcona - 1],
+                                        S[lcona - 1] - so[lcona - 1]);
+                if (rholo[lcona - 2] > rhoup[lcona - 1]) {
+                    lcona = lcona - 1;
+                    dztsum = dztsum + dzp[lcona - 1];
+                    for (int n = 0; n < 2; ++n) {
+                        trasum[n] = trasum[n] + X[n][lcona - 1] * dzp[lcona - 1];
+                        const double tramix = trasum[n] / dztsum;
+                        for (int lm = lcona; lm <= lconb; ++lm) X[n][lm - 1] = tramix;
+                    }
+                    continue;
+                }
+            }
+            break;
+        }
+        if (lconb == kcon) return;
+        lcon = lconb;
+        for (;;) {   // CONV_3
+            lcon = lcon + 1;
+            if (lcon == kcon) return;
+            if (rholo[lcon - 1] <= rhoup[lcon]) continue;
+            break;
+        }
+    }
+}
+
+// invtrit + halo-independent part of tracer_9 (tracer.cpp:1655-1699) +
+// convadj (convadj.cpp:258-275) on the physical columns, plus the reference's
+// in-place side effects on every column.
+__global__ void k_column(KGrid g, KPar p, int flags, double *at, double *atb, double *akt,
+                         const double *__restrict__ ori, const double *__restrict__ stf_n,
+                         double *utf, double *vtf, double *h0f, double *tend, double *dt_diff,
+                         double *dt_conv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.imt || j >= g.jmt) return;
+    const int km = g.km;
+    const size_t n2 = (size_t)g.imt * g.jmt, n3 = n2 * km, q = i2(g, j, i);
+    if (flags & LICOM_STEP_SIDE_EFFECTS) {
+        for (int k = 0; k < km; ++k) {
+            utf[(size_t)k * n2 + q] *= p.oncc;
+            vtf[(size_t)k * n2 + q] *= p.oncc;
+        }
+        h0f[q] = h0f[q] * p.onbc;
+        if (i >= 1 && i < g.imt - 1 && j >= 1 && j < g.jmt - 1) {
+            for (int n = 0; n < 2; ++n)
+                if ((n == 0 || p.hmix == LICOM_HMIX_DEL2) && akt[n * n3 + q] < p.dwndmix)
+                    akt[n * n3 + q] = p.dwndmix;
+        }
+    }
+    if (i < 2 || i >= g.imt - 2 || j < 2 || j >= g.jmt - 2) return;
+    const int kz = g.kmt[q];
+    double x[2][KMAX], e[KMAX + 1], f[KMAX + 1];
+    for (int n = 0; n < 2; ++n) {
+        const double *O = ori + n * n3;
+        for (int k = 0; k < km; ++k) x[n][k] = O[(size_t)k * n2 + q];
+        const double stf = stf_n[n * n2 + q];
+        if (kz > 0) {
+            // wkc is akt with the floor applied to level 0 (tracer.cpp:1315/1187)
+            thomas(g, p, akt + n * n3, n, q, n2, kz, stf, x[n], e, f);
+        }
+        const double *T = at + n * n3;
+        for (int k = 0; k < km; ++k) {
+            const size_t c = (size_t)k * n2 + q;
+            const double vk = k < kz ? 1.0 : 0.0;
+            if (dt_diff) {
+                double v = (x[n][k] - O[c]) / p.dts * vk;
+                if (k == 0) v = v - stf * 0.5 * g.odzp[0];
+                dt_diff[n * n3 + c] = v;
+            }
+            if (tend) tend[n * n3 + c] = (x[n][k] - T[c]) / p.dts * vk;
+        }
+    }
+    if (flags & LICOM_STEP_CONVADJ) {
+        // x[] holds at after tracer_9 (atb == at); convadj mixes a copy
+        double y[2][KMAX];
+        for (int n = 0; n < 2; ++n)
+            for (int k = 0; k < km; ++k) y[n][k] = x[n][k];
+        convect(g, kz, y[0], y[1], e, f);
+        for (int n = 0; n < 2; ++n)
+            for (int k = 0; k < km; ++k) {
+                const size_t c = (size_t)k * n2 + q;
+                const double vk = k < kz ? 1.0 : 0.0;
+                const double dc = (y[n][k] - x[n][k]) / p.dts * vk;
+                if (dt_conv) dt_conv[n * n3 + c] = dc;
+                if (tend) tend[n * n3 + c] += dc;
+                x[n][k] = y[n][k];
+            }
+    }
+    for (int n = 0; n < 2; ++n)
+        for (int k = 0; k < km; ++k) {
+            const size_t c = (size_t)k * n2 + q;
+            at[n * n3 + c] = x[n][k];
+            atb[n * (n3 + n2) + n2 + c] = x[n][k];
+        }
+}
+
+// Ghost cells of at and atb (levels 1..km of the 0:km array): cyclic in i,
+// zero in the closed rows (POP_HaloUpdate with fillValue 0, as
+// pop_haloupdate_tracer2.F90:10 followed by tracer_9's copy).
+__global__ void k_halo(int imt, int jmt, int nplanes, double *at, double *atb, int km) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int plane = blockIdx.y;   // 0..2*km-1 at, 2*km..4*km-1 atb
+    const int ncol = 4 * (jmt - 4), nrow = 4 * imt;
+    if (t >= ncol + nrow || plane >= nplanes) return;
+    const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
+    double *P;
+    if (plane < 2 * km) {
+        P = at + (size_t)plane * n2;
+    } else {
+        const int pl = plane - 2 * km, n = pl / km, k = pl % km;
+        P = atb + n * (n3 + n2) + n2 + (size_t)k * n2;
+    }
+    if (t < ncol) {
+        const int j = 2 + t / 4, w = t % 4;
+        const int dst = (w == 0) ? 0 : (w == 1) ? 1 : (w == 2) ? imt - 2 : imt - 1;
+        const int src = (w == 0) ? imt - 4 : (w == 1) ? imt - 3 : (w == 2) ? 2 : 3;
+        P[(size_t)j * imt + dst] = P[(size_t)j * imt + src];
+    } else {
+        const int r = t - ncol, w = r / imt, i = r % imt;
+        const int j = (w == 0) ? 0 : (w == 1) ? 1 : (w == 2) ? jmt - 2 : jmt - 1;
+        P[(size_t)j * imt + i] = 0.0;
+    }
+}
+
+// convadj_cu over every column, ghost columns included (convadj.cpp:29-288).
+__global__ void k_convadj(KGrid g, KPar p, double *at, double *atb, double *tend,
+                          double *dt_conv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.imt || j >= g.jmt) return;
+    const int km = g.km;
+    const size_t n2 = (size_t)g.imt * g.jmt, n3 = n2 * km, q = i2(g, j, i);
+    const int kz = g.kmt[q];
+    double y[2][KMAX], ru[KMAX], rl[KMAX];
+    for (int n = 0; n < 2; ++n)
+        for (int k = 0; k < km; ++k) y[n][k] = at[n * n3 + (size_t)k * n2 + q];
+    convect(g, kz, y[0], y[1], ru, rl);
+    const bool inner = i >= 2 && i < g.imt - 2 && j >= 2 && j < g.jmt - 2;
+    for (int n = 0; n < 2; ++n)
+        for (int k = 0; k < km; ++k) {
+            const size_t c = (size_t)k * n2 + q;
+            double *B = atb + n * (n3 + n2) + n2;
+            const double dc = (y[n][k] - B[c]) / p.dts * (k < kz ? 1.0 : 0.0);
+            if (inner) {
+                if (dt_conv) dt_conv[n * n3 + c] = dc;
+                if (tend) tend[n * n3 + c] += dc;
+            }
+            at[n * n3 + c] = y[n][k];
+            B[c] = y[n][k];
+        }
+}
+
+__global__ void k_copy_plane(const double *src, double *dst, size_t n) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (size_t)gridDim.x * blockDim.x)
+        dst[t] = src[t];
+}
+
+// ------------------------------------------------------------ launch glue
+struct Timer {
+    licom_ctx *ctx;
+    const char *name;
+    hipEvent_t a{}, b{};
+    Timer(licom_ctx *c, const char *n) : ctx(c), name(n) {
+        if (ctx->profiling) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, ctx->stream);
+        }
+    }
+    ~Timer() {
+        if (ctx->profiling) {
+            (void)hipEventRecord(b, ctx->stream);
+            ctx->pending.push_back({name, a, b});
+        }
+    }
+};
+
+inline dim3 grid2(int imt, int jmt, dim3 b) {
+    return dim3((imt + b.x - 1) / b.x, (jmt + b.y - 1) / b.y, 1);
+}
+
+}  // namespace
+
+int launch_step(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags) {
+    const KGrid &g = ctx->g;
+    const KPar &p = ctx->kp;
+    Scratch &s = ctx->s;
+    hipStream_t st = ctx->stream;
+    const dim3 b2(64, 4), b3(64, 4, 1);
+    const dim3 g2 = grid2(g.imt, g.jmt, b2);
+    const dim3 g3(g2.x, g2.y, g.km);
+    const dim3 g3i((g.imt - 4 + 63) / 64, (g.jmt - 4 + 3) / 4, g.km);
+
+    {
+        Timer t(ctx, "k_surface");
+        k_surface<<<g2, b2, 0, st>>>(g, p, f->h0f, f->h0l, f->atb, f->swv, f->nswv, f->sst,
+                                     f->seaice, f->sss, f->fresh, s.stf_pre, s.work_u, s.stf_n,
+                                     s.temp11, d ? d->net : nullptr);
+    }
+    {
+        Timer t(ctx, "k_sss");
+        k_sss_partial<<<s.nblk, kRedThreads, 0, st>>>(g, s.temp11, s.partials);
+        k_sss_final<<<1, kRedThreads, 0, st>>>(s.partials, s.nblk, ctx->area_t, p.sssnorm,
+                                               s.err_norm2, d ? d->fw_norm2 : nullptr);
+    }
+    {
+        Timer t(ctx, "k_faces");
+        k_faces<<<g3, b3, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl, s.uw, s.vs);
+    }
+    {
+        Timer t(ctx, "k_ws");
+        k_ws<<<g2, b2, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl, s.work_u, s.stf_pre, f->ws);
+    }
+    {
+        Timer t(ctx, "k_predict");
+        k_predict<<<g3, b3, 0, st>>>(g, p, f->at, s.uw, s.vs, f->ws, s.flags);
+    }
+    if (p.hmix == LICOM_HMIX_DEL4) {
+        Timer t(ctx, "k_d2tk");
+        k_d2tk<<<g3, b3, 0, st>>>(g, f->atb, s.d2tk);
+    }
+    {
+        Timer t(ctx, "k_correct");
+        k_correct<<<g3i, b3, 0, st>>>(g, p, f->at, f->atb, f->akt, s.uw, s.vs, f->ws, s.flags,
+                                      s.d2tk, s.stf_n, s.err_norm2, f->swv, f->restore, s.ori,
+                                      d ? d->ax : nullptr, d ? d->ay : nullptr,
+                                      d ? d->az : nullptr, d ? d->dz : nullptr,
+                                      d ? d->penetrate : nullptr);
+    }
+    {
+        Timer t(ctx, "k_column");
+        k_column<<<g2, b2, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n, f->utf,
+                                    f->vtf, f->h0f, d ? d->tend : nullptr,
+                                    d ? d->dt_diff : nullptr, d ? d->dt_conv : nullptr);
+    }
+    {
+        Timer t(ctx, "k_halo");
+        const int cells = 4 * (g.jmt - 4) + 4 * g.imt;
+        k_halo<<<dim3((cells + 255) / 256, 4 * g.km), 256, 0, st>>>(g.imt, g.jmt, 4 * g.km, f->at,
+                                                                     f->atb, g.km);
+    }
+    if (flags & LICOM_STEP_VTL_SCRATCH) {
+        Timer t(ctx, "k_copy_vtl");
+        k_copy_plane<<<1024, 256, 0, st>>>(f->at + ctx->n3, f->vtl, ctx->n3);
+    }
+    return hip_status(hipGetLastError(), "tracer step launch");
+}
+
+int launch_convadj(licom_ctx *ctx, const licom_fields *f, const licom_diag *d) {
+    const dim3 b2(64, 4);
+    Timer t(ctx, "k_convadj");
+    k_convadj<<<grid2(ctx->imt, ctx->jmt, b2), b2, 0, ctx->stream>>>(
+        ctx->g, ctx->kp, f->at, f->atb, d ? d->tend : nullptr, d ? d->dt_conv : nullptr);
+    return hip_status(hipGetLastError(), "convadj launch");
+}
+
+}  // namespace licom

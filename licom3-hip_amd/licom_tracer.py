@@ -1,0 +1,304 @@
+"""Host-side mirror of the reference's tracer interface over the C ABI.
+
+``liblicom_tracer.so`` (include/licom_tracer.h) is the product: HIP kernels
+for gfx950 behind a plain C ABI.  This module binds it with ctypes and keeps
+the model state resident on the device (torch tensors are used only as
+device allocations and for stream handles).
+
+:class:`TracerModel` mirrors the reference's calling convention:
+
+* ``tracer(ii)``  -> ``extern "C" void tracer_(int ii)``  (tracer.cpp:1701)
+* ``convadj()``   -> ``extern "C" void convadj_()``        (convadj.cpp:290)
+* ``stepon(n)``   -> the tracer/convadj part of ``stepon_`` (stepon.cpp:57-77)
+
+There is no CPU fallback: if the library or a GPU is missing every entry
+point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int32)
+_VP = ctypes.c_void_p
+
+LICOM_STEP_CONVADJ = 0x1
+LICOM_STEP_SIDE_EFFECTS = 0x2
+LICOM_STEP_VTL_SCRATCH = 0x4
+HMIX_DEL2, HMIX_DEL4 = 2, 4
+ADV_TSPAS = 2
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("km", ctypes.c_int),
+                ("imt_global", ctypes.c_int),
+                ("kmt", _IP), ("zkp", _DP), ("pen", _DP),
+                ("htw", _DP), ("hts", _DP), ("hun", _DP), ("hue", _DP), ("tarea", _DP),
+                ("au0", _DP), ("aus", _DP), ("auw", _DP), ("ausw", _DP),
+                ("to", _DP), ("so", _DP), ("eosc", _DP)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("dts", ctypes.c_double), ("onbc", ctypes.c_double), ("oncc", ctypes.c_double),
+                ("dwndmix", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("od0", ctypes.c_double), ("od0cp", ctypes.c_double), ("ah", ctypes.c_double),
+                ("hmix", ctypes.c_int), ("adv", ctypes.c_int),
+                ("boundary_restore", ctypes.c_int), ("sssnorm", ctypes.c_int)]
+
+
+class Fields(ctypes.Structure):
+    _fields_ = [("at", _VP), ("atb", _VP), ("akt", _VP), ("utf", _VP), ("vtf", _VP),
+                ("utl", _VP), ("vtl", _VP), ("ws", _VP), ("h0f", _VP), ("h0l", _VP),
+                ("swv", _VP), ("nswv", _VP), ("sst", _VP), ("seaice", _VP), ("sss", _VP),
+                ("fresh", _VP), ("restore", _VP)]
+
+
+class Diag(ctypes.Structure):
+    _fields_ = [("ax", _VP), ("ay", _VP), ("az", _VP), ("dz", _VP), ("tend", _VP),
+                ("dt_diff", _VP), ("dt_conv", _VP), ("penetrate", _VP), ("net", _VP),
+                ("fw_norm2", _VP)]
+
+
+class GridDerived(ctypes.Structure):
+    _fields_ = [("kmt", _IP), ("kmu", _IP), ("tarea_r", _DP), ("ohbt", _DP), ("ohbu", _DP),
+                ("dtn", _DP), ("dts", _DP), ("dte", _DP), ("dtw", _DP), ("ahf", _DP),
+                ("dzp", _DP), ("odzp", _DP), ("odzt", _DP), ("area_t", ctypes.c_double)]
+
+
+EXPORTED = ["licom_tracer_create", "licom_tracer_set_stream", "licom_tracer_step",
+            "licom_convadj", "licom_tracer_sync", "licom_grid_export", "licom_set_profiling",
+            "licom_kernel_times", "licom_scratch_bytes", "licom_tracer_destroy",
+            "licom_strerror", "licom_last_error"]
+
+
+class LicomError(RuntimeError):
+    pass
+
+
+def lib_path():
+    return os.path.join(_HERE, "liblicom_tracer.so")
+
+
+def lib():
+    """Load liblicom_tracer.so (raises if it was not built)."""
+    global _LIB
+    if _LIB is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise LicomError(f"{path} is missing: run __graft_entry__.build()")
+        L = ctypes.CDLL(path)
+        L.licom_tracer_create.argtypes = [ctypes.POINTER(Grid), ctypes.POINTER(Params),
+                                          ctypes.POINTER(_VP)]
+        L.licom_tracer_set_stream.argtypes = [_VP, _VP]
+        L.licom_tracer_step.argtypes = [_VP, ctypes.POINTER(Fields), ctypes.POINTER(Diag),
+                                        ctypes.c_int]
+        L.licom_convadj.argtypes = [_VP, ctypes.POINTER(Fields), ctypes.POINTER(Diag)]
+        L.licom_tracer_sync.argtypes = [_VP]
+        L.licom_grid_export.argtypes = [_VP, ctypes.POINTER(GridDerived)]
+        L.licom_set_profiling.argtypes = [_VP, ctypes.c_int]
+        L.licom_kernel_times.argtypes = [_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                         _DP, _IP, _IP]
+        L.licom_scratch_bytes.argtypes = [_VP]
+        L.licom_scratch_bytes.restype = ctypes.c_size_t
+        L.licom_tracer_destroy.argtypes = [_VP]
+        L.licom_strerror.argtypes = [ctypes.c_int]
+        L.licom_strerror.restype = ctypes.c_char_p
+        L.licom_last_error.restype = ctypes.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def check(rc, what):
+    if rc != 0:
+        L = lib()
+        raise LicomError(f"{what}: {L.licom_strerror(rc).decode()} ({rc}): "
+                         f"{L.licom_last_error().decode()}")
+
+
+def _d(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_DP)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def make_grid(case, keep):
+    g = Grid()
+    g.nx, g.ny, g.km, g.imt_global = case.nx, case.ny, case.km, case.imt_global
+    cg = case.grid
+    keep.append(cg)
+    g.kmt = cg["kmt"].ctypes.data_as(_IP)
+    for f in ("zkp", "pen", "htw", "hts", "hun", "hue", "tarea", "au0", "aus", "auw", "ausw",
+              "to", "so", "eosc"):
+        setattr(g, f, _d(cg[f]))
+    return g
+
+
+def make_params(p):
+    P = Params()
+    for f in ("dts", "onbc", "oncc", "dwndmix", "gamma", "od0", "od0cp", "ah"):
+        setattr(P, f, float(p[f]))
+    P.hmix = int(p["hmix"])
+    P.adv = ADV_TSPAS
+    P.boundary_restore = int(p.get("boundary_restore", 0))
+    P.sssnorm = int(p.get("sssnorm", 1))
+    return P
+
+
+STATE_KEYS = ("at", "atb", "akt", "utf", "vtf", "utl", "vtl", "ws", "h0f", "h0l")
+FORCING_KEYS = ("swv", "nswv", "sst", "seaice", "sss", "fresh")
+DIAG_KEYS = ("ax", "ay", "az", "dz", "tend", "dt_diff", "dt_conv", "penetrate", "net",
+             "fw_norm2")
+
+
+class TracerModel:
+    """Device-resident tracer state plus the reference's entry points.
+
+    ``state`` holds torch float64 tensors on the GPU with the reference
+    shapes; ``diag`` holds the diagnostics the reference produces.
+    """
+
+    def __init__(self, case, device="cuda:0", diagnostics=True, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise LicomError("TracerModel needs a GPU (torch.cuda.is_available() is False)")
+        self.torch = torch
+        self.case = case
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        L = lib()
+        self._keep = []
+        self._grid = make_grid(case, self._keep)
+        self._params = make_params(case.params)
+        h = _VP()
+        check(L.licom_tracer_create(ctypes.byref(self._grid), ctypes.byref(self._params),
+                                    ctypes.byref(h)), "licom_tracer_create")
+        self.ctx = h
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        check(L.licom_tracer_set_stream(self.ctx, _VP(self.stream.cuda_stream)), "set_stream")
+        self.state = {k: torch.from_numpy(np.ascontiguousarray(v)).to(self.device)
+                      for k, v in case.state.items()}
+        self.forcing = {k: (None if case.forcing.get(k) is None else
+                            torch.from_numpy(np.ascontiguousarray(case.forcing[k])).to(self.device))
+                        for k in FORCING_KEYS}
+        r = case.forcing.get("restore")
+        self.restore = None if r is None else torch.from_numpy(r).to(self.device)
+        km, jmt, imt = case.km, case.jmt, case.imt
+        self.diag = None
+        if diagnostics:
+            z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.device)  # noqa: E731
+            self.diag = dict(ax=z(2, km, jmt, imt), ay=z(2, km, jmt, imt), az=z(2, km, jmt, imt),
+                             dz=z(2, km, jmt, imt), tend=z(2, km, jmt, imt),
+                             dt_diff=z(2, km, jmt, imt), dt_conv=z(2, km, jmt, imt),
+                             penetrate=z(km, jmt, imt), net=z(2, jmt, imt), fw_norm2=z(1))
+        self.ist = 0
+
+    # -- C structs -------------------------------------------------------
+    def _fields(self):
+        F = Fields()
+        for k in STATE_KEYS:
+            setattr(F, k, _ptr(self.state[k]))
+        for k in FORCING_KEYS:
+            setattr(F, k, _ptr(self.forcing[k]))
+        F.restore = _ptr(self.restore)
+        return F
+
+    def _diag(self):
+        D = Diag()
+        if self.diag is not None:
+            for k in DIAG_KEYS:
+                setattr(D, k, _ptr(self.diag[k]))
+        return D
+
+    # -- reference entry points -----------------------------------------
+    def tracer(self, ii=0, flags=LICOM_STEP_SIDE_EFFECTS):
+        """tracer_(ii): one tracer step of T and S (tracer.cpp:1701-1903)."""
+        F, D = self._fields(), self._diag()
+        check(lib().licom_tracer_step(self.ctx, ctypes.byref(F), ctypes.byref(D), int(flags)),
+              "licom_tracer_step")
+        self.ist += 1
+
+    def convadj(self):
+        """convadj_() over every column (convadj.cpp:290-298)."""
+        F, D = self._fields(), self._diag()
+        check(lib().licom_convadj(self.ctx, ctypes.byref(F), ctypes.byref(D)), "licom_convadj")
+
+    def step(self, flags=LICOM_STEP_SIDE_EFFECTS | LICOM_STEP_CONVADJ):
+        """tracer_ followed by convadj_ in one launch sequence (the CPU
+        driver's per-step order, licom_drv.F90:357-380)."""
+        self.tracer(0, flags)
+
+    def stepon(self, nss, convadj_every_step=True):
+        """Tracer part of stepon_ (stepon.cpp:57-77): convadj every step, or
+        only at ii == 0 as the GPU driver does."""
+        for ii in range(nss):
+            if convadj_every_step:
+                self.tracer(ii, LICOM_STEP_SIDE_EFFECTS | LICOM_STEP_CONVADJ)
+            else:
+                self.tracer(ii, LICOM_STEP_SIDE_EFFECTS)
+                if ii == 0:
+                    self.convadj()
+
+    def sync(self):
+        check(lib().licom_tracer_sync(self.ctx), "licom_tracer_sync")
+
+    def derived(self):
+        c = self.case
+        out = dict(kmt=np.zeros((c.jmt, c.imt), np.int32), kmu=np.zeros((c.jmt, c.imt), np.int32))
+        for k in ("tarea_r", "ohbt", "ohbu", "dtn", "dts", "dte", "dtw", "ahf"):
+            out[k] = np.zeros((c.jmt, c.imt))
+        for k in ("dzp", "odzp", "odzt"):
+            out[k] = np.zeros(c.km)
+        G = GridDerived()
+        G.kmt = out["kmt"].ctypes.data_as(_IP)
+        G.kmu = out["kmu"].ctypes.data_as(_IP)
+        for k in ("tarea_r", "ohbt", "ohbu", "dtn", "dts", "dte", "dtw", "ahf", "dzp", "odzp", "odzt"):
+            setattr(G, k, _d(out[k]))
+        check(lib().licom_grid_export(self.ctx, ctypes.byref(G)), "licom_grid_export")
+        out["area_t"] = G.area_t
+        return out
+
+    def set_profiling(self, on=True):
+        check(lib().licom_set_profiling(self.ctx, int(on)), "licom_set_profiling")
+
+    def kernel_times(self):
+        n = ctypes.c_int(0)
+        names = (ctypes.c_char_p * 64)()
+        ms = (ctypes.c_double * 64)()
+        cnt = (ctypes.c_int32 * 64)()
+        check(lib().licom_kernel_times(self.ctx, 64, names, ms, cnt, ctypes.byref(n)),
+              "licom_kernel_times")
+        return {names[k].decode(): (ms[k], cnt[k]) for k in range(n.value)}
+
+    def scratch_bytes(self):
+        return int(lib().licom_scratch_bytes(self.ctx))
+
+    def host_state(self):
+        self.sync()
+        return {k: v.cpu().numpy() for k, v in self.state.items()}
+
+    def host_diag(self):
+        self.sync()
+        return {k: v.cpu().numpy() for k, v in self.diag.items()}
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.sync()
+            lib().licom_tracer_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,191 @@
+"""ctypes binding of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  See oracle.h for the reference
+statements it restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_DP = ctypes.POINTER(ctypes.c_double)
+_IP = ctypes.POINTER(ctypes.c_int32)
+
+
+class OrcGrid(ctypes.Structure):
+    _fields_ = [("imt", ctypes.c_int), ("jmt", ctypes.c_int), ("km", ctypes.c_int),
+                ("imt_global", ctypes.c_int), ("hmix", ctypes.c_int),
+                ("kmt", _IP), ("zkp", _DP), ("pen", _DP),
+                ("htw", _DP), ("hts", _DP), ("hun", _DP), ("hue", _DP), ("tarea", _DP),
+                ("au0", _DP), ("aus", _DP), ("auw", _DP), ("ausw", _DP),
+                ("to", _DP), ("so", _DP), ("eosc", _DP),
+                ("kmu", _IP), ("vit", _DP), ("viv", _DP),
+                ("tarea_r", _DP), ("ohbt", _DP), ("ohbu", _DP),
+                ("dtn", _DP), ("dts", _DP), ("dte", _DP), ("dtw", _DP), ("ahf", _DP),
+                ("dzp", _DP), ("odzp", _DP), ("odzt", _DP),
+                ("area_t", ctypes.c_double)]
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [("dts", ctypes.c_double), ("onbc", ctypes.c_double), ("oncc", ctypes.c_double),
+                ("dwndmix", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("od0", ctypes.c_double), ("od0cp", ctypes.c_double), ("ah", ctypes.c_double),
+                ("boundary_restore", ctypes.c_int), ("sssnorm", ctypes.c_int),
+                ("convadj", ctypes.c_int)]
+
+
+class OrcFields(ctypes.Structure):
+    _fields_ = [("at", _DP), ("atb", _DP), ("akt", _DP),
+                ("utf", _DP), ("vtf", _DP), ("utl", _DP), ("vtl", _DP), ("ws", _DP),
+                ("h0f", _DP), ("h0l", _DP),
+                ("swv", _DP), ("nswv", _DP), ("sst", _DP), ("seaice", _DP), ("sss", _DP),
+                ("fresh", _DP), ("restore", _DP)]
+
+
+class OrcDiag(ctypes.Structure):
+    _fields_ = [("ax", _DP), ("ay", _DP), ("az", _DP), ("dz", _DP),
+                ("tend", _DP), ("dt_diff", _DP), ("dt_conv", _DP),
+                ("penetrate", _DP), ("net", _DP), ("fw_norm2", ctypes.c_double)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run `make -C oracle`")
+        _LIB = ctypes.CDLL(path)
+        _LIB.orc_derive.argtypes = [ctypes.POINTER(OrcGrid)]
+        _LIB.orc_tracer_step.argtypes = [ctypes.POINTER(OrcGrid), ctypes.POINTER(OrcParams),
+                                         ctypes.POINTER(OrcFields), ctypes.POINTER(OrcDiag)]
+        _LIB.orc_convadj.argtypes = _LIB.orc_tracer_step.argtypes
+        _LIB.orc_invtri_column.argtypes = [ctypes.c_int, _DP, _DP, _DP, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, _DP]
+        _LIB.orc_dens.argtypes = [_DP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        _LIB.orc_dens.restype = ctypes.c_double
+        _LIB.orc_halo_fill_f64.argtypes = [_DP, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        _LIB.orc_nthreads.restype = ctypes.c_int
+    return _LIB
+
+
+def _d(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous, a.dtype
+    return a.ctypes.data_as(_DP)
+
+
+def _i(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_IP)
+
+
+class Oracle:
+    """Derived grid + step driver over numpy arrays of one Case."""
+
+    def __init__(self, case):
+        self.case = case
+        imt, jmt, km = case.imt, case.jmt, case.km
+        g = case.grid
+        self.kmt = g["kmt"].copy()
+        z2 = lambda: np.zeros((jmt, imt))  # noqa: E731
+        self.d = dict(kmu=np.zeros((jmt, imt), np.int32), vit=np.zeros((km, jmt, imt)),
+                      viv=np.zeros((km, jmt, imt)), tarea_r=z2(), ohbt=z2(), ohbu=z2(),
+                      dtn=z2(), dts=z2(), dte=z2(), dtw=z2(), ahf=z2(),
+                      dzp=np.zeros(km), odzp=np.zeros(km), odzt=np.zeros(km))
+        gr = OrcGrid()
+        gr.imt, gr.jmt, gr.km = imt, jmt, km
+        gr.imt_global = case.imt_global
+        gr.hmix = int(case.params["hmix"])
+        gr.kmt = _i(self.kmt)
+        for f in ("zkp", "pen", "htw", "hts", "hun", "hue", "tarea", "au0", "aus", "auw", "ausw",
+                  "to", "so", "eosc"):
+            setattr(gr, f, _d(g[f]))
+        gr.kmu = _i(self.d["kmu"])
+        for f in ("vit", "viv", "tarea_r", "ohbt", "ohbu", "dtn", "dts", "dte", "dtw", "ahf",
+                  "dzp", "odzp", "odzt"):
+            setattr(gr, f, _d(self.d[f]))
+        self.g = gr
+        lib().orc_derive(ctypes.byref(gr))
+        self.area_t = gr.area_t
+
+    def params(self, **over):
+        p = dict(self.case.params)
+        p.update(over)
+        op = OrcParams()
+        for f, _ in OrcParams._fields_:
+            setattr(op, f, p[f])
+        return op
+
+    @staticmethod
+    def new_diag(case):
+        km, jmt, imt = case.km, case.jmt, case.imt
+        z = lambda *s: np.zeros(s)  # noqa: E731
+        return dict(ax=z(2, km, jmt, imt), ay=z(2, km, jmt, imt), az=z(2, km, jmt, imt),
+                    dz=z(2, km, jmt, imt), tend=z(2, km, jmt, imt), dt_diff=z(2, km, jmt, imt),
+                    dt_conv=z(2, km, jmt, imt), penetrate=z(km, jmt, imt), net=z(2, jmt, imt))
+
+    def _fields(self, st, forcing):
+        f = OrcFields()
+        for k in ("at", "atb", "akt", "utf", "vtf", "utl", "vtl", "ws", "h0f", "h0l"):
+            setattr(f, k, _d(st[k]))
+        for k in ("swv", "nswv", "sst", "seaice", "sss", "fresh"):
+            setattr(f, k, _d(forcing[k]))
+        f.restore = _d(forcing.get("restore"))
+        return f
+
+    def step(self, st, diag=None, forcing=None, **over):
+        """Advance `st` (dict of numpy arrays, modified in place) by one step."""
+        forcing = forcing or self.case.forcing
+        diag = diag if diag is not None else self.new_diag(self.case)
+        od = OrcDiag(**{k: _d(v) for k, v in diag.items() if k != "fw_norm2"})
+        lib().orc_tracer_step(ctypes.byref(self.g), ctypes.byref(self.params(**over)),
+                              ctypes.byref(self._fields(st, forcing)), ctypes.byref(od))
+        diag["fw_norm2"] = od.fw_norm2
+        return diag
+
+    def convadj(self, st, diag=None, **over):
+        diag = diag if diag is not None else self.new_diag(self.case)
+        od = OrcDiag(**{k: _d(v) for k, v in diag.items() if k != "fw_norm2"})
+        lib().orc_convadj(ctypes.byref(self.g), ctypes.byref(self.params(**over)),
+                          ctypes.byref(self._fields(st, self.case.forcing)), ctypes.byref(od))
+        return diag
+
+
+def invtri_column(kz, wkc, odzt, odzp, dts, stf, vit0, d):
+    x = np.ascontiguousarray(d, dtype=np.float64).copy()
+    lib().orc_invtri_column(int(kz), _d(np.ascontiguousarray(wkc, np.float64)),
+                            _d(np.ascontiguousarray(odzt, np.float64)),
+                            _d(np.ascontiguousarray(odzp, np.float64)), float(dts), float(stf),
+                            float(vit0), _d(x))
+    return x
+
+
+def dens(eosc, k, t, s):
+    e = np.ascontiguousarray(eosc, np.float64)
+    return lib().orc_dens(_d(e), e.shape[1], int(k), float(t), float(s))
+
+
+def halo_fill(a):
+    a = np.ascontiguousarray(a)
+    jmt, imt = a.shape[-2], a.shape[-1]
+    nz = int(np.prod(a.shape[:-2])) if a.ndim > 2 else 1
+    lib().orc_halo_fill_f64(_d(a), imt, jmt, nz)
+    return a
+
+
+def content(case, oracle, st):
+    """Compensated T and S content sums sum(dzp*tarea*vit*at) over the
+    physical cells (the quantity energy_c.c:81-82 prints), via math.fsum."""
+    import math
+    j0, j1, i0, i1 = 2, case.jmt - 2, 2, case.imt - 2
+    w = (oracle.d["dzp"][:, None, None] * case.grid["tarea"][None] * oracle.d["vit"])[:, j0:j1, i0:i1]
+    out = []
+    for n in range(2):
+        out.append(math.fsum((w * st["at"][n][:, j0:j1, i0:i1]).ravel().tolist()))
+    return out

@@ -1,0 +1,55 @@
+"""The C-ABI library loads on the CPU host and exports every symbol that
+include/licom_tracer.h declares; the generator library likewise."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(licom_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_tracer_library_exports_every_declared_symbol():
+    import licom_tracer
+    names = declared("licom_tracer.h")
+    assert names, "no declarations found"
+    L = ctypes.CDLL(licom_tracer.lib_path())
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(licom_tracer.EXPORTED) == names
+
+
+def test_synth_library_exports_every_declared_symbol():
+    import licom_synth
+    names = declared("licom_synth.h")
+    L = licom_synth.lib()
+    assert [n for n in names if not hasattr(L, n)] == []
+
+
+def test_strerror_and_invalid_create_without_gpu():
+    """Argument validation runs before any device call."""
+    import licom_tracer as lt
+    L = lt.lib()
+    assert L.licom_strerror(-1).decode() == "invalid argument"
+    g, p, h = lt.Grid(), lt.Params(), ctypes.c_void_p()
+    g.nx, g.ny, g.km, g.imt_global = 2, 2, 1, 360   # invalid extents
+    rc = L.licom_tracer_create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(h))
+    assert rc == -1
+    assert b"invalid grid" in L.licom_last_error()
+    assert not h.value
+
+
+def test_product_does_not_touch_the_oracle():
+    """Nothing in the package imports, links or opens the oracle."""
+    pkg = os.path.join(ROOT, "licom3-hip_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".cpp", ".h", "Makefile")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in text.lower(), f
