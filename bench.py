@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark: LICOM3 tracer step (T and S, tspas + del4 + implicit vertical
+diffusion + convadj) on one MI355X at 3600 x 2302 x 55 (0.1 deg).
+
+Prints ONE JSON line.  `value` is tracer cell-updates per second
+(nx*ny*km*2 per step, land counted) with every input resident in HBM.
+`roofline` is the dominant kernel's algorithmic HBM bytes per launch over its
+average launch time (HIP events on the library's stream, timed region);
+`step_roofline` is the same for the whole step with the path's 128 B per
+tracer cell-update (FP64 drop-in, diagnostics on).  `cpu_baseline` times the
+CPU restatement (oracle/, OpenMP) on a band of the same workload on this
+host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--variant fp64]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "licom3-hip_amd"))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, vendor peak
+STEP_BYTES_PER_UPDATE = {"fp64": 128.0}
+
+
+def kernel_algorithmic_bytes(name, case):
+    """Compulsory HBM bytes of one launch of `name` on `case`: every array
+    the kernel must read or write, counted once, over the cells it covers.
+    Per interior cell (nx*ny*km), both tracers; 2-D terms neglected."""
+    cells = case.nx * case.ny * case.km
+    del4 = case.params["hmix"] == 4
+    per_cell = {
+        # fused path
+        # utf vtf utl vtl -> u_wface v_sface ws
+        "k_prelude": 8 * 4 + 8 * 3,
+        # at atb akt (x2) u_wface v_sface ws -> ori ax ay az dz (x2), penetrate
+        "k_march": 8 * (6 + 3) + 8 * (10 + 1),
+        # ori akt at (x2) -> at atb tend dt_diff dt_conv (x2)
+        "k_solve": 8 * 6 + 8 * 10,
+        # utf vtf read and written
+        "k_side": 8 * 4,
+        # per-stage path
+        # at(2) atb(2) akt(2) uw vs ws flags(2x1B) [d2tk(2)] -> ori(2) ax ay az dz (x2) pen
+        "k_correct": 8 * (2 + 2 + 2 + 2 + 1) + 2 + (16 if del4 else 0) + 8 * (2 + 8 + 1),
+        # ori(2) akt(2) at(2) utf vtf -> at(2) atb(2) tend dt_diff dt_conv (x2) utf vtf
+        "k_column": 8 * (2 + 2 + 2 + 2) + 8 * (2 + 2 + 6 + 2),
+        "k_predict": 8 * (2 + 2 + 1) + 2,
+        "k_faces": 8 * 4 + 8 * 2,
+        "k_ws": 8 * 4 + 8,
+        "k_d2tk": 8 * 2 + 8 * 2,
+    }
+    return per_cell.get(name, 0) * cells
+
+
+def traffic_from_profiles(kernel, workload):
+    """HBM bytes per launch from committed rocprofv3 --pmc summaries
+    (profiles/pmc_<kernel>_<workload>.json), or None."""
+    p = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(args):
+    """CPU restatement on a 3600 x ny_band x 55 band of the same workload
+    (Mercator rows around the equator, del4, dt=600), median of timed steps
+    after one warm-up step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import licom_synth as ls
+    import orc
+    band = args.cpu_band
+    case = ls.make_case("C3", ny=band)
+    o = orc.Oracle(case)
+    st = case.copy_state()
+    dg = orc.Oracle.new_diag(case)
+    o.step(st, dg)
+    times = []
+    t_end = time.time() + args.cpu_seconds
+    while len(times) < 5 or (time.time() < t_end and len(times) < 20):
+        t0 = time.perf_counter()
+        o.step(st, dg)
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    med = times[len(times) // 2]
+    upd = case.nx * case.ny * case.km * 2
+    return {"value": upd / med, "unit": "tracer cell-updates/s", "cores": orc.lib().orc_nthreads(),
+            "kind": "port",
+            "sample": f"oracle (C, OpenMP) on a {case.nx}x{case.ny}x{case.km} band of the 0.1 deg "
+                      f"workload, median of {len(times)} steps after 1 warm-up"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--variant", default="fp64", choices=["fp64"])
+    ap.add_argument("--workload", default="C3")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-band", type=int, default=128)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the tracer step would shard by 2-D horizontal tiles; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import licom_synth as ls
+    import licom_tracer as lt
+
+    t0 = time.time()
+    case = ls.make_case(args.workload)
+    print(f"[bench] generated {args.workload} {case.nx}x{case.ny}x{case.km} in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    model = lt.TracerModel(case, diagnostics=True)
+    case.state = None          # free host copies
+    flags = lt.LICOM_STEP_SIDE_EFFECTS | lt.LICOM_STEP_CONVADJ
+    for _ in range(args.warmup):
+        model.tracer(0, flags)
+    model.sync()
+    torch.cuda.synchronize()
+    print(f"[bench] warm-up done, timing {args.steps} steps", file=sys.stderr, flush=True)
+    if not args.no_kernel_timing:
+        model.kernel_times()       # drain
+        model.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        model.tracer(0, flags)
+    model.sync()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kt = {}
+    if not args.no_kernel_timing:
+        kt = model.kernel_times()
+        model.set_profiling(False)
+    ms_step = 1000.0 * dt / args.steps
+    updates = case.nx * case.ny * case.km * 2
+    value = updates / (dt / args.steps)
+
+    roof = None
+    if kt:
+        dom = max(kt, key=lambda k: kt[k][0])
+        avg_ms = kt[dom][0] / kt[dom][1]
+        alg = kernel_algorithmic_bytes(dom, case)
+        ach = alg / (avg_ms * 1e-3) / 1e9
+        tr = traffic_from_profiles(dom, args.workload)
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": tr, "algorithmic_bytes": alg, "avg_launch_ms": round(avg_ms, 4)}
+    step_bytes = STEP_BYTES_PER_UPDATE[args.variant] * updates
+    step_roof = {"bytes_per_update": STEP_BYTES_PER_UPDATE[args.variant],
+                 "achieved": round(step_bytes / (dt / args.steps) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                 "unit": "GB/s",
+                 "frac": round(step_bytes / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
+    model.close()
+    del model
+    torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args)
+
+    out = {
+        "metric": "tracer cell-updates/s at 3600x2302x55, FP64 vs mixed; p50/p99 request latency",
+        "value": value, "unit": "tracer cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.workload} {case.nx}x{case.ny}x{case.km} tracer step "
+                               f"(tspas, {'del4' if case.params['hmix'] == 4 else 'del2'}, "
+                               f"implicit vertical diffusion, convadj), FP64 drop-in, "
+                               f"diagnostics on",
+                   "nx": case.nx, "ny": case.ny, "km": case.km, "ntra": 2,
+                   "dts": case.params["dts"], "variant": args.variant},
+        "roofline": roof,
+        "step_roofline": step_roof,
+        "kernel_ms": {k: round(v[0] / v[1], 4) for k, v in kt.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -52,6 +52,9 @@ enum licom_adv { LICOM_ADV_TSPAS = 2 };
 #define LICOM_STEP_VTL_SCRATCH 0x4  /* leave the new S in vtl, as the reference
                                        does with its scratch reuse
                                        (tracer.cpp:1638,1891)                 */
+#define LICOM_STEP_STAGED 0x8       /* run one kernel per reference stage
+                                       instead of the fused passes (same bits;
+                                       a cross-check, not a fast path)        */
 
 /* Static grid (host pointers, copied by licom_tracer_create).  Replaces the
  * grid/pconst module arrays read through cuda_init.cpp:708-864. */
@@ -161,6 +164,16 @@ int licom_kernel_times(licom_ctx *ctx, int max, const char **names,
 
 /* Bytes of device scratch held by the context. */
 size_t licom_scratch_bytes(const licom_ctx *ctx);
+
+/* Number of columns that convadj mixed in the last fused step (waits for the
+ * stream).  Diagnostic with no reference counterpart (the reference counts
+ * convective levels in icmon only under LOWRES, convadj.cpp:247-254). */
+int licom_last_mix_count(licom_ctx *ctx, int *count);
+
+/* Device-to-device streaming copy of `bytes` on `hip_stream`, used to
+ * calibrate achievable HBM bandwidth and profiler byte counters.  No
+ * reference counterpart. */
+int licom_stream_copy(void *dst, const void *src, size_t bytes, void *hip_stream);
 
 void licom_tracer_destroy(licom_ctx *ctx);
 const char *licom_strerror(int status);

@@ -313,9 +313,9 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
     rc = rc ? rc : scratch(c, &s.err_norm2, 1);
     rc = rc ? rc : scratch(c, &s.uw, n3);
     rc = rc ? rc : scratch(c, &s.vs, n3);
-    rc = rc ? rc : scratch(c, &s.flags, 2 * n3);
     rc = rc ? rc : scratch(c, &s.ori, 2 * n3);
-    if (P->hmix == LICOM_HMIX_DEL4) rc = rc ? rc : scratch(c, &s.d2tk, 2 * n3);
+    rc = rc ? rc : scratch(c, &s.mix_count, 1);
+    rc = rc ? rc : scratch(c, &s.mix_list, n2);
     if (rc) {
         licom_tracer_destroy(c);
         return rc;
@@ -348,7 +348,16 @@ int licom_tracer_step(licom_ctx *c, const licom_fields *f, const licom_diag *d, 
         set_error("boundary_restore=1 needs the restore field");
         return LICOM_EINVAL;
     }
-    return launch_step(c, f, d, flags);
+    if (flags & LICOM_STEP_STAGED) {
+        // scratch of the per-stage kernels, allocated on first use
+        if (!c->s.flags) {
+            int rc = scratch(c, &c->s.flags, 2 * c->n3);
+            if (!rc && c->p.hmix == LICOM_HMIX_DEL4) rc = scratch(c, &c->s.d2tk, 2 * c->n3);
+            if (rc) return rc;
+        }
+        return launch_step_simple(c, f, d, flags);
+    }
+    return launch_step_fused(c, f, d, flags);
 }
 
 int licom_convadj(licom_ctx *c, const licom_fields *f, const licom_diag *d) {
@@ -424,6 +433,21 @@ int licom_kernel_times(licom_ctx *c, int max, const char **names, double *ms, in
 }
 
 size_t licom_scratch_bytes(const licom_ctx *c) { return c ? c->scratch_bytes : 0; }
+
+int licom_last_mix_count(licom_ctx *c, int *count) {
+    if (!c || !count) return LICOM_EINVAL;
+    if (int rc = hip_status(hipStreamSynchronize(c->stream), "sync")) return rc;
+    return hip_status(hipMemcpy(count, c->s.mix_count, sizeof(int), hipMemcpyDeviceToHost),
+                      "mix count copy");
+}
+
+int licom_stream_copy(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!dst || !src || (bytes % 16) != 0) {
+        set_error("licom_stream_copy: NULL pointer or size not a multiple of 16");
+        return LICOM_EINVAL;
+    }
+    return launch_stream_copy(dst, src, bytes, static_cast<hipStream_t>(stream));
+}
 
 void licom_tracer_destroy(licom_ctx *c) {
     if (!c) return;

@@ -37,6 +37,8 @@ struct Scratch {
     uint8_t *flags = nullptr;    // [2][km][jmt][imt] tspas limiter flags
     double *d2tk = nullptr;      // [2][km][jmt][imt] del4 first pass
     double *ori = nullptr;       // [2][km][jmt][imt] explicit update
+    int *mix_count = nullptr;    // [1] columns queued for convective mixing
+    int *mix_list = nullptr;     // [jmt*imt] their 2-D indices
     int nblk = 0;
 };
 
@@ -77,8 +79,25 @@ namespace licom {
 void set_error(const char *fmt, ...);
 int hip_status(hipError_t e, const char *what);
 
-// Kernel launchers (tracer_kernels.hip).  All enqueue on ctx->stream.
-int launch_step(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags);
+// Kernel launchers.  All enqueue on ctx->stream.
+// tracer_kernels.hip: one kernel per reference stage (cross-check path)
+int launch_step_simple(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags);
 int launch_convadj(licom_ctx *ctx, const licom_fields *f, const licom_diag *d);
+// shared pieces of tracer_kernels.hip used by both paths
+int launch_surface(licom_ctx *ctx, const licom_fields *f, const licom_diag *d);
+int launch_halo(licom_ctx *ctx, const licom_fields *f, int flags);
+// tracer_fused.hip: prelude / k-march / column solve (production path)
+int launch_step_fused(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags);
+
+int launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
+
+// RAII HIP-event bracket used when ctx->profiling is set.
+struct Timer {
+    licom_ctx *ctx;
+    const char *name;
+    hipEvent_t a{}, b{};
+    Timer(licom_ctx *c, const char *n);
+    ~Timer();
+};
 
 }  // namespace licom

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 
+#include "tracer_device.h"
 #include "tracer_internal.h"
 
 namespace licom {
@@ -49,9 +50,10 @@ __global__ void k_surface(KGrid g, KPar p, const double *__restrict__ h0f,
                           const double *swv, const double *nswv, const double *sst,
                           const double *seaice, const double *sss, const double *fresh,
                           double *stf_pre, double *work_u, double *stf_n, double *temp11,
-                          double *net) {
+                          double *net, int *mix_count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i == 0 && j == 0 && mix_count) *mix_count = 0;
     if (i >= g.imt || j >= g.jmt) return;
     const size_t q = i2(g, j, i), n2 = (size_t)g.imt * g.jmt;
     const double s0 = stf0(h0f, h0l, p.onbc, q);
@@ -90,11 +92,7 @@ __global__ void k_surface(KGrid g, KPar p, const double *__restrict__ h0f,
 // physical cells; the reference sums on the host in j-major order and then
 // MPI_ALLREDUCEs (tracer.cpp:1849-1868, mpi3.F90:26).  The order here is
 // fixed by the launch shape, so repeated runs give identical bits.
-__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
-    s = a + b;
-    double bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
+using dev::two_sum;
 
 constexpr int kRedThreads = 256;
 
@@ -230,48 +228,22 @@ __global__ void k_ws(KGrid g, KPar p, const double *__restrict__ utf,
 }
 
 // ------------------------------------------------------------------ tspas
-struct PTerms {
-    double x0, y0, xy1, xy2, xy3, xy4, c1, c2, za, zb1, zb2, zc;
-};
+using dev::PTerms;
 
-// advection_tracer_2 predictor terms (tracer.cpp:311-414)
+// advection_tracer_2 predictor terms (tracer.cpp:311-414) read from global
+// memory for cell (k,j,i).
 __device__ __forceinline__ void predictor_terms(const KGrid &g, double dts, const double *T,
                                                 const double *U, const double *V,
                                                 const double *W, int k, int j, int i,
                                                 PTerms &t) {
     const size_t c = i3(g, k, j, i), q = i2(g, j, i);
     const size_t sj = g.imt, sk = (size_t)g.imt * g.jmt;
-    const double tr = g.tarea_r[q];
-    const double Tc = T[c], Te = T[c + 1], Tw = T[c - 1], Ts = T[c + sj], Tn = T[c - sj];
-    const double Ue = U[c + 1], Uw = U[c], Vs = V[c], Vn = V[c - sj];
-    t.x0 = (Te + Tc) * Ue * tr - (Tc + Tw) * Uw * tr;
-    t.y0 = ((Ts + Tc) * Vs - (Tc + Tn) * Vn) * tr;
-    t.xy1 = -dts * (Te - Tc) * 2.0 * tr * Ue * Ue / (g.htw[q + 1] * g.hun[q + 1]);
-    t.xy2 = dts * (Tc - Tw) * 2.0 * tr * Uw * Uw / (g.htw[q] * g.hun[q]);
-    t.xy3 = -dts * (Ts - Tc) * 2.0 * tr * Vs * Vs / (g.hts[q] * g.hue[q]);
-    t.xy4 = dts * (Tc - Tn) * 2.0 * tr * Vn * Vn / (g.hts[q - sj] * g.hue[q - sj]);
-    t.c1 = -Tc * (Ue - Uw) * tr * 2.0;
-    t.c2 = -Tc * (Vs - Vn) * tr * 2.0;
-    const double *odzp = g.odzp, *odzt = g.odzt;
-    if (k == 0) {
-        const double W1 = W[c + sk], T1 = T[c + sk];
-        t.za = -0.5 * odzp[0] * W1 * (T1 + Tc);
-        t.zb1 = 0.0;
-        t.zb2 = 0.5 * odzp[0] * W1 * W1 * odzt[1] * (Tc - T1) * dts;
-        t.zc = odzp[0] * Tc * W1;
-    } else if (k == g.km - 1) {
-        const double Wk = W[c], Tu = T[c - sk];
-        t.za = 0.5 * odzp[k] * Wk * (Tc + Tu);
-        t.zb1 = -0.5 * odzp[k] * Wk * Wk * odzt[k] * (Tu - Tc) * dts;
-        t.zb2 = 0.0;
-        t.zc = -odzp[k] * Tc * Wk;
-    } else {
-        const double Wk = W[c], Wb = W[c + sk], Tu = T[c - sk], Td = T[c + sk];
-        t.za = 0.5 * odzp[k] * Wk * (Tc + Tu) - 0.5 * odzp[k] * Wb * (Tc + Td);
-        t.zb1 = -0.5 * odzp[k] * Wk * Wk * odzt[k] * (Tu - Tc) * dts;
-        t.zb2 = 0.5 * odzp[k] * Wb * Wb * odzt[k + 1] * (Tc - Td) * dts;
-        t.zc = -odzp[k] * Tc * (Wk - Wb);
-    }
+    const double Tu = k > 0 ? T[c - sk] : 0.0;
+    const double Td = k < g.km - 1 ? T[c + sk] : 0.0;
+    const double Wk = W[c], Wb = k < g.km - 1 ? W[c + sk] : 0.0;
+    dev::pterms(T[c], T[c + 1], T[c - 1], T[c + sj], T[c - sj], Tu, Td, U[c + 1], U[c], V[c],
+                V[c - sj], Wk, Wb, g.tarea_r[q], g.htw[q + 1] * g.hun[q + 1], g.htw[q] * g.hun[q],
+                g.hts[q] * g.hue[q], g.hts[q - sj] * g.hue[q - sj], k, g.km, g.odzp, g.odzt, dts, t);
 }
 
 // Value entering the 7-point bounds: at on ocean cells, the sentinel on land
@@ -298,10 +270,7 @@ __global__ void k_predict(KGrid g, KPar p, const double *__restrict__ at,
             const double *T = at + n * n3;
             PTerms t;
             predictor_terms(g, p.dts, T, uw, vs, ws, k, j, i, t);
-            const double xx = -(t.x0 + t.xy1 + t.xy2 + t.c1);
-            const double yy = -(t.y0 + t.xy3 + t.xy4 + t.c2);
-            const double zz = -(t.zb1 + t.zb2 + t.za + t.zc);
-            const double a00 = T[c] + (xx + yy + zz) * p.dts;
+            const double a00 = dev::predictor(t, T[c], p.dts);
             const double lo = -1.0e10, hi = 1.0e10;
             double mx = bval(g, T, k, j, i, lo), mn = bval(g, T, k, j, i, hi);
             const int nb[4][2] = {{j - 1, i}, {j + 1, i}, {j, i - 1}, {j, i + 1}};
@@ -360,11 +329,7 @@ __global__ void k_d2tk(KGrid g, const double *__restrict__ atb, double *d2tk) {
 
 __device__ __forceinline__ double akt_floored(const KPar &p, const double *akt, int n, int k,
                                               size_t c) {
-    double a = akt[c];
-    // AKT surface floor: del4 applies it to T only (tracer.cpp:1315), del2
-    // to every tracer (:1187)
-    if (k == 0 && (n == 0 || p.hmix == LICOM_HMIX_DEL2) && a < p.dwndmix) a = p.dwndmix;
-    return a;
+    return dev::floor_akt(akt[c], k, n, p.hmix, p.dwndmix);
 }
 
 // advection_tracer_3 (tracer.cpp:549-921), tracer_del4_2 / tracer_4
@@ -408,20 +373,20 @@ __global__ void k_correct(KGrid g, KPar p, const double *__restrict__ at,
             up3 = (fc || fs) && j <= g.jmt - 3;
             up4 = (fc || fn) && j >= 2;
         }
-        const double xy1 = up1 ? -(T[c + 1] - Tc) * fabs(uw[c + 1]) * tr : t.xy1;
-        const double xy2 = up2 ? (Tc - T[c - 1]) * fabs(uw[c]) * tr : t.xy2;
-        const double xy3 = up3 ? -(T[c + sj] - Tc) * fabs(vs[c]) * tr : t.xy3;
-        const double xy4 = up4 ? (Tc - T[c - sj]) * fabs(vs[c - sj]) * tr : t.xy4;
+        const double xy1 = up1 ? dev::up_xy1(T[c + 1], Tc, uw[c + 1], tr) : t.xy1;
+        const double xy2 = up2 ? dev::up_xy2(Tc, T[c - 1], uw[c], tr) : t.xy2;
+        const double xy3 = up3 ? dev::up_xy1(T[c + sj], Tc, vs[c], tr) : t.xy3;
+        const double xy4 = up4 ? dev::up_xy2(Tc, T[c - sj], vs[c - sj], tr) : t.xy4;
         double zb1, zb2;
         if (k == 0) {
             zb1 = 0.0;
-            zb2 = (fc || F[c + n2]) ? 0.5 * fabs(ws[c + n2]) * odzp[0] * (Tc - T[c + n2]) : t.zb2;
+            zb2 = (fc || F[c + n2]) ? dev::up_zb2(Tc, T[c + n2], ws[c + n2], odzp[0]) : t.zb2;
         } else if (k == km - 1) {
-            zb1 = (fc || F[c - n2]) ? -0.5 * fabs(ws[c]) * odzp[k] * (T[c - n2] - Tc) : t.zb1;
+            zb1 = (fc || F[c - n2]) ? dev::up_zb1(T[c - n2], Tc, ws[c], odzp[k]) : t.zb1;
             zb2 = 0.0;
         } else {
-            zb1 = (fc || F[c - n2]) ? -0.5 * fabs(ws[c]) * odzp[k] * (T[c - n2] - Tc) : t.zb1;
-            zb2 = (fc || F[c + n2]) ? 0.5 * fabs(ws[c + n2]) * odzp[k] * (Tc - T[c + n2]) : t.zb2;
+            zb1 = (fc || F[c - n2]) ? dev::up_zb1(T[c - n2], Tc, ws[c], odzp[k]) : t.zb1;
+            zb2 = (fc || F[c + n2]) ? dev::up_zb2(Tc, T[c + n2], ws[c + n2], odzp[k]) : t.zb2;
         }
         const double xx = -(t.x0 + xy1 + xy2 + t.c1);
         const double yy = -(t.y0 + xy3 + xy4 + t.c2);
@@ -541,82 +506,11 @@ __device__ void thomas(const KGrid &g, const KPar &p, const double *A, int n, si
     for (int k = 1; k < kz; ++k) x[k] = (e[k] * x[k - 1] + f[k]) * 1.0;
 }
 
-__device__ __forceinline__ double dens(const double *C, int km, int k, double t, double s) {
-    // dens.F90:12-14, C(k,m) at C[(m-1)*km+k]
-    return (C[k] + (C[3 * km + k] + C[6 * km + k] * s) * s +
-            (C[2 * km + k] + C[7 * km + k] * s + C[5 * km + k] * t) * t) * t +
-           (C[1 * km + k] + (C[4 * km + k] + C[8 * km + k] * s) * s) * s;
-}
+using dev::dens;
 
-// convadj_cu mixing (convadj.cpp:52-256) on columns T[], S[] (length km).
-__device__ void convect(const KGrid &g, int kcon, double *T, double *S, double *rhoup,
-                        double *rholo) {
-    const int km = g.km;
-    const double *C = g.eosc, *to = g.to, *so = g.so, *dzp = g.dzp;
-    if (kcon == 0) return;
-    for (int l = 0; l < km; ++l) rhoup[l] = rholo[l] = 0.0;
-    for (int l = 0; l < km - 1; ++l) {
-        const int l1 = l + 1;
-        rhoup[l1] = dens(C, km, l1, T[l1] - to[l1], S[l1] - so[l1]);
-        rholo[l] = dens(C, km, l1, T[l] - to[l1], S[l] - so[l1]);
-    }
-    int lcon = 0;
-    for (int k = kcon - 1; k >= 1; --k)
-        if (rholo[k - 1] > rhoup[k]) lcon = k;
-    if (lcon == 0) return;
-    double *X[2] = {T, S};
-    for (;;) {   // CONV_1
-        int lcona = lcon, lconb = lcon + 1;
-        double dztsum = dzp[lcona - 1] + dzp[lconb - 1];
-        double trasum[2];
-        for (int n = 0; n < 2; ++n) {
-            trasum[n] = X[n][lcona - 1] * dzp[lcona - 1] + X[n][lconb - 1] * dzp[lconb - 1];
-            const double tramix = trasum[n] / dztsum;
-            X[n][lcona - 1] = tramix;
-            X[n][lconb - 1] = tramix;
-        }
-        for (;;) {   // CONV_2
-            if (lconb != kcon) {
-                const int l1 = lconb + 1;
-                rholo[lconb - 1] = dens(C, km, l1 - 1, T[lconb - 1] - to[l1 - 1], S[lconb - 1] - so[l1 - 1]);
-                if (rholo[lconb - 1] > rhoup[l1 - 1]) {
-                    lconb = lconb + 1;
-                    dztsum = dztsum + dzp[lconb - 1];
-                    for (int n = 0; n < 2; ++n) {
-                        trasum[n] = trasum[n] + X[n][lconb - 1] * dzp[lconb - 1];
-                        const double tramix = trasum[n] / dztsum;
-                        for (int lm = lcona - 1; lm < lconb; ++lm) X[n][lm] = tramix;
-                    }
-                    continue;
-                }
-            }
-            if (lcona > 1) {
-                const int l1 = lcona - 1;
-                rholo[l1 - 1] = dens(C, km, lcona - 1, T[l1 - 1] - to[lcona - 1], S[l1 - 1] - so[lcona - 1]);
-                rhoup[lcona - 1] = dens(C, km, lcona - 1, T[lcona - 1] - to[lcona - 1],
-                                        S[lcona - 1] - so[lcona - 1]);
-                if (rholo[lcona - 2] > rhoup[lcona - 1]) {
-                    lcona = lcona - 1;
-                    dztsum = dztsum + dzp[lcona - 1];
-                    for (int n = 0; n < 2; ++n) {
-                        trasum[n] = trasum[n] + X[n][lcona - 1] * dzp[lcona - 1];
-                        const double tramix = trasum[n] / dztsum;
-                        for (int lm = lcona; lm <= lconb; ++lm) X[n][lm - 1] = tramix;
-                    }
-                    continue;
-                }
-            }
-            break;
-        }
-        if (lconb == kcon) return;
-        lcon = lconb;
-        for (;;) {   // CONV_3
-            lcon = lcon + 1;
-            if (lcon == kcon) return;
-            if (rholo[lcon - 1] <= rhoup[lcon]) continue;
-            break;
-        }
-    }
+__device__ __forceinline__ void convect(const KGrid &g, int kcon, double *T, double *S,
+                                        double *rhoup, double *rholo) {
+    dev::convect_column(g.eosc, g.to, g.so, g.dzp, g.km, kcon, T, S, rhoup, rholo);
 }
 
 // invtrit + halo-independent part of tracer_9 (tracer.cpp:1655-1699) +
@@ -746,6 +640,13 @@ __global__ void k_convadj(KGrid g, KPar p, double *at, double *atb, double *tend
         }
 }
 
+// 16-byte vector copy used for bandwidth calibration.
+__global__ void k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n) {
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (size_t)gridDim.x * blockDim.x)
+        dst[t] = src[t];
+}
+
 __global__ void k_copy_plane(const double *src, double *dst, size_t n) {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
          t += (size_t)gridDim.x * blockDim.x)
@@ -753,32 +654,65 @@ __global__ void k_copy_plane(const double *src, double *dst, size_t n) {
 }
 
 // ------------------------------------------------------------ launch glue
-struct Timer {
-    licom_ctx *ctx;
-    const char *name;
-    hipEvent_t a{}, b{};
-    Timer(licom_ctx *c, const char *n) : ctx(c), name(n) {
-        if (ctx->profiling) {
-            (void)hipEventCreate(&a);
-            (void)hipEventCreate(&b);
-            (void)hipEventRecord(a, ctx->stream);
-        }
-    }
-    ~Timer() {
-        if (ctx->profiling) {
-            (void)hipEventRecord(b, ctx->stream);
-            ctx->pending.push_back({name, a, b});
-        }
-    }
-};
-
 inline dim3 grid2(int imt, int jmt, dim3 b) {
     return dim3((imt + b.x - 1) / b.x, (jmt + b.y - 1) / b.y, 1);
 }
 
 }  // namespace
 
-int launch_step(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags) {
+Timer::Timer(licom_ctx *c, const char *n) : ctx(c), name(n) {
+    if (ctx->profiling) {
+        (void)hipEventCreate(&a);
+        (void)hipEventCreate(&b);
+        (void)hipEventRecord(a, ctx->stream);
+    }
+}
+
+Timer::~Timer() {
+    if (ctx->profiling) {
+        (void)hipEventRecord(b, ctx->stream);
+        ctx->pending.push_back({name, a, b});
+    }
+}
+
+int launch_surface(licom_ctx *ctx, const licom_fields *f, const licom_diag *d) {
+    const KGrid &g = ctx->g;
+    const KPar &p = ctx->kp;
+    Scratch &s = ctx->s;
+    hipStream_t st = ctx->stream;
+    const dim3 b2(64, 4);
+    {
+        Timer t(ctx, "k_surface");
+        k_surface<<<grid2(g.imt, g.jmt, b2), b2, 0, st>>>(
+            g, p, f->h0f, f->h0l, f->atb, f->swv, f->nswv, f->sst, f->seaice, f->sss, f->fresh,
+            s.stf_pre, s.work_u, s.stf_n, s.temp11, d ? d->net : nullptr, s.mix_count);
+    }
+    {
+        Timer t(ctx, "k_sss");
+        k_sss_partial<<<s.nblk, kRedThreads, 0, st>>>(g, s.temp11, s.partials);
+        k_sss_final<<<1, kRedThreads, 0, st>>>(s.partials, s.nblk, ctx->area_t, p.sssnorm,
+                                               s.err_norm2, d ? d->fw_norm2 : nullptr);
+    }
+    return hip_status(hipGetLastError(), "surface launch");
+}
+
+int launch_halo(licom_ctx *ctx, const licom_fields *f, int flags) {
+    const KGrid &g = ctx->g;
+    hipStream_t st = ctx->stream;
+    {
+        Timer t(ctx, "k_halo");
+        const int cells = 4 * (g.jmt - 4) + 4 * g.imt;
+        k_halo<<<dim3((cells + 255) / 256, 4 * g.km), 256, 0, st>>>(g.imt, g.jmt, 4 * g.km, f->at,
+                                                                     f->atb, g.km);
+    }
+    if (flags & LICOM_STEP_VTL_SCRATCH) {
+        Timer t(ctx, "k_copy_vtl");
+        k_copy_plane<<<1024, 256, 0, st>>>(f->at + ctx->n3, f->vtl, ctx->n3);
+    }
+    return hip_status(hipGetLastError(), "halo launch");
+}
+
+int launch_step_simple(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags) {
     const KGrid &g = ctx->g;
     const KPar &p = ctx->kp;
     Scratch &s = ctx->s;
@@ -787,19 +721,7 @@ int launch_step(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int 
     const dim3 g2 = grid2(g.imt, g.jmt, b2);
     const dim3 g3(g2.x, g2.y, g.km);
     const dim3 g3i((g.imt - 4 + 63) / 64, (g.jmt - 4 + 3) / 4, g.km);
-
-    {
-        Timer t(ctx, "k_surface");
-        k_surface<<<g2, b2, 0, st>>>(g, p, f->h0f, f->h0l, f->atb, f->swv, f->nswv, f->sst,
-                                     f->seaice, f->sss, f->fresh, s.stf_pre, s.work_u, s.stf_n,
-                                     s.temp11, d ? d->net : nullptr);
-    }
-    {
-        Timer t(ctx, "k_sss");
-        k_sss_partial<<<s.nblk, kRedThreads, 0, st>>>(g, s.temp11, s.partials);
-        k_sss_final<<<1, kRedThreads, 0, st>>>(s.partials, s.nblk, ctx->area_t, p.sssnorm,
-                                               s.err_norm2, d ? d->fw_norm2 : nullptr);
-    }
+    if (int rc = launch_surface(ctx, f, d)) return rc;
     {
         Timer t(ctx, "k_faces");
         k_faces<<<g3, b3, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl, s.uw, s.vs);
@@ -830,17 +752,13 @@ int launch_step(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int 
                                     f->vtf, f->h0f, d ? d->tend : nullptr,
                                     d ? d->dt_diff : nullptr, d ? d->dt_conv : nullptr);
     }
-    {
-        Timer t(ctx, "k_halo");
-        const int cells = 4 * (g.jmt - 4) + 4 * g.imt;
-        k_halo<<<dim3((cells + 255) / 256, 4 * g.km), 256, 0, st>>>(g.imt, g.jmt, 4 * g.km, f->at,
-                                                                     f->atb, g.km);
-    }
-    if (flags & LICOM_STEP_VTL_SCRATCH) {
-        Timer t(ctx, "k_copy_vtl");
-        k_copy_plane<<<1024, 256, 0, st>>>(f->at + ctx->n3, f->vtl, ctx->n3);
-    }
-    return hip_status(hipGetLastError(), "tracer step launch");
+    return launch_halo(ctx, f, flags);
+}
+
+int launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {
+    const size_t n = bytes / 16;
+    k_stream_copy<<<8192, 256, 0, st>>>(static_cast<const double2 *>(src), static_cast<double2 *>(dst), n);
+    return hip_status(hipGetLastError(), "stream copy launch");
 }
 
 int launch_convadj(licom_ctx *ctx, const licom_fields *f, const licom_diag *d) {

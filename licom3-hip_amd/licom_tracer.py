@@ -31,6 +31,7 @@ _VP = ctypes.c_void_p
 LICOM_STEP_CONVADJ = 0x1
 LICOM_STEP_SIDE_EFFECTS = 0x2
 LICOM_STEP_VTL_SCRATCH = 0x4
+LICOM_STEP_STAGED = 0x8
 HMIX_DEL2, HMIX_DEL4 = 2, 4
 ADV_TSPAS = 2
 
@@ -73,8 +74,8 @@ class GridDerived(ctypes.Structure):
 
 EXPORTED = ["licom_tracer_create", "licom_tracer_set_stream", "licom_tracer_step",
             "licom_convadj", "licom_tracer_sync", "licom_grid_export", "licom_set_profiling",
-            "licom_kernel_times", "licom_scratch_bytes", "licom_tracer_destroy",
-            "licom_strerror", "licom_last_error"]
+            "licom_kernel_times", "licom_scratch_bytes", "licom_last_mix_count",
+            "licom_stream_copy", "licom_tracer_destroy", "licom_strerror", "licom_last_error"]
 
 
 class LicomError(RuntimeError):
@@ -106,6 +107,8 @@ def lib():
                                          _DP, _IP, _IP]
         L.licom_scratch_bytes.argtypes = [_VP]
         L.licom_scratch_bytes.restype = ctypes.c_size_t
+        L.licom_last_mix_count.argtypes = [_VP, _IP]
+        L.licom_stream_copy.argtypes = [_VP, _VP, ctypes.c_size_t, _VP]
         L.licom_tracer_destroy.argtypes = [_VP]
         L.licom_strerror.argtypes = [ctypes.c_int]
         L.licom_strerror.restype = ctypes.c_char_p
@@ -280,6 +283,11 @@ class TracerModel:
               "licom_kernel_times")
         return {names[k].decode(): (ms[k], cnt[k]) for k in range(n.value)}
 
+    def mix_count(self):
+        n = ctypes.c_int32(0)
+        check(lib().licom_last_mix_count(self.ctx, ctypes.byref(n)), "licom_last_mix_count")
+        return int(n.value)
+
     def scratch_bytes(self):
         return int(lib().licom_scratch_bytes(self.ctx))
 
@@ -302,3 +310,22 @@ class TracerModel:
             self.close()
         except Exception:
             pass
+
+
+def stream_copy_bandwidth(nbytes=4 << 30, reps=5, device="cuda:0"):
+    """Achieved GB/s (read + write) of a 16-byte vector copy of `nbytes`."""
+    import torch
+    src = torch.empty(nbytes // 8, dtype=torch.float64, device=device).fill_(1.0)
+    dst = torch.empty_like(src)
+    s = torch.cuda.current_stream()
+    check(lib().licom_stream_copy(_ptr(dst), _ptr(src), nbytes, _VP(s.cuda_stream)), "copy")
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        check(lib().licom_stream_copy(_ptr(dst), _ptr(src), nbytes, _VP(s.cuda_stream)), "copy")
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    return 2.0 * nbytes / (ms * 1e-3) / 1e9

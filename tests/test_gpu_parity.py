@@ -101,6 +101,15 @@ def test_ten_steps_c2_bitwise(need_gpu):
     run_pair(ls.make_case("C2"), 10, tol=1e-10)
 
 
+def test_staged_kernels_match_oracle(need_gpu):
+    """The one-kernel-per-stage cross-check path gives the same bits."""
+    announce("staged path C1-land del4 x3")
+    case = ls.make_case("C1", depth_min=-3000.0, sigma=1.5, seed=11, hmix=4, ah=-1.0e10)
+    case.name = "C1-staged"
+    run_pair(case, 3, flags=lt.LICOM_STEP_SIDE_EFFECTS | lt.LICOM_STEP_CONVADJ | lt.LICOM_STEP_STAGED,
+             tol=1e-10)
+
+
 def test_forcing_restore_sssnorm(need_gpu):
     """Non-zero surface forcing (FRC_CORE fluxes, SSSNORM) and restoring.
     The SSSNORM sum runs in a different order on the device (compensated
