@@ -7,9 +7,10 @@
 //                  continuity w (upwell_2..4, operators_div); the column's
 //                  divergence profile stays in registers between the
 //                  column sum and the w recursion.
-//   k_march        one workgroup per 32x16 tile of columns and tracer,
-//                  marching k top to bottom.  Planes of at/atb (tile plus a
-//                  2-cell halo) are staged in LDS; the tspas predictor of
+//   k_march        one workgroup per 16x16 tile of columns and tracer,
+//                  marching k top to bottom.  Planes of at, atb and the
+//                  velocities (tile plus a 2-cell halo) are staged in LDS,
+//                  fetched a level ahead into registers; the tspas predictor of
 //                  level k+1 (tile plus a 1-cell ring) feeds the limiter of
 //                  level k through LDS flag planes; the del4 first pass is
 //                  kept in LDS.  Writes the explicit update (tracer_8's vtl)
@@ -115,12 +116,13 @@ __global__ __launch_bounds__(256) void k_prelude(KGrid g, KPar p, const double *
 }
 
 // --------------------------------------------------------------- march
-constexpr int BX = 32, BY = 16;                 // interior tile (one lane per column)
+constexpr int BX = 16, BY = 16;                 // interior tile (one lane per column)
 constexpr int PX = BX + 2, PY = BY + 2;         // tile + 1-cell ring (predictor, d2tk)
 constexpr int RX = BX + 4, RY = BY + 4;         // tile + 2-cell halo (staged planes)
-constexpr int NT = BX * BY;                     // 512 lanes
+constexpr int NT = BX * BY;                     // 256 lanes
 constexpr int NP = PX * PY, NR = RX * RY;
-constexpr int NRING = NP - BX * BY;             // 100 ring cells
+constexpr int NRING = NP - BX * BY;             // 68 ring cells
+constexpr int NPF = (NR + NT - 1) / NT;         // plane cells staged per lane
 
 struct MarchArgs {
     const double *at, *atb, *akt, *uw, *vs, *ws, *stf_n, *err_norm2, *swv, *restore;
@@ -140,74 +142,52 @@ __device__ __forceinline__ void ring_cell(int r, int &pj, int &pi) {
     pi = PX - 1;
 }
 
-// Own P cell: static metrics kept in registers for the whole march.
-struct OwnCell {
-    int rr;            // index in the staged R-planes
-    bool ok;           // inside [1,jmt-1) x [1,imt-1): predicted and mixed
-    double tr, D1, D2, D3, D4;
-    double dtn, dts, dte, dtw, ahf;
+// Static metrics of a predicted cell: tarea_r, the four face products
+// htw*hun (i+1, i), hts*hue (j, j-1) and the del2 stencil (DTN..DTW, AHF).
+struct Metrics {
+    double tr, D1, D2, D3, D4, dtn, dts, dte, dtw, ahf;
 };
 
-// Lax-Wendroff terms of the own cell carried from the predictor of level k
-// to the limiter of level k.
+__device__ __forceinline__ Metrics load_metrics(const KGrid &g, int gj, int gi) {
+    const size_t q = (size_t)gj * g.imt + gi, sj = g.imt;
+    Metrics m;
+    m.tr = g.tarea_r[q];
+    m.D1 = g.htw[q + 1] * g.hun[q + 1];
+    m.D2 = g.htw[q] * g.hun[q];
+    m.D3 = g.hts[q] * g.hue[q];
+    m.D4 = g.hts[q - sj] * g.hue[q - sj];
+    m.dtn = g.dtn[q];
+    m.dts = g.dts[q];
+    m.dte = g.dte[q];
+    m.dtw = g.dtw[q];
+    m.ahf = g.ahf[q];
+    return m;
+}
+
+// LW terms carried from the predictor of level k to the limiter of level k.
 struct LW {
     double xy1, xy2, xy3, xy4, zb1, zb2;
 };
 
-// Velocities of one level at a P cell.
-struct Vel {
-    double Ue, Uw, Vs, Vn, Wk, Wb;
+// Staged planes of the levels around the one being predicted.
+struct Lv {
+    const double *Pk, *Pu, *Pd;     // at of levels K, K-1, K+1
+    const double *U, *V, *Wk, *Wb;  // u_wface, v_sface of level K; ws at K and K+1
 };
 
-__device__ __forceinline__ void own_init(const KGrid &g, OwnCell &c, int gj, int gi, int rr) {
-    c.rr = rr;
-    c.ok = gj >= 1 && gj < g.jmt - 1 && gi >= 1 && gi < g.imt - 1;
-    c.tr = c.D1 = c.D2 = c.D3 = c.D4 = 0.0;
-    c.dtn = c.dts = c.dte = c.dtw = c.ahf = 0.0;
-    if (!c.ok) return;
-    const size_t q = (size_t)gj * g.imt + gi, sj = g.imt;
-    c.tr = g.tarea_r[q];
-    c.D1 = g.htw[q + 1] * g.hun[q + 1];
-    c.D2 = g.htw[q] * g.hun[q];
-    c.D3 = g.hts[q] * g.hue[q];
-    c.D4 = g.hts[q - sj] * g.hue[q - sj];
-    c.dtn = g.dtn[q];
-    c.dts = g.dts[q];
-    c.dte = g.dte[q];
-    c.dtw = g.dtw[q];
-    c.ahf = g.ahf[q];
-}
-
-__device__ __forceinline__ bool inside1(const KGrid &g, int gj, int gi) {
-    return gj >= 1 && gj < g.jmt - 1 && gi >= 1 && gi < g.imt - 1;
-}
-
-// Velocities of level K at global cell (gj, gi) (cached global loads).
-__device__ __forceinline__ Vel load_vel(const KGrid &g, const MarchArgs &a, int gj, int gi, int K) {
-    const size_t n2 = (size_t)g.imt * g.jmt;
-    const size_t o = (size_t)K * n2 + (size_t)gj * g.imt + gi;
-    Vel v;
-    v.Uw = a.uw[o];
-    v.Ue = a.uw[o + 1];
-    v.Vs = a.vs[o];
-    v.Vn = a.vs[o - g.imt];
-    v.Wk = a.ws[o];
-    v.Wb = (K < g.km - 1) ? a.ws[o + n2] : 0.0;
-    return v;
-}
-
-// Predictor and limiter flag of level K at R-plane index r from the staged
-// planes Pk (level K), Pu (K-1), Pd (K+1); returns the LW terms in lw.
-__device__ __forceinline__ bool pred_flag(const KGrid &g, int r, const double *Pk,
-                                          const double *Pu, const double *Pd, const int *sK,
-                                          int K, double dts, const Vel &v, double tr, double D1,
-                                          double D2, double D3, double D4, LW &lw) {
-    const double Tc = Pk[r], Te = Pk[r + 1], Tw = Pk[r - 1], Ts = Pk[r + RX], Tn = Pk[r - RX];
-    const double Tu = K > 0 ? Pu[r] : 0.0;
-    const double Td = K < g.km - 1 ? Pd[r] : 0.0;
+// Predictor and limiter flag of level K at R-plane index r.
+__device__ __forceinline__ bool pred_flag(const KGrid &g, int r, const Lv &L, const int *sK, int K,
+                                          double dts, double tr, double D1, double D2, double D3,
+                                          double D4, LW &lw) {
+    const double Tc = L.Pk[r], Te = L.Pk[r + 1], Tw = L.Pk[r - 1], Ts = L.Pk[r + RX],
+                 Tn = L.Pk[r - RX];
+    const double Tu = K > 0 ? L.Pu[r] : 0.0;
+    const double Td = K < g.km - 1 ? L.Pd[r] : 0.0;
+    const double Ue = L.U[r + 1], Uw = L.U[r], Vs = L.V[r], Vn = L.V[r - RX];
+    const double Wk = L.Wk[r], Wb = K < g.km - 1 ? L.Wb[r] : 0.0;
     dev::PTerms t;
-    dev::pterms(Tc, Te, Tw, Ts, Tn, Tu, Td, v.Ue, v.Uw, v.Vs, v.Vn, v.Wk, v.Wb, tr, D1, D2, D3, D4,
-                K, g.km, g.odzp, g.odzt, dts, t);
+    dev::pterms(Tc, Te, Tw, Ts, Tn, Tu, Td, Ue, Uw, Vs, Vn, Wk, Wb, tr, D1, D2, D3, D4, K, g.km,
+                g.odzp, g.odzt, dts, t);
     lw.xy1 = t.xy1;
     lw.xy2 = t.xy2;
     lw.xy3 = t.xy3;
@@ -239,16 +219,6 @@ __device__ __forceinline__ bool pred_flag(const KGrid &g, int r, const double *P
     return a00 > mx || a00 < mn;
 }
 
-// Ring cell: metrics loaded on the fly (the ring is ~1/5 of the tile).
-__device__ __forceinline__ bool ring_flag(const KGrid &g, int gj, int gi, int r, const double *Pk,
-                                          const double *Pu, const double *Pd, const int *sK, int K,
-                                          double dts, const Vel &v) {
-    const size_t q = (size_t)gj * g.imt + gi, sj = g.imt;
-    LW lw;
-    return pred_flag(g, r, Pk, Pu, Pd, sK, K, dts, v, g.tarea_r[q], g.htw[q + 1] * g.hun[q + 1],
-                     g.htw[q] * g.hun[q], g.hts[q] * g.hue[q], g.hts[q - sj] * g.hue[q - sj], lw);
-}
-
 // del4 first pass d2tk = AHF * del2(atb) at R-plane index r (tracer.cpp:1249-1286).
 __device__ __forceinline__ double d2tk_at(int r, const double *B, const int *sK, int k, double dtn,
                                           double dts, double dte, double dtw, double ahf) {
@@ -263,24 +233,43 @@ __device__ __forceinline__ double d2tk_at(int r, const double *B, const int *sK,
     return ahf * (cc * B[r] + hdtk);
 }
 
-__device__ __forceinline__ void load_plane(double *dst, const double *src, int gj0, int gi0,
-                                           int imt, int jmt) {
-    for (int r = threadIdx.x; r < NR; r += NT) {
-        const int gj = gj0 + r / RX, gi = gi0 + r % RX;
-        dst[r] = (gj >= 0 && gj < jmt && gi >= 0 && gi < imt) ? src[(size_t)gj * imt + gi] : 0.0;
+// One lane's share of a staged plane: the R-plane cells t, t+NT, ...
+struct PlaneSlots {
+    size_t off[NPF];   // offset inside one (jmt,imt) plane
+    bool in[NPF];      // cell inside the array (else staged as 0)
+};
+
+__device__ __forceinline__ void fetch(const PlaneSlots &s, const double *src, double (&v)[NPF]) {
+#pragma unroll
+    for (int u = 0; u < NPF; ++u) v[u] = s.in[u] ? src[s.off[u]] : 0.0;
+}
+
+__device__ __forceinline__ void stage(const double (&v)[NPF], double *dst) {
+#pragma unroll
+    for (int u = 0; u < NPF; ++u) {
+        const int r = threadIdx.x + u * NT;
+        if (r < NR) dst[r] = v[u];
     }
 }
 
+// k-march over one 16x16 tile of columns and one tracer.  Every level's
+// inputs (at, atb, u_wface, v_sface, ws) are staged as LDS planes covering
+// the tile plus a 2-cell halo; each lane fetches its share of the next level
+// into registers while the current level is computed.
 __global__ __launch_bounds__(NT) void k_march(KGrid g, KPar p, MarchArgs a) {
-    __shared__ double sAT[4][NR];
-    __shared__ double sB[3][NR];
+    __shared__ double sAT[3][NR];
+    __shared__ double sB[2][NR];
+    __shared__ double sU[2][NR];
+    __shared__ double sV[2][NR];
+    __shared__ double sW[3][NR];
     __shared__ double sD2[NP];
+    __shared__ double sRM[10][NRING];
     __shared__ unsigned char sFL[2][NP];
     __shared__ int sK[NR];
 
     // XCD-aware placement: consecutive blocks of one XCD take the T and S
     // halves of one tile, then the next tile of the same row of tiles.
-    const int L = blockIdx.x, xcd = L & 7, slot = L >> 3;
+    const int Lb = blockIdx.x, xcd = Lb & 7, slot = Lb >> 3;
     const int tile = xcd * a.tiles_per_xcd + (slot >> 1);
     const int n = slot & 1;
     if (tile >= a.ntiles) return;
@@ -298,49 +287,77 @@ __global__ __launch_bounds__(NT) void k_march(KGrid g, KPar p, MarchArgs a) {
     const double *A = a.akt + n * n3;
     const double dts = p.dts;
 
-    OwnCell own;
-    own_init(g, own, gj, gi, (lj + 2) * RX + (li + 2));
-    const int pOwn = (lj + 1) * PX + (li + 1);
-    // ring slot of this lane (lanes >= NRING have none)
-    int rpj = 0, rpi = 0;
-    if (t < NRING) ring_cell(t, rpj, rpi);
-    const int rgj = gj0 - 1 + rpj, rgi = gi0 - 1 + rpi;
-    const bool rok = t < NRING && inside1(g, rgj, rgi);
-    const int rR = (rpj + 1) * RX + (rpi + 1), pR = rpj * PX + rpi;
-    double r_dtn = 0, r_dts = 0, r_dte = 0, r_dtw = 0, r_ahf = 0;
-    if (rok && p.hmix == LICOM_HMIX_DEL4) {
-        const size_t rq = (size_t)rgj * imt + rgi;
-        r_dtn = g.dtn[rq];
-        r_dts = g.dts[rq];
-        r_dte = g.dte[rq];
-        r_dtw = g.dtw[rq];
-        r_ahf = g.ahf[rq];
+    PlaneSlots ps;
+#pragma unroll
+    for (int u = 0; u < NPF; ++u) {
+        const int r = t + u * NT;
+        const int rj = gj0 - 2 + r / RX, ri = gi0 - 2 + r % RX;
+        ps.in[u] = r < NR && rj >= 0 && rj < jmt && ri >= 0 && ri < imt;
+        ps.off[u] = ps.in[u] ? (size_t)rj * imt + ri : 0;
     }
-
     for (int r = t; r < NR; r += NT) {
         const int rj = gj0 - 2 + r / RX, ri = gi0 - 2 + r % RX;
         sK[r] = (rj >= 0 && rj < jmt && ri >= 0 && ri < imt) ? g.kmt[(size_t)rj * imt + ri] : 0;
+    }
+    // own cell and ring cell
+    const int rOwn = (lj + 2) * RX + (li + 2), pOwn = (lj + 1) * PX + (li + 1);
+    const bool own_ok = gj >= 1 && gj < jmt - 1 && gi >= 1 && gi < imt - 1;
+    Metrics M{};
+    if (own_ok) M = load_metrics(g, gj, gi);
+    int rpj = 0, rpi = 0;
+    if (t < NRING) ring_cell(t, rpj, rpi);
+    const int rgj = gj0 - 1 + rpj, rgi = gi0 - 1 + rpi;
+    const bool rok = t < NRING && rgj >= 1 && rgj < jmt - 1 && rgi >= 1 && rgi < imt - 1;
+    const int rR = (rpj + 1) * RX + (rpi + 1), pR = rpj * PX + rpi;
+    if (rok) {
+        const Metrics m = load_metrics(g, rgj, rgi);
+        sRM[0][t] = m.tr; sRM[1][t] = m.D1; sRM[2][t] = m.D2; sRM[3][t] = m.D3; sRM[4][t] = m.D4;
+        sRM[5][t] = m.dtn; sRM[6][t] = m.dts; sRM[7][t] = m.dte; sRM[8][t] = m.dtw; sRM[9][t] = m.ahf;
     }
     const double sw = (n == 0 && act && a.swv) ? a.swv[q] : 0.0;
     const int kmt_own = act ? g.kmt[q] : 0;
     const double stf_n = act ? a.stf_n[n * n2 + q] : 0.0;
     const double enorm = (n == 1 && p.sssnorm && act) ? a.err_norm2[0] : 0.0;
+    const bool rest = p.boundary_restore == 1 && a.restore != nullptr;
 
-    load_plane(sAT[0], T, gj0 - 2, gi0 - 2, imt, jmt);
-    load_plane(sAT[1], T + n2, gj0 - 2, gi0 - 2, imt, jmt);
-    load_plane(sB[0], TB, gj0 - 2, gi0 - 2, imt, jmt);
-    Vel vc = own.ok ? load_vel(g, a, gj, gi, 0) : Vel{0, 0, 0, 0, 0, 0};
-    Vel vr = rok ? load_vel(g, a, rgj, rgi, 0) : Vel{0, 0, 0, 0, 0, 0};
+    // prologue: levels 0 and 1 of at and ws, level 0 of atb, u_wface, v_sface
+    double v0[NPF], v1[NPF];
+    fetch(ps, T, v0);
+    fetch(ps, T + n2, v1);
+    stage(v0, sAT[0]);
+    stage(v1, sAT[1]);
+    fetch(ps, TB, v0);
+    stage(v0, sB[0]);
+    fetch(ps, a.uw, v0);
+    fetch(ps, a.vs, v1);
+    stage(v0, sU[0]);
+    stage(v1, sV[0]);
+    fetch(ps, a.ws, v0);
+    fetch(ps, a.ws + n2, v1);
+    stage(v0, sW[0]);
+    stage(v1, sW[1]);
+    // registers holding the next level's share of every plane
+    double fAT[NPF], fB[NPF], fU[NPF], fV[NPF], fW[NPF];
+    if (km > 2) fetch(ps, T + 2 * n2, fAT);
+    fetch(ps, TB + n2, fB);
+    fetch(ps, a.uw + n2, fU);
+    fetch(ps, a.vs + n2, fV);
+    if (km > 2) fetch(ps, a.ws + 2 * n2, fW);
+    double fA = act ? A[q] : 0.0;
+    double fR = 0.0;
     __syncthreads();
 
     LW lc{}, ln{};
     {
+        const Lv L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
         bool f0 = false;
-        if (own.ok)
-            f0 = pred_flag(g, own.rr, sAT[0], sAT[0], sAT[1], sK, 0, dts, vc, own.tr, own.D1,
-                           own.D2, own.D3, own.D4, lc);
+        if (own_ok) f0 = pred_flag(g, rOwn, L0, sK, 0, dts, M.tr, M.D1, M.D2, M.D3, M.D4, lc);
         sFL[0][pOwn] = f0;
-        if (rok) sFL[0][pR] = ring_flag(g, rgj, rgi, rR, sAT[0], sAT[0], sAT[1], sK, 0, dts, vr);
+        if (rok) {
+            LW tmp;
+            sFL[0][pR] = pred_flag(g, rR, L0, sK, 0, dts, sRM[0][t], sRM[1][t], sRM[2][t],
+                                   sRM[3][t], sRM[4][t], tmp);
+        }
     }
     double Tprev = 0.0, Bprev = 0.0, Aprev = 0.0;
     bool fprev = false;
@@ -348,43 +365,60 @@ __global__ __launch_bounds__(NT) void k_march(KGrid g, KPar p, MarchArgs a) {
     const double aidif = 0.5;
 
     for (int k = 0; k < km; ++k) {
-        // ---- phase L: stage the planes of levels k+2 (at) and k+1 (atb)
-        if (k + 2 < km) load_plane(sAT[(k + 2) & 3], T + (size_t)(k + 2) * n2, gj0 - 2, gi0 - 2, imt, jmt);
-        if (k + 1 < km) load_plane(sB[(k + 1) % 3], TB + (size_t)(k + 1) * n2, gj0 - 2, gi0 - 2, imt, jmt);
-        const double Acur = act ? dev::floor_akt(A[(size_t)k * n2 + q], k, n, p.hmix, p.dwndmix) : 0.0;
-        const double Rk = (p.boundary_restore == 1 && a.restore && act && k >= 1)
-                              ? a.restore[n * n3 + (size_t)k * n2 + q] : 0.0;
-        Vel vn{0, 0, 0, 0, 0, 0}, vrn{0, 0, 0, 0, 0, 0};
+        // ---- phase L: stage the prefetched planes, prefetch the next level
+        if (k + 2 < km) stage(fAT, sAT[(k + 2) % 3]);
         if (k + 1 < km) {
-            if (own.ok) vn = load_vel(g, a, gj, gi, k + 1);
-            if (rok) vrn = load_vel(g, a, rgj, rgi, k + 1);
+            stage(fB, sB[(k + 1) & 1]);
+            stage(fU, sU[(k + 1) & 1]);
+            stage(fV, sV[(k + 1) & 1]);
+        }
+        if (k + 2 < km) stage(fW, sW[(k + 2) % 3]);
+        const double Acur = act ? dev::floor_akt(fA, k, n, p.hmix, p.dwndmix) : 0.0;
+        const double Rk = fR;
+        if (k + 3 < km) fetch(ps, T + (size_t)(k + 3) * n2, fAT);
+        if (k + 2 < km) {
+            fetch(ps, TB + (size_t)(k + 2) * n2, fB);
+            fetch(ps, a.uw + (size_t)(k + 2) * n2, fU);
+            fetch(ps, a.vs + (size_t)(k + 2) * n2, fV);
+        }
+        if (k + 3 < km) fetch(ps, a.ws + (size_t)(k + 3) * n2, fW);
+        if (k + 1 < km) {
+            fA = act ? A[(size_t)(k + 1) * n2 + q] : 0.0;
+            fR = (rest && act) ? a.restore[n * n3 + (size_t)(k + 1) * n2 + q] : 0.0;
         }
         __syncthreads();
 
         // ---- phase P: predictor of level k+1 (own + ring), d2tk of level k
-        const double *Pk = sAT[k & 3], *Pk1 = sAT[(k + 1) & 3], *Pk2 = sAT[(k + 2) & 3];
-        const double *Bk = sB[k % 3], *Bk1 = sB[(k + 1) % 3];
+        const double *Pk = sAT[k % 3], *Pk1 = sAT[(k + 1) % 3];
+        const double *Bk = sB[k & 1], *Bk1 = sB[(k + 1) & 1];
         if (k + 1 < km) {
+            const Lv L1{Pk1, Pk, sAT[(k + 2) % 3], sU[(k + 1) & 1], sV[(k + 1) & 1],
+                        sW[(k + 1) % 3], sW[(k + 2) % 3]};
             bool f = false;
-            if (own.ok)
-                f = pred_flag(g, own.rr, Pk1, Pk, Pk2, sK, k + 1, dts, vn, own.tr, own.D1, own.D2,
-                              own.D3, own.D4, ln);
+            if (own_ok) f = pred_flag(g, rOwn, L1, sK, k + 1, dts, M.tr, M.D1, M.D2, M.D3, M.D4, ln);
             sFL[(k + 1) & 1][pOwn] = f;
-            if (rok) sFL[(k + 1) & 1][pR] = ring_flag(g, rgj, rgi, rR, Pk1, Pk, Pk2, sK, k + 1, dts, vrn);
+            if (rok) {
+                LW tmp;
+                sFL[(k + 1) & 1][pR] = pred_flag(g, rR, L1, sK, k + 1, dts, sRM[0][t], sRM[1][t],
+                                                 sRM[2][t], sRM[3][t], sRM[4][t], tmp);
+            }
         }
         if (p.hmix == LICOM_HMIX_DEL4) {
-            sD2[pOwn] = own.ok ? d2tk_at(own.rr, Bk, sK, k, own.dtn, own.dts, own.dte, own.dtw, own.ahf)
-                               : 0.0;
-            if (rok) sD2[pR] = d2tk_at(rR, Bk, sK, k, r_dtn, r_dts, r_dte, r_dtw, r_ahf);
+            sD2[pOwn] = own_ok ? d2tk_at(rOwn, Bk, sK, k, M.dtn, M.dts, M.dte, M.dtw, M.ahf) : 0.0;
+            if (rok)
+                sD2[pR] = d2tk_at(rR, Bk, sK, k, sRM[5][t], sRM[6][t], sRM[7][t], sRM[8][t], sRM[9][t]);
         }
         __syncthreads();
 
         // ---- phase C: limited fluxes and the explicit update of level k
         if (act) {
             const unsigned char *F = sFL[k & 1];
-            const int r = own.rr, pp = pOwn;
+            const int r = rOwn, pp = pOwn;
             const double Tc = Pk[r], Te = Pk[r + 1], Tw = Pk[r - 1], Ts = Pk[r + RX], Tn = Pk[r - RX];
             const double Tu = Tprev, Td = (k + 1 < km) ? Pk1[r] : 0.0;
+            const double *U = sU[k & 1], *V = sV[k & 1];
+            const double Ue = U[r + 1], Uw = U[r], Vs = V[r], Vn = V[r - RX];
+            const double Wk = sW[k % 3][r], Wb = (k + 1 < km) ? sW[(k + 1) % 3][r] : 0.0;
             const bool fc = F[pp], fe = F[pp + 1], fw = F[pp - 1], fs = F[pp + PX], fn = F[pp - PX];
             const bool fd = (k + 1 < km) ? sFL[(k + 1) & 1][pp] : false;
             bool up1, up2, up3, up4;
@@ -399,24 +433,23 @@ __global__ __launch_bounds__(NT) void k_march(KGrid g, KPar p, MarchArgs a) {
                 up3 = (fc || fs) && gj <= jmt - 3;
                 up4 = (fc || fn) && gj >= 2;
             }
-            const double tr = own.tr;
+            const double tr = M.tr;
             dev::PTerms tc;
-            dev::pterms_cheap(Tc, Te, Tw, Ts, Tn, Tu, Td, vc.Ue, vc.Uw, vc.Vs, vc.Vn, vc.Wk, vc.Wb, tr,
-                              k, km, g.odzp, tc);
-            const double xy1 = up1 ? dev::up_xy1(Te, Tc, vc.Ue, tr) : lc.xy1;
-            const double xy2 = up2 ? dev::up_xy2(Tc, Tw, vc.Uw, tr) : lc.xy2;
-            const double xy3 = up3 ? dev::up_xy1(Ts, Tc, vc.Vs, tr) : lc.xy3;
-            const double xy4 = up4 ? dev::up_xy2(Tc, Tn, vc.Vn, tr) : lc.xy4;
+            dev::pterms_cheap(Tc, Te, Tw, Ts, Tn, Tu, Td, Ue, Uw, Vs, Vn, Wk, Wb, tr, k, km, g.odzp, tc);
+            const double xy1 = up1 ? dev::up_xy1(Te, Tc, Ue, tr) : lc.xy1;
+            const double xy2 = up2 ? dev::up_xy2(Tc, Tw, Uw, tr) : lc.xy2;
+            const double xy3 = up3 ? dev::up_xy1(Ts, Tc, Vs, tr) : lc.xy3;
+            const double xy4 = up4 ? dev::up_xy2(Tc, Tn, Vn, tr) : lc.xy4;
             double zb1, zb2;
             if (k == 0) {
                 zb1 = 0.0;
-                zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, vc.Wb, g.odzp[0]) : lc.zb2;
+                zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, Wb, g.odzp[0]) : lc.zb2;
             } else if (k == km - 1) {
-                zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, vc.Wk, g.odzp[k]) : lc.zb1;
+                zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, Wk, g.odzp[k]) : lc.zb1;
                 zb2 = 0.0;
             } else {
-                zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, vc.Wk, g.odzp[k]) : lc.zb1;
-                zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, vc.Wb, g.odzp[k]) : lc.zb2;
+                zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, Wk, g.odzp[k]) : lc.zb1;
+                zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, Wb, g.odzp[k]) : lc.zb2;
             }
             const double xx = -(tc.x0 + xy1 + xy2 + tc.c1);
             const double yy = -(tc.y0 + xy3 + xy4 + tc.c2);
@@ -435,10 +468,10 @@ __global__ __launch_bounds__(NT) void k_march(KGrid g, KPar p, MarchArgs a) {
                 double hdtk = 0.0, cc = 0.0;
                 if (k <= sK[r] - 1) {
                     double cf;
-                    if (k <= sK[r - RX] - 1) { cf = own.dtn; hdtk += cf * X[xs - xw]; cc -= cf; }
-                    if (k <= sK[r + RX] - 1) { cf = own.dts; hdtk += cf * X[xs + xw]; cc -= cf; }
-                    if (k <= sK[r + 1] - 1) { cf = own.dte; hdtk += cf * X[xs + 1]; cc -= cf; }
-                    if (k <= sK[r - 1] - 1) { cf = own.dtw; hdtk += cf * X[xs - 1]; cc -= cf; }
+                    if (k <= sK[r - RX] - 1) { cf = M.dtn; hdtk += cf * X[xs - xw]; cc -= cf; }
+                    if (k <= sK[r + RX] - 1) { cf = M.dts; hdtk += cf * X[xs + xw]; cc -= cf; }
+                    if (k <= sK[r + 1] - 1) { cf = M.dte; hdtk += cf * X[xs + 1]; cc -= cf; }
+                    if (k <= sK[r - 1] - 1) { cf = M.dtw; hdtk += cf * X[xs - 1]; cc -= cf; }
                 }
                 tf += p.ah * (cc * X[xs] + hdtk);
             }
@@ -490,7 +523,7 @@ __global__ __launch_bounds__(NT) void k_march(KGrid g, KPar p, MarchArgs a) {
                     tf = tf + stf_n * odzp[0] * 0.5;
                 }
             }
-            if (p.boundary_restore == 1 && a.restore && k >= 1) tf = tf + vk * (Rk - Bc) * lamda;
+            if (rest && k >= 1) tf = tf + vk * (Rk - Bc) * lamda;
             // tracer_8: the explicit update entering the implicit solve
             a.ori[n * n3 + c] = Tc + dts * tf;
             Tprev = Tc;
@@ -499,7 +532,7 @@ __global__ __launch_bounds__(NT) void k_march(KGrid g, KPar p, MarchArgs a) {
         }
         Aprev = Acur;
         lc = ln;
-        vc = vn;
+        __syncthreads();
     }
 }
 
@@ -510,9 +543,72 @@ __global__ __launch_bounds__(NT) void k_march(KGrid g, KPar p, MarchArgs a) {
 // from the top.  Holding e(k), f(k) for the whole column would take 2*km
 // doubles per lane; instead the elimination keeps them only at every SEG-th
 // level, and the substitution recomputes each SEG-level segment from its
-// checkpoint with the same operations (identical bits), re-reading the
-// segment's inputs while they are still in cache.
+// checkpoint with the same operations (identical bits).  Both sweeps load
+// the next segment's inputs while computing the current one.
 constexpr int SEG = 8;
+
+// convadj_cu (convadj.cpp:52-256) for one column, driven by the mask of
+// statically unstable interfaces found with the unmixed densities (bit k:
+// rholo(k-1) > rhoup(k)).  Every density the reference reads after mixing
+// starts is either recomputed by it or belongs to a level that mixing has not
+// touched, so the mask plus on-demand recomputation reproduces its decisions.
+// The two lanes of a column run this together: each holds its own tracer's
+// column X (the state just written, stride n2) and they swap values with
+// __shfl_xor.  Returns the mixed level range in lo/hi.
+__device__ __noinline__ void convect_sparse(const double *C, const double *to, const double *so,
+                                            const double *dzp, int km, int kcon, uint64_t mask,
+                                            int n, double *X, size_t n2, int &lo, int &hi) {
+    lo = km;
+    hi = -1;
+    int lcon = __ffsll((unsigned long long)mask) - 1;
+    for (;;) {   // CONV_1
+        int lcona = lcon, lconb = lcon + 1;
+        double dztsum = dzp[lcona - 1] + dzp[lconb - 1];
+        double trasum = X[(size_t)(lcona - 1) * n2] * dzp[lcona - 1] + X[(size_t)(lconb - 1) * n2] * dzp[lconb - 1];
+        double tramix = trasum / dztsum;
+        for (;;) {   // CONV_2
+            if (lconb != kcon) {
+                const double v = X[(size_t)lconb * n2];
+                const double vo = __shfl_xor(v, 1), mo = __shfl_xor(tramix, 1);
+                const double Tm = n == 0 ? tramix : mo, Sm = n == 0 ? mo : tramix;
+                const double Tv = n == 0 ? v : vo, Sv = n == 0 ? vo : v;
+                const double rl = dev::dens(C, km, lconb, Tm - to[lconb], Sm - so[lconb]);
+                const double ru = dev::dens(C, km, lconb, Tv - to[lconb], Sv - so[lconb]);
+                if (rl > ru) {
+                    lconb = lconb + 1;
+                    dztsum = dztsum + dzp[lconb - 1];
+                    trasum = trasum + v * dzp[lconb - 1];
+                    tramix = trasum / dztsum;
+                    continue;
+                }
+            }
+            if (lcona > 1) {
+                const double v = X[(size_t)(lcona - 2) * n2];
+                const double vo = __shfl_xor(v, 1), mo = __shfl_xor(tramix, 1);
+                const double Tm = n == 0 ? tramix : mo, Sm = n == 0 ? mo : tramix;
+                const double Tv = n == 0 ? v : vo, Sv = n == 0 ? vo : v;
+                const double rl = dev::dens(C, km, lcona - 1, Tv - to[lcona - 1], Sv - so[lcona - 1]);
+                const double ru = dev::dens(C, km, lcona - 1, Tm - to[lcona - 1], Sm - so[lcona - 1]);
+                if (rl > ru) {
+                    lcona = lcona - 1;
+                    dztsum = dztsum + dzp[lcona - 1];
+                    trasum = trasum + v * dzp[lcona - 1];
+                    tramix = trasum / dztsum;
+                    continue;
+                }
+            }
+            break;
+        }
+        for (int l = lcona - 1; l < lconb; ++l) X[(size_t)l * n2] = tramix;
+        lo = min(lo, lcona - 1);
+        hi = max(hi, lconb - 1);
+        if (lconb == kcon) return;
+        // CONV_3: next unstable interface below the mixed region
+        const uint64_t below = lconb >= 63 ? 0ull : (mask & ~((2ull << lconb) - 1ull));
+        if (below == 0) return;
+        lcon = __ffsll((unsigned long long)below) - 1;
+    }
+}
 
 template <int KM>
 __global__ __launch_bounds__(256) void k_solve(KGrid g, KPar p, int flags, double *__restrict__ at,
@@ -532,66 +628,120 @@ __global__ __launch_bounds__(256) void k_solve(KGrid g, KPar p, int flags, doubl
     const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
     const size_t q = act ? (size_t)gj * imt + gi : 0;
     const int kz = act ? g.kmt[q] : 0;
-    const double *O = ori + n * n3;
-    const double *A = akt + n * n3;
+    const double *O = ori + n * n3 + q;
+    const double *A = akt + n * n3 + q;
+    double *TT = at + n * n3 + q, *BB = atb + n * (n3 + n2) + n2 + q;
     const double *odzp = g.odzp, *odzt = g.odzt;
     const double dts = p.dts;
     const double stf = act ? stf_n[n * n2 + q] : 0.0;
-    auto wkc = [&](int k) { return dev::floor_akt(A[(size_t)k * n2 + q], k, n, p.hmix, p.dwndmix); };
+    const int hmix = p.hmix;
+    const double dwm = p.dwndmix;
 
-    // elimination from the bottom; checkpoints e,f at levels SEG*m, m >= 1
+    // ---- elimination from the bottom; checkpoints e,f at levels SEG*m (m >= 1)
     double cke[NSEG + 1], ckf[NSEG + 1];
 #pragma unroll
     for (int m = 0; m <= NSEG; ++m) cke[m] = ckf[m] = 0.0;
     {
+        double pd[SEG], pa[SEG];
+#pragma unroll
+        for (int r = 0; r < SEG; ++r) {
+            const int k = SEG * (NSEG - 1) + r;
+            pd[r] = pa[r] = 0.0;
+            if (k < KM && k < kz) {
+                pd[r] = O[(size_t)k * n2];
+                pa[r] = k >= 1 ? A[(size_t)(k - 1) * n2] : 0.0;
+            }
+        }
         double en = 0.0, fn = 0.0, wk_k = 0.0;
 #pragma unroll
-        for (int k = KM - 1; k >= 0; --k) {
-            if (k < kz) {
-                const double d = O[(size_t)k * n2 + q];
-                const double wkm1 = k >= 1 ? wkc(k - 1) : 0.0;
-                double aa, bb, cc;
-                dev::tri_row(k, kz, wkm1, wk_k, odzp, odzt, dts, aa, bb, cc);
-                const double g0 = 1.0 / (bb - cc * en);
-                en = aa * g0;
-                fn = (d + cc * fn) * g0;
-                wk_k = wkm1;
+        for (int m = NSEG - 1; m >= 0; --m) {
+            double cd[SEG], ca[SEG];
+#pragma unroll
+            for (int r = 0; r < SEG; ++r) {
+                cd[r] = pd[r];
+                ca[r] = pa[r];
             }
-            if (k > 0 && k % SEG == 0) {
-                cke[k / SEG] = en;
-                ckf[k / SEG] = fn;
+            if (m > 0) {
+#pragma unroll
+                for (int r = 0; r < SEG; ++r) {
+                    const int k = SEG * (m - 1) + r;
+                    pd[r] = pa[r] = 0.0;
+                    if (k < kz) {
+                        pd[r] = O[(size_t)k * n2];
+                        pa[r] = k >= 1 ? A[(size_t)(k - 1) * n2] : 0.0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = SEG - 1; r >= 0; --r) {
+                const int k = SEG * m + r;
+                if (k < KM && k < kz) {
+                    const double wkm1 = k >= 1 ? dev::floor_akt(ca[r], k - 1, n, hmix, dwm) : 0.0;
+                    double aa, bb, cc;
+                    dev::tri_row(k, kz, wkm1, wk_k, odzp, odzt, dts, aa, bb, cc);
+                    const double g0 = 1.0 / (bb - cc * en);
+                    en = aa * g0;
+                    fn = (cd[r] + cc * fn) * g0;
+                    wk_k = wkm1;
+                }
+            }
+            if (m > 0) {
+                cke[m] = en;
+                ckf[m] = fn;
             }
         }
     }
 
-    // substitution from the top, tracer_9, and the convadj stability scan
+    // ---- substitution from the top, tracer_9, convadj stability scan
     const bool conv = flags & LICOM_STEP_CONVADJ;
     const double *C = g.eosc, *to = g.to, *so = g.so;
     double xprev = 0.0, Tp = 0.0, Sp = 0.0;
-    bool unstable = false;
-    double *TT = at + n * n3, *BB = atb + n * (n3 + n2) + n2;
+    uint64_t mask = 0;
+    double qd[SEG], qa[SEG], qt[SEG], qw = 0.0;
+    auto load_seg = [&](int m) {
+#pragma unroll
+        for (int r = 0; r < SEG; ++r) {
+            const int k = SEG * m + r;
+            qd[r] = qa[r] = qt[r] = 0.0;
+            if (k < KM && k < km && act) {
+                qd[r] = O[(size_t)k * n2];
+                qt[r] = TT[(size_t)k * n2];
+                if (k >= 1 && k < kz) qa[r] = A[(size_t)(k - 1) * n2];
+            }
+        }
+        const int kb = SEG * m + SEG - 1;
+        qw = (kb < KM && kb < kz - 1) ? A[(size_t)kb * n2] : 0.0;
+    };
+    load_seg(0);
 #pragma unroll
     for (int m = 0; m < NSEG; ++m) {
+        double cd[SEG], ca[SEG], ct[SEG];
+#pragma unroll
+        for (int r = 0; r < SEG; ++r) {
+            cd[r] = qd[r];
+            ca[r] = qa[r];
+            ct[r] = qt[r];
+        }
+        const double cw = qw;
+        if (m + 1 < NSEG) load_seg(m + 1);
         const int top = SEG * m;
-        double se[SEG], sf[SEG], sd[SEG];
+        double se[SEG], sf[SEG];
         {
-            double ee = cke[m + 1], ff = ckf[m + 1], wk_k = 0.0;
+            double ee = cke[m + 1], ff = ckf[m + 1];
+            double wk_k = dev::floor_akt(cw, top + SEG - 1, n, hmix, dwm);
 #pragma unroll
             for (int r = SEG - 1; r >= 0; --r) {
                 const int k = top + r;
-                se[r] = sf[r] = sd[r] = 0.0;
+                se[r] = sf[r] = 0.0;
                 if (k < KM && k < kz) {
-                    if (r == SEG - 1 && k < kz - 1) wk_k = wkc(k);
-                    const double d = O[(size_t)k * n2 + q];
-                    const double wkm1 = k >= 1 ? wkc(k - 1) : 0.0;
+                    const double wkm1 = k >= 1 ? dev::floor_akt(ca[r], k - 1, n, hmix, dwm) : 0.0;
                     double aa, bb, cc;
                     dev::tri_row(k, kz, wkm1, wk_k, odzp, odzt, dts, aa, bb, cc);
                     const double g0 = 1.0 / (bb - cc * ee);
                     ee = aa * g0;
-                    ff = (d + cc * ff) * g0;
+                    ff = (cd[r] + cc * ff) * g0;
                     se[r] = ee;
                     sf[r] = ff;
-                    sd[r] = d;
                     wk_k = wkm1;
                 }
             }
@@ -600,22 +750,21 @@ __global__ __launch_bounds__(256) void k_solve(KGrid g, KPar p, int flags, doubl
         for (int r = 0; r < SEG; ++r) {
             const int k = top + r;
             if (k < KM && k < km) {
-                const size_t c = (size_t)k * n2 + q;
+                const size_t c = (size_t)k * n2;
                 const bool wet = k < kz;
-                const double o = wet ? sd[r] : (act ? O[c] : 0.0);
+                const double o = cd[r];
                 double x = o;
                 if (wet) x = (k == 0) ? (se[0] * stf + sf[0]) * 1.0 : (se[r] * xprev + sf[r]) * 1.0;
                 xprev = x;
                 const double vk = wet ? 1.0 : 0.0;
                 if (act) {
-                    const double told = TT[c];
                     if (dt_diff) {
                         double v = (x - o) / dts * vk;
                         if (k == 0) v = v - stf * 0.5 * odzp[0];
-                        dt_diff[n * n3 + c] = v;
+                        dt_diff[n * n3 + q + c] = v;
                     }
-                    if (tend) tend[n * n3 + c] = (x - told) / dts * vk;
-                    if (conv && dt_conv) dt_conv[n * n3 + c] = 0.0;
+                    if (tend) tend[n * n3 + q + c] = (x - ct[r]) / dts * vk;
+                    if (conv && dt_conv) dt_conv[n * n3 + q + c] = 0.0;
                     TT[c] = x;
                     BB[c] = x;
                 }
@@ -625,7 +774,7 @@ __global__ __launch_bounds__(256) void k_solve(KGrid g, KPar p, int flags, doubl
                     if (k >= 1 && k <= kz - 1) {
                         const double up = dev::dens(C, km, k, Tk - to[k], Sk - so[k]);
                         const double lo = dev::dens(C, km, k, Tp - to[k], Sp - so[k]);
-                        if (lo > up) unstable = true;
+                        if (lo > up) mask |= 1ull << k;
                     }
                     Tp = Tk;
                     Sp = Sk;
@@ -634,24 +783,19 @@ __global__ __launch_bounds__(256) void k_solve(KGrid g, KPar p, int flags, doubl
         }
     }
 
-    // convadj_cu (convadj.cpp:52-275) for statically unstable columns, on
-    // the state just written (at == atb == tracer_9's values, cache-resident)
-    if (conv && act && n == 0 && unstable) {
-        atomicAdd(mix_count, 1);
-        double rhoup[KMAXF], rholo[KMAXF];
-        const dev::Strided Tg{at + q, n2}, Sg{at + n3 + q, n2};
-        dev::convect_column(C, to, so, g.dzp, km, kz, Tg, Sg, rhoup, rholo);
-        for (int nn = 0; nn < 2; ++nn) {
-            double *Tn = at + nn * n3, *Bn = atb + nn * (n3 + n2) + n2;
-            for (int k = 0; k < kz; ++k) {
-                const size_t c = (size_t)k * n2 + q;
-                const double y = Tn[c], b = Bn[c];
-                if (y != b) {
-                    const double dc = (y - b) / dts * 1.0;
-                    if (dt_conv) dt_conv[nn * n3 + c] = dc;
-                    if (tend) tend[nn * n3 + c] += dc;
-                    Bn[c] = y;
-                }
+    // ---- convadj for statically unstable columns (both lanes of the pair)
+    if (conv && act && mask != 0) {
+        if (n == 0) atomicAdd(mix_count, 1);
+        int lo, hi;
+        convect_sparse(C, to, so, g.dzp, km, kz, mask, n, TT, n2, lo, hi);
+        for (int l = lo; l <= hi; ++l) {
+            const size_t c = (size_t)l * n2;
+            const double y = TT[c], b = BB[c];
+            if (y != b) {
+                const double dc = (y - b) / dts * 1.0;
+                if (dt_conv) dt_conv[n * n3 + q + c] = dc;
+                if (tend) tend[n * n3 + q + c] += dc;
+                BB[c] = y;
             }
         }
     }

@@ -431,6 +431,26 @@ static void del2_sums(const orc_grid *g, const double *X, int k, int j, int i,
     *cc_out = cc;
 }
 
+void orc_advect(const orc_grid *g, double dts, const double *T, const double *uw,
+                const double *vs, const double *ws, double *adv_tt) {
+    const size_t n3 = (size_t)g->imt * g->jmt * g->km;
+    double *b = (double *)calloc(6 * n3, sizeof(double));
+    tspas(g, dts, T, uw, vs, ws, b, b + n3, b + 2 * n3, adv_tt, b + 3 * n3, b + 4 * n3, b + 5 * n3);
+    free(b);
+}
+
+void orc_del2_level(const orc_grid *g, const double *X, int k, int use_ahf, double ah,
+                    double *out) {
+    const int imt = g->imt, jmt = g->jmt;
+    for (int j = 2; j < jmt - 2; ++j)
+        for (int i = 2; i < imt - 2; ++i) {
+            double h, cc;
+            del2_sums(g, X, k, j, i, &h, &cc);
+            const double f = use_ahf ? g->ahf[I2(j, i)] : ah;
+            out[I2(j, i)] = f * (cc * X[I2(j, i)] + h);
+        }
+}
+
 /* ------------------------------------------------------------ the step */
 void orc_tracer_step(const orc_grid *g, const orc_params *p, orc_fields *f,
                      orc_diag *d) {

@@ -87,6 +87,14 @@ void orc_tracer_step(const orc_grid *g, const orc_params *p, orc_fields *f,
 void orc_convadj(const orc_grid *g, const orc_params *p, orc_fields *f,
                  orc_diag *d);
 /* Stand-alone pieces used by the invariant tests. */
+/* tspas advection only (advection_tracer_2/3) of one tracer T [km][jmt][imt]
+ * with face velocities uw, vs and vertical velocity ws; writes adv_tt. */
+void orc_advect(const orc_grid *g, double dts, const double *T, const double *uw,
+                const double *vs, const double *ws, double *adv_tt);
+/* One 5-point del2 pass hdtk = fac*(cc*X + sum c*X_nbr) on the physical cells
+ * of level k, fac = AH (del2 build) or AHF (first del4 pass). */
+void orc_del2_level(const orc_grid *g, const double *X, int k, int use_ahf, double ah,
+                    double *out);
 void orc_invtri_column(int kz, const double *wkc_col, const double *odzt,
                        const double *odzp, double dts, double stf, double vit0,
                        double *x /* in: d, out: solution */);

@@ -70,6 +70,9 @@ def lib():
         _LIB.orc_dens.restype = ctypes.c_double
         _LIB.orc_halo_fill_f64.argtypes = [_DP, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _LIB.orc_nthreads.restype = ctypes.c_int
+        _LIB.orc_advect.argtypes = [ctypes.POINTER(OrcGrid), ctypes.c_double, _DP, _DP, _DP, _DP, _DP]
+        _LIB.orc_del2_level.argtypes = [ctypes.POINTER(OrcGrid), _DP, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_double, _DP]
     return _LIB
 
 
@@ -113,6 +116,18 @@ class Oracle:
         self.g = gr
         lib().orc_derive(ctypes.byref(gr))
         self.area_t = gr.area_t
+
+    def advect(self, T, uw, vs, ws, dts):
+        adv = np.zeros_like(T)
+        args = [np.ascontiguousarray(a, np.float64) for a in (T, uw, vs, ws)]
+        lib().orc_advect(ctypes.byref(self.g), float(dts), *[_d(a) for a in args], _d(adv))
+        return adv
+
+    def del2_level(self, X, k, use_ahf=False, ah=1.0):
+        out = np.zeros_like(X)
+        Xc = np.ascontiguousarray(X, np.float64)
+        lib().orc_del2_level(ctypes.byref(self.g), _d(Xc), int(k), int(use_ahf), float(ah), _d(out))
+        return out
 
     def params(self, **over):
         p = dict(self.case.params)
