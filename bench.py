@@ -55,14 +55,91 @@ def kernel_algorithmic_bytes(name, case):
     return per_cell.get(name, 0) * cells
 
 
+COPY_BYTES = 4 << 30            # size of the calibration copy (read and written)
+
+
+def traffic_file(workload):
+    return os.path.join(ROOT, "profiles", f"hbm_traffic_tracer_step_{workload}.json")
+
+
 def traffic_from_profiles(kernel, workload):
-    """HBM bytes per launch from committed rocprofv3 --pmc summaries
-    (profiles/pmc_<kernel>_<workload>.json), or None."""
-    p = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}.json")
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload, or None."""
+    p = traffic_file(workload)
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        k = json.load(f).get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
+
+
+def short_kernel_name(name):
+    """'void licom::(anonymous namespace)::k_solve<55>(KGrid, ...)' -> 'k_solve'."""
+    n = name.replace("(anonymous namespace)", "")
+    n = n.split("(")[0]
+    n = n.split("<")[0]
+    return n.split("::")[-1].split()[-1]
+
+
+def read_rocprof_csv(directory, suffix):
+    """Rows of every rocprofv3 CSV under `directory` whose name ends in `suffix`."""
+    import csv
+    rows = []
+    for dp, _, files in os.walk(directory):
+        for fn in sorted(files):
+            if fn.endswith(suffix):
+                with open(os.path.join(dp, fn), newline="") as f:
+                    rows.extend(csv.DictReader(f))
+    if not rows:
+        raise SystemExit(f"no *{suffix} under {directory}")
+    return rows
+
+
+def summarize_pmc(fetch_dir, write_dir, workload, commands):
+    """Per-kernel HBM traffic from two rocprofv3 --pmc runs (FETCH_SIZE, then
+    WRITE_SIZE; KiB per dispatch), each scaled by what the same counter reads
+    for k_stream_copy moving COPY_BYTES each way.  Writes the summary json."""
+    def per_kernel(d, counter):
+        acc = {}
+        for r in read_rocprof_csv(d, "counter_collection.csv"):
+            if r.get("Counter_Name") != counter:
+                continue
+            k = short_kernel_name(r["Kernel_Name"])
+            acc.setdefault(k, {})
+            acc[k].setdefault(r.get("Dispatch_Id", r.get("Correlation_Id")), 0.0)
+            acc[k][r.get("Dispatch_Id", r.get("Correlation_Id"))] += float(r["Counter_Value"])
+        return {k: sum(v.values()) / len(v) for k, v in acc.items()}
+    fetch, write = per_kernel(fetch_dir, "FETCH_SIZE"), per_kernel(write_dir, "WRITE_SIZE")
+    cf = COPY_BYTES / (fetch["k_stream_copy"] * 1024.0)
+    cw = COPY_BYTES / (write["k_stream_copy"] * 1024.0)
+    kernels = {}
+    for k in sorted(set(fetch) & set(write)):
+        b = fetch[k] * 1024.0 * cf + write[k] * 1024.0 * cw
+        kernels[k] = {"fetch_kib": round(fetch[k], 1), "write_kib": round(write[k], 1),
+                      "hbm_bytes_per_launch": round(b)}
+    out = {"workload": workload, "commands": commands,
+           "calibration": {"kernel": "k_stream_copy", "bytes_each_way": COPY_BYTES,
+                           "fetch_factor": round(cf, 4), "write_factor": round(cw, 4)},
+           "kernels": kernels}
+    with open(traffic_file(workload), "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
+
+
+def summarize_trace(trace_dir, workload, command):
+    """Markdown table of a rocprofv3 --kernel-trace --stats run."""
+    rows = read_rocprof_csv(trace_dir, "kernel_stats.csv")
+    lines = [f"# Kernel times, tracer step, {workload}", "", "Command (repo root):", "",
+             "```", command, "```", "",
+             "| kernel | calls | avg ms | min ms | max ms | share % |", "|---|---|---|---|---|---|"]
+    for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
+        lines.append(f"| {short_kernel_name(r['Name'])} | {r['Calls']} | "
+                     f"{float(r['AverageNs']) / 1e6:.3f} | {float(r['MinNs']) / 1e6:.3f} | "
+                     f"{float(r['MaxNs']) / 1e6:.3f} | {float(r['Percentage']):.1f} |")
+    p = os.path.join(ROOT, "profiles", f"kernel_trace_tracer_step_{workload}.md")
+    with open(p, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
 
 
 def cpu_baseline(args):
@@ -104,7 +181,19 @@ def main():
     ap.add_argument("--cpu-band", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write the HBM traffic summary from two rocprofv3 --pmc runs")
+    ap.add_argument("--summarize-trace", metavar="TRACE_DIR",
+                    help="write the kernel-time table from a rocprofv3 --stats run")
+    ap.add_argument("--command", action="append", default=[],
+                    help="profiling command(s) recorded in the summary")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc, args.workload, args.command)
+        return
+    if args.summarize_trace:
+        summarize_trace(args.summarize_trace, args.workload, " && ".join(args.command))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the tracer step would shard by 2-D horizontal tiles; see DESIGN.md)",
@@ -115,6 +204,7 @@ def main():
     import licom_synth as ls
     import licom_tracer as lt
 
+    copy_gbs = lt.stream_copy_bandwidth(COPY_BYTES)
     t0 = time.time()
     case = ls.make_case(args.workload)
     print(f"[bench] generated {args.workload} {case.nx}x{case.ny}x{case.km} in "
@@ -183,6 +273,7 @@ def main():
         "roofline": roof,
         "step_roofline": step_roof,
         "kernel_ms": {k: round(v[0] / v[1], 4) for k, v in kt.items()},
+        "stream_copy_gbs": round(copy_gbs, 1),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
