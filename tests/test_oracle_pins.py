@@ -193,6 +193,64 @@ def test_del2_constant_field_and_zero_net_flux():
     assert abs(tot) <= 1e-12 * scale
 
 
+# ------------------------------------------------------------ upwell / div
+def _faces(c):
+    """tspas face velocities (advection_tracer_1_0) of the step's inputs."""
+    s0, g, oncc = c.state, c.grid, c.params["oncc"]
+    wkd = 0.5 * (s0["utf"] * oncc) + 0.5 * s0["utl"]
+    wkb = 0.5 * (s0["vtf"] * oncc) + 0.5 * s0["vtl"]
+    uw = np.zeros_like(wkd)
+    vs = np.zeros_like(wkb)
+    uw[:, 1:, :] = (wkd[:, :-1, :] + wkd[:, 1:, :]) * g["htw"][1:, :] * 0.25
+    vs[:, :, :-1] = (wkb[:, :, :-1] + wkb[:, :, 1:]) * g["hts"][:, :-1] * 0.25
+    return uw, vs
+
+
+@pytest.mark.parametrize("net_transport", [False, True])
+def test_w_closes_discrete_continuity_with_the_advective_faces(net_transport):
+    """The w of upwell (and operators_div) is the one that makes the flux
+    divergence of tspas's own face velocities vanish in 3-D, with the
+    barotropic correction work*ohbt taking out any net column transport so
+    that w reaches zero at the sea floor."""
+    c = ls.make_case("C1", nx=32, ny=24, km=10, seed=3, depth_min=-2000.0, sigma=1.5)
+    if net_transport:
+        rng = np.random.default_rng(4)
+        for f in ("utf", "utl", "vtf", "vtl"):
+            c.state[f] += rng.normal(0.0, 0.05, c.state[f].shape)
+    o = orc.Oracle(c)
+    st = c.copy_state()
+    o.step(st, convadj=0)
+    ws = st["ws"]
+    uw, vs = _faces(c)
+    div = np.zeros_like(uw)
+    div[:, :, :-1] = uw[:, :, 1:] - uw[:, :, :-1]
+    div[:, 1:, :] += vs[:, 1:, :] - vs[:, :-1, :]
+    tr = c.grid["tarea"].copy()
+    tr[tr > 0] = 1.0 / tr[tr > 0]
+    div *= 2.0 * tr
+    dz, vit, kmt = o.d["dzp"], o.d["vit"], o.kmt
+    work = -(dz[:, None, None] * div * vit).sum(0)
+    corr = work * o.d["ohbt"]
+    if not net_transport:
+        assert np.abs(work[2:-2, 2:-2]).max() <= 1e-12 * np.abs(div).max() * dz.sum()
+    scale = np.abs(ws).max()
+    assert scale > 0
+    for k in range(c.km - 1):
+        wet = (vit[k + 1] > 0)[2:-2, 2:-2]          # interface k+1 inside the column
+        got = (ws[k + 1] - ws[k])[2:-2, 2:-2][wet]
+        ref = (dz[k] * (corr + div[k]))[2:-2, 2:-2][wet]
+        if k == 0:
+            got = ws[1][2:-2, 2:-2][wet]              # ws(0) is not part of the recursion
+        assert np.abs(got - ref).max() <= 1e-11 * scale
+    # sea floor: the unmasked continuation of the recursion below the last
+    # wet level is zero
+    jj, ii = np.nonzero(kmt[2:-2, 2:-2] > 1)
+    jj, ii = jj + 2, ii + 2
+    kb = kmt[jj, ii] - 1
+    floor = ws[kb, jj, ii] + dz[kb] * (corr[jj, ii] + div[kb, jj, ii])
+    assert np.abs(floor).max() <= 1e-11 * scale
+
+
 # -------------------------------------------------------------------- tspas
 def _adv_inputs(c, u=0.0, v=0.0, w=0.0):
     km, jmt, imt = c.km, c.jmt, c.imt
