@@ -23,49 +23,42 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "licom3-hip_amd"))
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, vendor peak
-STEP_BYTES_PER_UPDATE = {"fp64": 128.0}
+STEP_BYTES_PER_UPDATE = {"fp64": 128.0, "mixed": 80.0}
 
 
-def kernel_algorithmic_bytes(name, case):
+def kernel_algorithmic_bytes(name, case, variant="fp64"):
     """Compulsory HBM bytes of one launch of `name` on `case`: every array
     the kernel must read or write, counted once, over the cells it covers.
     Per interior cell (nx*ny*km), both tracers; 2-D terms neglected."""
     cells = case.nx * case.ny * case.km
     del4 = case.params["hmix"] == 4
-    per_cell = {
-        # fused path
-        # utf vtf utl vtl -> u_wface v_sface ws
-        "k_prelude": 8 * 4 + 8 * 3,
-        # at atb akt (x2) u_wface v_sface ws -> ori ax ay az dz (x2), penetrate
-        "k_march": 8 * (6 + 3) + 8 * (10 + 1),
-        # ori akt at (x2) -> at atb tend dt_diff dt_conv (x2)
-        "k_solve": 8 * 6 + 8 * 10,
-        # utf vtf read and written
-        "k_side": 8 * 4,
-        # per-stage path
-        # at(2) atb(2) akt(2) uw vs ws flags(2x1B) [d2tk(2)] -> ori(2) ax ay az dz (x2) pen
-        "k_correct": 8 * (2 + 2 + 2 + 2 + 1) + 2 + (16 if del4 else 0) + 8 * (2 + 8 + 1),
-        # ori(2) akt(2) at(2) utf vtf -> at(2) atb(2) tend dt_diff dt_conv (x2) utf vtf
-        "k_column": 8 * (2 + 2 + 2 + 2) + 8 * (2 + 2 + 6 + 2),
-        "k_predict": 8 * (2 + 2 + 1) + 2,
-        "k_faces": 8 * 4 + 8 * 2,
-        "k_ws": 8 * 4 + 8,
-        "k_d2tk": 8 * 2 + 8 * 2,
-    }
+    if variant == "mixed":
+        per_cell = {
+            # utf vtf utl vtl (f32) -> u_wface v_sface wka (f32); wka -> ws (f32)
+            "k_prelude": 4 * 4 + 4 * 3 + 4 + 4,
+            # at atb (f64 x2) akt (f32 x2) faces ws (f32) -> ori (f64 x2), ax ay az dz (f32 x2), pen
+            "k_march": 8 * 4 + 4 * 2 + 4 * 3 + 8 * 2 + 4 * 8 + 4,
+            # ori at (f64 x2) akt (f32 x2) -> at atb (f64 x2) tend dt_diff dt_conv (f32 x2)
+            "k_solve": 8 * 4 + 4 * 2 + 8 * 4 + 4 * 6,
+            # utf vtf (f32) read and written
+            "k_side": 4 * 4,
+        }
+        return per_cell.get(name, 0) * cells
     return per_cell.get(name, 0) * cells
 
 
 COPY_BYTES = 4 << 30            # size of the calibration copy (read and written)
 
 
-def traffic_file(workload):
-    return os.path.join(ROOT, "profiles", f"hbm_traffic_tracer_step_{workload}.json")
+def traffic_file(workload, variant="fp64"):
+    tag = "tracer_step" if variant == "fp64" else "tracer_step_mixed"
+    return os.path.join(ROOT, "profiles", f"hbm_traffic_{tag}_{workload}.json")
 
 
-def traffic_from_profiles(kernel, workload):
+def traffic_from_profiles(kernel, workload, variant="fp64"):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary of this workload, or None."""
-    p = traffic_file(workload)
+    p = traffic_file(workload, variant)
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -95,7 +88,7 @@ def read_rocprof_csv(directory, suffix):
     return rows
 
 
-def summarize_pmc(fetch_dir, write_dir, workload, commands):
+def summarize_pmc(fetch_dir, write_dir, workload, commands, variant="fp64"):
     """Per-kernel HBM traffic from two rocprofv3 --pmc runs (FETCH_SIZE, then
     WRITE_SIZE; KiB per dispatch), each scaled by what the same counter reads
     for k_stream_copy moving COPY_BYTES each way.  Writes the summary json."""
@@ -105,6 +98,8 @@ def summarize_pmc(fetch_dir, write_dir, workload, commands):
             if r.get("Counter_Name") != counter:
                 continue
             k = short_kernel_name(r["Kernel_Name"])
+            if k.endswith("_mx"):      # the mixed form's kernels, named as the FP64 ones
+                k = k[:-3]
             acc.setdefault(k, {})
             acc[k].setdefault(r.get("Dispatch_Id", r.get("Correlation_Id")), 0.0)
             acc[k][r.get("Dispatch_Id", r.get("Correlation_Id"))] += float(r["Counter_Value"])
@@ -121,7 +116,7 @@ def summarize_pmc(fetch_dir, write_dir, workload, commands):
            "calibration": {"kernel": "k_stream_copy", "bytes_each_way": COPY_BYTES,
                            "fetch_factor": round(cf, 4), "write_factor": round(cw, 4)},
            "kernels": kernels}
-    with open(traffic_file(workload), "w") as f:
+    with open(traffic_file(workload, variant), "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out, indent=1))
 
@@ -170,12 +165,58 @@ def cpu_baseline(args):
                       f"workload, median of {len(times)} steps after 1 warm-up"}
 
 
+def run_variant(model, args, torch, lt, case, updates):
+    """Warm-up, then K timed steps of one precision form; per-kernel HIP-event
+    times on the library's stream over the timed steps."""
+    flags = lt.LICOM_STEP_SIDE_EFFECTS | lt.LICOM_STEP_CONVADJ
+    v = model.precision
+    for _ in range(args.warmup):
+        model.tracer(0, flags)
+    model.sync()
+    torch.cuda.synchronize()
+    print(f"[bench] {v}: warm-up done, timing {args.steps} steps", file=sys.stderr, flush=True)
+    if not args.no_kernel_timing:
+        model.kernel_times()       # drain
+        model.set_profiling(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        model.tracer(0, flags)
+    model.sync()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    kt = {}
+    if not args.no_kernel_timing:
+        kt = model.kernel_times()
+        model.set_profiling(False)
+    sec = dt / args.steps
+    roof = None
+    if kt:
+        dom = max(kt, key=lambda k: kt[k][0])
+        avg_ms = kt[dom][0] / kt[dom][1]
+        alg = kernel_algorithmic_bytes(dom, case, v)
+        ach = alg / (avg_ms * 1e-3) / 1e9
+        tr = traffic_from_profiles(dom, args.workload, v)
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": tr, "algorithmic_bytes": alg, "avg_launch_ms": round(avg_ms, 4)}
+    sb = STEP_BYTES_PER_UPDATE[v]
+    step_roof = {"bytes_per_update": sb, "achieved": round(sb * updates / sec / 1e9, 1),
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(sb * updates / sec / 1e9 / HBM_PEAK_GBS, 4)}
+    print(f"[bench] {v}: {1000 * sec:.2f} ms/step", file=sys.stderr, flush=True)
+    return {"value": updates / sec, "ms_per_step": 1000.0 * sec, "roofline": roof,
+            "step_roofline": step_roof, "kernel_ms": {k: round(t[0] / t[1], 4) for k, t in kt.items()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--variant", default="fp64", choices=["fp64"])
+    ap.add_argument("--variant", default="both", choices=["fp64", "mixed", "both"],
+                    help="both: the FP64 drop-in is the headline value, the mixed form "
+                         "and its error against FP64 are reported beside it")
     ap.add_argument("--workload", default="C3")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-band", type=int, default=128)
@@ -189,7 +230,8 @@ def main():
                     help="profiling command(s) recorded in the summary")
     args = ap.parse_args()
     if args.summarize_pmc:
-        summarize_pmc(*args.summarize_pmc, args.workload, args.command)
+        summarize_pmc(*args.summarize_pmc, args.workload, args.command,
+                      "mixed" if args.variant == "mixed" else "fp64")
         return
     if args.summarize_trace:
         summarize_trace(args.summarize_trace, args.workload, " && ".join(args.command))
@@ -209,49 +251,28 @@ def main():
     case = ls.make_case(args.workload)
     print(f"[bench] generated {args.workload} {case.nx}x{case.ny}x{case.km} in "
           f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
-    model = lt.TracerModel(case, diagnostics=True)
+    variants = ["fp64", "mixed"] if args.variant == "both" else [args.variant]
+    models = {v: lt.TracerModel(case, diagnostics=True, precision=v) for v in variants}
     case.state = None          # free host copies
-    flags = lt.LICOM_STEP_SIDE_EFFECTS | lt.LICOM_STEP_CONVADJ
-    for _ in range(args.warmup):
-        model.tracer(0, flags)
-    model.sync()
-    torch.cuda.synchronize()
-    print(f"[bench] warm-up done, timing {args.steps} steps", file=sys.stderr, flush=True)
-    if not args.no_kernel_timing:
-        model.kernel_times()       # drain
-        model.set_profiling(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        model.tracer(0, flags)
-    model.sync()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    kt = {}
-    if not args.no_kernel_timing:
-        kt = model.kernel_times()
-        model.set_profiling(False)
-    ms_step = 1000.0 * dt / args.steps
     updates = case.nx * case.ny * case.km * 2
-    value = updates / (dt / args.steps)
+    runs = {v: run_variant(models[v], args, torch, lt, case, updates) for v in variants}
 
-    roof = None
-    if kt:
-        dom = max(kt, key=lambda k: kt[k][0])
-        avg_ms = kt[dom][0] / kt[dom][1]
-        alg = kernel_algorithmic_bytes(dom, case)
-        ach = alg / (avg_ms * 1e-3) / 1e9
-        tr = traffic_from_profiles(dom, args.workload)
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": tr, "algorithmic_bytes": alg, "avg_launch_ms": round(avg_ms, 4)}
-    step_bytes = STEP_BYTES_PER_UPDATE[args.variant] * updates
-    step_roof = {"bytes_per_update": STEP_BYTES_PER_UPDATE[args.variant],
-                 "achieved": round(step_bytes / (dt / args.steps) / 1e9, 1), "peak": HBM_PEAK_GBS,
-                 "unit": "GB/s",
-                 "frac": round(step_bytes / (dt / args.steps) / 1e9 / HBM_PEAK_GBS, 4)}
-    model.close()
-    del model
+    mixed_err = None
+    if len(variants) == 2:
+        # max |dT|, |dS| of the mixed form against the FP64 drop-in (bit-identical
+        # to the oracle) after the same warmup + steps, over wet interior cells
+        ref, mix = models["fp64"].state["at"], models["mixed"].state["at"]
+        kmt = torch.from_numpy(case.grid["kmt"]).to(ref.device)
+        wet = (torch.arange(case.km, device=ref.device)[:, None, None] < kmt[None])[:, 2:-2, 2:-2]
+        err = []
+        for n in range(2):
+            d = (mix[n, :, 2:-2, 2:-2] - ref[n, :, 2:-2, 2:-2]).abs()
+            err.append(float(torch.where(wet, d, torch.zeros_like(d)).max()))
+        mixed_err = {"max_dT_K": err[0], "max_dS_psu": err[1] * 1000.0,
+                     "steps": args.warmup + args.steps, "against": "FP64 drop-in"}
+    for m in models.values():
+        m.close()
+    del models
     torch.cuda.empty_cache()
 
     cpu = None
@@ -259,23 +280,33 @@ def main():
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args)
 
+    main_v = variants[0]
+    r = runs[main_v]
     out = {
         "metric": "tracer cell-updates/s at 3600x2302x55, FP64 vs mixed; p50/p99 request latency",
-        "value": value, "unit": "tracer cell-updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "value": r["value"], "unit": "tracer cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if main_v == "fp64" else "f64/f32",
+        "data": "synthetic",
         "config": {"workload": f"{args.workload} {case.nx}x{case.ny}x{case.km} tracer step "
                                f"(tspas, {'del4' if case.params['hmix'] == 4 else 'del2'}, "
-                               f"implicit vertical diffusion, convadj), FP64 drop-in, "
+                               f"implicit vertical diffusion, convadj), "
+                               f"{'FP64 drop-in' if main_v == 'fp64' else 'mixed precision'}, "
                                f"diagnostics on",
                    "nx": case.nx, "ny": case.ny, "km": case.km, "ntra": 2,
-                   "dts": case.params["dts"], "variant": args.variant},
-        "roofline": roof,
-        "step_roofline": step_roof,
-        "kernel_ms": {k: round(v[0] / v[1], 4) for k, v in kt.items()},
+                   "dts": case.params["dts"], "variant": main_v},
+        "roofline": r["roofline"],
+        "step_roofline": r["step_roofline"],
+        "kernel_ms": r["kernel_ms"],
         "stream_copy_gbs": round(copy_gbs, 1),
         "cpu_baseline": cpu,
     }
+    if "mixed" in runs and main_v != "mixed":
+        rm = runs["mixed"]
+        out["mixed"] = {"value": rm["value"], "ms_per_step": rm["ms_per_step"], "dtype": "f64/f32",
+                        "speedup_vs_fp64": rm["value"] / r["value"],
+                        "roofline": rm["roofline"], "step_roofline": rm["step_roofline"],
+                        "kernel_ms": rm["kernel_ms"], "error": mixed_err}
     print(json.dumps(out), flush=True)
 
 

@@ -145,6 +145,84 @@ int licom_tracer_set_stream(licom_ctx *ctx, void *hip_stream);
 int licom_tracer_step(licom_ctx *ctx, const licom_fields *f,
                       const licom_diag *d, int flags);
 
+/* Mixed-precision form of the step: the tracer state at/atb and every
+ * 2-D field stay FP64; velocities, ws, akt and the 3-D diagnostics are FP32.
+ * Advective fluxes, the del4/del2 stencils and the explicit vertical
+ * diffusion run in FP32 on differences formed in FP64 (anomalies about the
+ * cell's own value), and the increments are added to the FP64 state; the
+ * implicit vertical solve, convadj and the SSSNORM sum run in FP64.  Results
+ * differ from the FP64 form at roundoff level of the fluxes (and where that
+ * flips a limiter decision).  No reference counterpart: a variant of
+ * tracer_ (tracer.cpp:1701-1903) for callers that keep FP32 dynamics. */
+typedef struct licom_fields_mixed {
+    double *at;               /* [2][km][jmt][imt]        in/out (FP64)       */
+    double *atb;              /* [2][km+1][jmt][imt]      in/out (FP64)       */
+    float *akt;               /* [2][km][jmt][imt]        in (k=0 floor out)  */
+    float *utf, *vtf;         /* [km][jmt][imt]           in (scaled out)     */
+    const float *utl, *vtl;   /* [km][jmt][imt]           in                  */
+    float *ws;                /* [km+1][jmt][imt]         out (plane 0 in)    */
+    double *h0f;              /* [jmt][imt]               in (scaled out)     */
+    const double *h0l;        /* [jmt][imt]               in                  */
+    const double *swv, *nswv, *sst, *seaice, *sss, *fresh; /* [jmt][imt] or NULL */
+    const double *restore;    /* [2][km][jmt][imt] or NULL                    */
+} licom_fields_mixed;
+
+typedef struct licom_diag_mixed {
+    float *ax, *ay, *az, *dz, *tend, *dt_diff, *dt_conv; /* [2][km][jmt][imt]   */
+    float *penetrate;         /* [km][jmt][imt]                               */
+    double *net;              /* [2][jmt][imt]                                */
+    double *fw_norm2;         /* [1]                                          */
+} licom_diag_mixed;
+
+/* The step in the mixed-precision form; flags as licom_tracer_step except
+ * LICOM_STEP_VTL_SCRATCH and LICOM_STEP_STAGED (LICOM_ENOTSUP).  Needs an
+ * even imt and 16-byte aligned at/atb. */
+int licom_tracer_step_mixed(licom_ctx *ctx, const licom_fields_mixed *f,
+                            const licom_diag_mixed *d, int flags);
+
+/* ---- Batched ensemble engine (mixed-precision form) --------------------
+ * Advances many members, each a context of its own grid (any size), in one
+ * launch per stage and step: every stage kernel walks a work list of
+ * (member, block) items; the k-march runs persistent workgroups over
+ * (member, tile, tracer) items.  A member's results are bit-identical
+ * whether it runs alone (licom_tracer_step_mixed), in any batch, or under
+ * any launch-shape choice.  No reference counterpart (the reference runs one
+ * domain per rank). */
+typedef struct licom_batch licom_batch;
+
+/* Launch-shape choices of one member (mapping only, never the arithmetic). */
+typedef struct licom_member_tuning {
+    int facediv_shape;        /* faces/divergence workgroup: 0: 64x4 cells,
+                                 1: 32x8, 2: 128x2                          */
+    int solve_shape;          /* column solve workgroup: 0: 4 rows x 32
+                                 columns, 1: 2 x 64, 2: 1 x 128             */
+} licom_member_tuning;
+
+/* members[m] with its fields[m] and diags[m] (diags may be NULL, or an entry
+ * with NULL pointers); tuning may be NULL (shape 0 everywhere).  All members
+ * must live on the current device.  The batch owns a stream of its own
+ * (default priority) unless licom_batch_set_stream is called. */
+int licom_batch_create(licom_ctx *const *members, const licom_fields_mixed *fields,
+                       const licom_diag_mixed *diags, const licom_member_tuning *tuning,
+                       int n, licom_batch **out);
+int licom_batch_set_stream(licom_batch *b, void *hip_stream);
+/* Workgroups of the persistent k-march; 0 = one workgroup per item. */
+int licom_batch_set_march_grid(licom_batch *b, int workgroups);
+/* Enqueues nsteps steps of every member (flags: CONVADJ, SIDE_EFFECTS). */
+int licom_batch_step(licom_batch *b, int nsteps, int flags);
+/* Captures nsteps steps into a HIP graph (replacing any previous one) and
+ * launches nothing; licom_batch_replay launches the graph. */
+int licom_batch_capture(licom_batch *b, int nsteps, int flags);
+int licom_batch_replay(licom_batch *b);
+int licom_batch_sync(licom_batch *b);
+void licom_batch_destroy(licom_batch *b);
+
+/* Autotuner: times every licom_member_tuning candidate on this member (reps
+ * steps each, HIP events, a one-member batch) and returns the fastest with
+ * its milliseconds per step.  Advances the member's state. */
+int licom_autotune_member(licom_ctx *ctx, const licom_fields_mixed *f, const licom_diag_mixed *d,
+                          int reps, licom_member_tuning *best, double *best_ms);
+
 /* Convective adjustment alone: convadj_() of convadj.cpp:290-298 over every
  * column, ghost columns included. */
 int licom_convadj(licom_ctx *ctx, const licom_fields *f, const licom_diag *d);

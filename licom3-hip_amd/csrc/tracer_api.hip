@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -304,7 +305,11 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
     k.sssnorm = P->sssnorm;
 
     Scratch &s = c->s;
-    s.nblk = 1024;
+    // SSSNORM partial-sum blocks: about 8 physical cells per lane, at most 1024
+    {
+        const size_t cells = (size_t)(c->imt - 4) * (c->jmt - 4);
+        s.nblk = (int)std::min<size_t>(1024, std::max<size_t>(1, (cells + 2047) / 2048));
+    }
     rc = rc ? rc : scratch(c, &s.stf_pre, n2);
     rc = rc ? rc : scratch(c, &s.work_u, n2);
     rc = rc ? rc : scratch(c, &s.stf_n, 2 * n2);
@@ -348,6 +353,12 @@ int licom_tracer_step(licom_ctx *c, const licom_fields *f, const licom_diag *d, 
         set_error("boundary_restore=1 needs the restore field");
         return LICOM_EINVAL;
     }
+    // The fused k-march stages planes in 16-byte chunks: it needs an even imt
+    // and 16-byte aligned state arrays; other layouts take the per-stage
+    // kernels (same results).
+    const uintptr_t al = reinterpret_cast<uintptr_t>(f->at) | reinterpret_cast<uintptr_t>(f->atb) |
+                         reinterpret_cast<uintptr_t>(f->ws);
+    if (c->imt % 2 != 0 || (al & 15) != 0) flags |= LICOM_STEP_STAGED;
     if (flags & LICOM_STEP_STAGED) {
         // scratch of the per-stage kernels, allocated on first use
         if (!c->s.flags) {
@@ -358,6 +369,32 @@ int licom_tracer_step(licom_ctx *c, const licom_fields *f, const licom_diag *d, 
         return launch_step_simple(c, f, d, flags);
     }
     return launch_step_fused(c, f, d, flags);
+}
+
+int licom_tracer_step_mixed(licom_ctx *c, const licom_fields_mixed *f, const licom_diag_mixed *d,
+                            int flags) {
+    if (!c || !f) {
+        set_error("licom_tracer_step_mixed: NULL argument");
+        return LICOM_EINVAL;
+    }
+    const char *req[] = {"at", "atb", "akt", "utf", "vtf", "utl", "vtl", "ws", "h0f", "h0l"};
+    const void *ptrs[] = {f->at, f->atb, f->akt, f->utf, f->vtf, f->utl, f->vtl, f->ws, f->h0f, f->h0l};
+    for (int t = 0; t < 10; ++t)
+        if (int rc = check_ptr(ptrs[t], req[t])) return rc;
+    if (c->p.boundary_restore == 1 && !f->restore) {
+        set_error("boundary_restore=1 needs the restore field");
+        return LICOM_EINVAL;
+    }
+    if (flags & (LICOM_STEP_VTL_SCRATCH | LICOM_STEP_STAGED)) {
+        set_error("licom_tracer_step_mixed: VTL_SCRATCH and STAGED are FP64-only");
+        return LICOM_ENOTSUP;
+    }
+    const uintptr_t al = reinterpret_cast<uintptr_t>(f->at) | reinterpret_cast<uintptr_t>(f->atb);
+    if (c->imt % 2 != 0 || (al & 15) != 0) {
+        set_error("licom_tracer_step_mixed: needs an even imt and 16-byte aligned at/atb");
+        return LICOM_ENOTSUP;
+    }
+    return launch_step_mixed(c, f, d, flags);
 }
 
 int licom_convadj(licom_ctx *c, const licom_fields *f, const licom_diag *d) {

@@ -1,25 +1,27 @@
 // tracer_fused.hip -- production path of the FP64 tracer step on gfx950.
 //
-// Three HBM passes instead of the reference's ~21 launches per tracer:
+// Five kernels instead of the reference's ~21 launches per tracer:
 //
-//   k_prelude<KM>  one lane per column: face velocities u_wface/v_sface
-//                  (advection_tracer_1_0) and the barotropic-corrected
-//                  continuity w (upwell_2..4, operators_div); the column's
-//                  divergence profile stays in registers between the
-//                  column sum and the w recursion.
-//   k_march        one workgroup per 16x16 tile of columns and tracer,
-//                  marching k top to bottom.  Planes of at, atb and the
-//                  velocities (tile plus a 2-cell halo) are staged in LDS,
-//                  fetched a level ahead into registers; the tspas predictor of
-//                  level k+1 (tile plus a 1-cell ring) feeds the limiter of
-//                  level k through LDS flag planes; the del4 first pass is
-//                  kept in LDS.  Writes the explicit update (tracer_8's vtl)
-//                  and the advective / vertical-diffusion diagnostics.
+//   k_facediv      one lane per cell: face velocities u_wface/v_sface
+//                  (advection_tracer_1_0) and the divergence of the
+//                  barotropic-corrected velocity (upwell_2, operators_div).
+//   k_wcol<KM>     one lane per column: continuity w (upwell_3/4), the
+//                  divergence profile held in registers.
+//   k_march        one five-wave workgroup per 16x16 tile of columns and
+//                  tracer, marching k top to bottom.  Waves 0-3 own the tile,
+//                  wave 4 the 64-cell ring the limiter and del4 read around
+//                  it; wave 4 also streams the planes of at, atb and the
+//                  velocities (tile plus a 2-cell halo) straight into LDS a
+//                  level ahead.  The tspas predictor of level k+1 feeds the
+//                  limiter of level k through LDS flag planes; the del4 first
+//                  pass stays in LDS.  Writes the explicit update (tracer_8's
+//                  vtl) and the advective / vertical-diffusion diagnostics.
 //   k_solve<KM>    two lanes per column (T and S): the implicit vertical
 //                  diffusion (invtrit) with checkpointed elimination
 //                  coefficients in registers, tracer_9's tendencies, and a
 //                  streaming stability check of convadj; statically unstable
 //                  columns are then mixed in place exactly as convadj_cu does.
+//   k_side3/k_side2  the in-place side effects on utf, vtf, h0f and akt.
 //
 // Arithmetic is shared with the per-stage kernels through tracer_device.h,
 // so both paths give identical bits.
@@ -27,6 +29,7 @@
 
 #include "tracer_device.h"
 #include "tracer_internal.h"
+#include "tracer_solve.h"
 
 namespace licom {
 namespace {
@@ -34,81 +37,97 @@ namespace {
 constexpr int KMAXF = LICOM_MAX_KM;
 
 // ------------------------------------------------------------- prelude
-template <int KM>
-__global__ __launch_bounds__(256) void k_prelude(KGrid g, KPar p, const double *__restrict__ utf,
+// Two passes.  k_facediv, one lane per cell of every level: the face
+// velocities u_wface / v_sface of advection_tracer_1_0 and the divergence
+// wka of operators_div on upwell_2's barotropic-corrected uk, vk
+// (tracer.cpp:83-119, 980-1027).  k_wcol, one lane per column: the column
+// sum and the continuity recursion of upwell_3 / upwell_4
+// (tracer.cpp:1029-1080), the divergence profile held in registers.
+__global__ __launch_bounds__(256) void k_facediv(KGrid g, KPar p, const double *__restrict__ utf,
                                                  const double *__restrict__ utl,
                                                  const double *__restrict__ vtf,
                                                  const double *__restrict__ vtl,
                                                  const double *__restrict__ work_u,
-                                                 const double *__restrict__ stf_pre,
-                                                 double *__restrict__ ws, double *__restrict__ uw,
-                                                 double *__restrict__ vs) {
+                                                 double *__restrict__ uw, double *__restrict__ vs,
+                                                 double *__restrict__ wka) {
+    // wkd / wkb of a 65 x 5 window (the 64 x 4 block plus its east column
+    // and north row), each element read from HBM once
+    __shared__ double sD[5][65], sBv[5][65];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
+    const int i = i0 + tx, j = j0 + ty;
+    const int k = blockIdx.z;
+    const int imt = g.imt, jmt = g.jmt;
+    const size_t n2 = (size_t)imt * jmt, o = (size_t)k * n2;
+    const double oncc = p.oncc;
+    for (int e = ty * 64 + tx; e < 5 * 65; e += 256) {
+        const int wr = e / 65, wc = e - wr * 65;
+        const int jj = j0 - 1 + wr, ii = i0 + wc;
+        double d = 0.0, b = 0.0;
+        if (jj >= 0 && jj < jmt && ii < imt) {
+            const size_t c = o + (size_t)jj * imt + ii;
+            d = dev::wk(utf[c], utl[c], oncc);
+            b = dev::wk(vtf[c], vtl[c], oncc);
+        }
+        sD[wr][wc] = d;
+        sBv[wr][wc] = b;
+    }
+    __syncthreads();
+    if (i >= imt || j >= jmt) return;
+    const size_t q = (size_t)j * imt + i, c = o + q;
+    const int wr = ty + 1, wc = tx;          // window position of (j, i)
+    const double dc = sD[wr][wc], bc = sBv[wr][wc];
+    double u = 0.0, v = 0.0, d = 0.0;
+    if (j >= 1 && j < jmt - 1) u = (sD[wr - 1][wc] + dc) * g.htw[q] * 0.25;
+    if (i >= 1 && i < imt - 1) {
+        const double be = sBv[wr][wc + 1];
+        v = (bc + be) * g.hts[q] * 0.25;
+        if (j >= 1 && j < jmt - 1 && k <= g.kmt[q] - 1) {
+            const size_t qe = q + 1, qn = q - imt, qne = q - imt + 1;
+            const double fc = 1.0 + work_u[q] * g.ohbu[q];
+            const double fe = 1.0 + work_u[qe] * g.ohbu[qe];
+            const double fn = 1.0 + work_u[qn] * g.ohbu[qn];
+            const double fne = 1.0 + work_u[qne] * g.ohbu[qne];
+            const double de = sD[wr][wc + 1], dn = sD[wr - 1][wc], dne = sD[wr - 1][wc + 1];
+            const double bn = sBv[wr - 1][wc], bne = sBv[wr - 1][wc + 1];
+            d = 0.5 *
+                ((fe * de + fne * dne) * g.htw[qe] - (fc * dc + fn * dn) * g.htw[q] +
+                 (fe * be + fc * bc) * g.hts[q] - (fne * bne + fn * bn) * g.hts[qn]) *
+                g.tarea_r[q];
+        }
+    }
+    uw[c] = u;
+    vs[c] = v;
+    wka[c] = d;
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void k_wcol(KGrid g, const double *__restrict__ wka,
+                                              const double *__restrict__ stf_pre,
+                                              double *__restrict__ ws) {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
-    if (i >= imt || j >= jmt - 1) return;
+    if (i < 1 || i >= imt - 1 || j < 1 || j >= jmt - 1) return;
     const size_t n2 = (size_t)imt * jmt, q = (size_t)j * imt + i;
-    const bool do_u = j >= 1 && i >= 1;                // u_wface: j in [1,jmt-1), all i
-    const bool do_v = i >= 1 && i < imt - 1;           // v_sface: i in [1,imt-1)
-    const bool do_w = j >= 1 && i >= 1 && i < imt - 1; // w: [1,jmt-1) x [1,imt-1)
-    const double oncc = p.oncc;
-    const size_t qe = q + 1, qn = q - imt, qne = q - imt + 1;
-    double fc = 0, fe = 0, fn = 0, fne = 0, htw_e = 0, hts_n = 0, tr = 0;
-    const double htw_c = g.htw[q], hts_c = g.hts[q];
-    int kmt = 0;
-    if (do_w) {
-        fc = 1.0 + work_u[q] * g.ohbu[q];
-        fe = 1.0 + work_u[qe] * g.ohbu[qe];
-        fn = 1.0 + work_u[qn] * g.ohbu[qn];
-        fne = 1.0 + work_u[qne] * g.ohbu[qne];
-        htw_e = g.htw[qe];
-        hts_n = g.hts[qn];
-        tr = g.tarea_r[q];
-        kmt = g.kmt[q];
-    }
-    double wka[KM];
+    const int kmt = g.kmt[q];
+    double d[KM];
     double w = 0.0;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
-        wka[k] = 0.0;
-        if (k < km) {
-            const size_t o = (size_t)k * n2;
-            const double dc = dev::wk(utf[o + q], utl[o + q], oncc);
-            const double bc = dev::wk(vtf[o + q], vtl[o + q], oncc);
-            if (do_u) {
-                const double dn = dev::wk(utf[o + qn], utl[o + qn], oncc);
-                uw[o + q] = (dn + dc) * htw_c * 0.25;
-            }
-            if (do_v) {
-                const double be = dev::wk(vtf[o + qe], vtl[o + qe], oncc);
-                vs[o + q] = (bc + be) * hts_c * 0.25;
-            }
-            if (do_w && k <= kmt - 1) {
-                const double de = dev::wk(utf[o + qe], utl[o + qe], oncc);
-                const double dn = dev::wk(utf[o + qn], utl[o + qn], oncc);
-                const double dne = dev::wk(utf[o + qne], utl[o + qne], oncc);
-                const double be = dev::wk(vtf[o + qe], vtl[o + qe], oncc);
-                const double bn = dev::wk(vtf[o + qn], vtl[o + qn], oncc);
-                const double bne = dev::wk(vtf[o + qne], vtl[o + qne], oncc);
-                // operators_div (tracer.cpp:995-1003) on uk,vk of upwell_2
-                const double d = 0.5 *
-                                 ((fe * de + fne * dne) * htw_e - (fc * dc + fn * dn) * htw_c +
-                                  (fe * be + fc * bc) * hts_c - (fne * bne + fn * bn) * hts_n) *
-                                 tr;
-                wka[k] = d;
-                w = w - g.dzp[k] * d * 1.0;
-            }
+        d[k] = 0.0;
+        if (k < km && k <= kmt - 1) {
+            d[k] = wka[(size_t)k * n2 + q];
+            w = w - g.dzp[k] * d[k] * 1.0;
         }
     }
-    if (!do_w) return;
-    // upwell_3 / upwell_4 (tracer.cpp:1045-1079)
     const double ohbt = g.ohbt[q];
     const double sc = 1.0 / (1.0 + stf_pre[q] * ohbt);
     double prev = ws[q];
 #pragma unroll
     for (int k = 1; k < KM; ++k) {
         if (k < km) {
-            const double cur = (k < kmt ? 1.0 : 0.0) * (prev + g.dzp[k - 1] * (w * ohbt + wka[k - 1]));
+            const double cur = (k < kmt ? 1.0 : 0.0) * (prev + g.dzp[k - 1] * (w * ohbt + d[k - 1]));
             ws[(size_t)k * n2 + q] = cur * sc;
             prev = cur;
         }
@@ -119,10 +138,12 @@ __global__ __launch_bounds__(256) void k_prelude(KGrid g, KPar p, const double *
 constexpr int BX = 16, BY = 16;                 // interior tile (one lane per column)
 constexpr int PX = BX + 2, PY = BY + 2;         // tile + 1-cell ring (predictor, d2tk)
 constexpr int RX = BX + 4, RY = BY + 4;         // tile + 2-cell halo (staged planes)
-constexpr int NT = BX * BY;                     // 256 lanes
+constexpr int NOWN = BX * BY;                   // 256 tile lanes: waves 0-3
+constexpr int NRING = 2 * (BX + BY);            // 64 ring cells: wave 4 (the ring's
+                                                // corners are never read)
+constexpr int NT = NOWN + NRING;                // 320 lanes, five waves
 constexpr int NP = PX * PY, NR = RX * RY;
-constexpr int NRING = NP - BX * BY;             // 68 ring cells
-constexpr int NPF = (NR + NT - 1) / NT;         // plane cells staged per lane
+constexpr int NAT = 4, NUV = 3, NW = 4, NB = 3; // LDS slots per staged array
 
 struct MarchArgs {
     const double *at, *atb, *akt, *uw, *vs, *ws, *stf_n, *err_norm2, *swv, *restore;
@@ -130,16 +151,14 @@ struct MarchArgs {
     int ntx, ntiles, tiles_per_xcd;
 };
 
-// Ring cell r (0..NRING-1) -> P-plane coordinates.
-__device__ __forceinline__ void ring_cell(int r, int &pj, int &pi) {
-    if (r < PX) { pj = 0; pi = r; return; }
-    r -= PX;
-    if (r < PX) { pj = PY - 1; pi = r; return; }
-    r -= PX;
-    if (r < PY - 2) { pj = 1 + r; pi = 0; return; }
-    r -= PY - 2;
-    pj = 1 + r;
-    pi = PX - 1;
+// Ring cell r (0..NRING-1) -> P-plane coordinates (pi, pj): north row, south
+// row, west column, east column, corners excluded.
+static_assert(BX == 16 && BY == 16, "ring_cell assumes a 16x16 tile");
+__device__ __forceinline__ int2 ring_cell(int r) {
+    const int side = r >> 4, o = 1 + (r & 15);
+    const int pj = side == 0 ? 0 : (side == 1 ? PY - 1 : o);
+    const int pi = side <= 1 ? o : (side == 2 ? 0 : PX - 1);
+    return make_int2(pi, pj);
 }
 
 // Static metrics of a predicted cell: tarea_r, the four face products
@@ -233,37 +252,51 @@ __device__ __forceinline__ double d2tk_at(int r, const double *B, const int *sK,
     return ahf * (cc * B[r] + hdtk);
 }
 
-// One lane's share of a staged plane: the R-plane cells t, t+NT, ...
-struct PlaneSlots {
-    size_t off[NPF];   // offset inside one (jmt,imt) plane
-    bool in[NPF];      // cell inside the array (else staged as 0)
-};
+#define LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
 
-__device__ __forceinline__ void fetch(const PlaneSlots &s, const double *src, double (&v)[NPF]) {
+// One wave copies the R-window (tile plus 2-cell halo) of one plane straight
+// from global memory into LDS (global_load_lds: lane l of an instruction
+// lands at LDS base + 16 l), in 16-byte chunks, which needs an even imt and
+// 16-byte aligned arrays (the launcher checks).  off[u] is this lane's chunk
+// offset for instruction u inside the plane, clamped into the plane, so
+// window cells outside the array receive other cells' values, which only
+// feed cells whose results are discarded.  The loads complete asynchronously
+// (vmcnt).
+constexpr int NCH = NR / 2;                 // 16-byte chunks per window
+constexpr int NCI = (NCH + 63) / 64;        // instructions per window
+__device__ __forceinline__ void stage_plane(const double *src, double *dst, const int (&off)[NCI],
+                                            int lane) {
 #pragma unroll
-    for (int u = 0; u < NPF; ++u) v[u] = s.in[u] ? src[s.off[u]] : 0.0;
+    for (int u = 0; u < NCI; ++u)
+        if (u * 64 + lane < NCH)
+            __builtin_amdgcn_global_load_lds(src + off[u], LDS_PTR(dst + 128 * u), 16, 0, 0);
 }
 
-__device__ __forceinline__ void stage(const double (&v)[NPF], double *dst) {
-#pragma unroll
-    for (int u = 0; u < NPF; ++u) {
-        const int r = threadIdx.x + u * NT;
-        if (r < NR) dst[r] = v[u];
-    }
-}
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-// k-march over one 16x16 tile of columns and one tracer.  Every level's
-// inputs (at, atb, u_wface, v_sface, ws) are staged as LDS planes covering
-// the tile plus a 2-cell halo; each lane fetches its share of the next level
-// into registers while the current level is computed.
-__global__ __launch_bounds__(NT) void k_march(KGrid g, KPar p, MarchArgs a) {
-    __shared__ double sAT[3][NR];
-    __shared__ double sB[2][NR];
-    __shared__ double sU[2][NR];
-    __shared__ double sV[2][NR];
-    __shared__ double sW[3][NR];
+// Workgroup barrier that waits only for this wave's LDS traffic.  The
+// staging wave's direct-to-LDS loads are waited for explicitly (wait_vmem)
+// before the barrier that publishes them, so the tile waves never wait on
+// vector-memory counters at a barrier.
+__device__ __forceinline__ void bar_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// k-march over one 16x16 tile of columns and one tracer, five waves.
+// Waves 0-3 own the tile's cells; wave 4 owns the 64 ring cells the limiter
+// and the del4 second pass read around the tile, and stages every plane.
+// Per level k two phases, each closed by a barrier:
+//   P(k): tspas predictor of level k+1 and del4 first pass of level k at
+//         every cell (tile and ring, one cell per lane); wave 4 then issues
+//         the direct-to-LDS loads of the planes P(k+1) and C(k+1) need.
+//   C(k): limiter, fluxes and explicit update of level k on the tile (waves
+//         0-3); wave 4 waits for its loads.
+// LDS slots rotate: at x4, u_wface/v_sface x3, ws x4, atb x3.
+__global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
+    __shared__ __align__(16) double sAT[NAT][NR];
+    __shared__ __align__(16) double sU[NUV][NR];
+    __shared__ __align__(16) double sV[NUV][NR];
+    __shared__ __align__(16) double sW[NW][NR];
+    __shared__ __align__(16) double sB[NB][NR];
     __shared__ double sD2[NP];
-    __shared__ double sRM[10][NRING];
     __shared__ unsigned char sFL[2][NP];
     __shared__ int sK[NR];
 
@@ -276,149 +309,137 @@ __global__ __launch_bounds__(NT) void k_march(KGrid g, KPar p, MarchArgs a) {
     const int imt = g.imt, jmt = g.jmt, km = g.km;
     const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
     const int gi0 = 2 + (tile % a.ntx) * BX, gj0 = 2 + (tile / a.ntx) * BY;
+    const long e0 = (long)(gj0 - 2) * imt + (gi0 - 2);
 
-    const int t = threadIdx.x, li = t % BX, lj = t / BX;
-    const int gi = gi0 + li, gj = gj0 + lj;
-    const bool act = gi < imt - 2 && gj < jmt - 2;
-    const size_t q = (size_t)gj * imt + gi;
+    const int t = threadIdx.x, lane = t & 63;
+    const bool ringw = t >= NOWN;
+    const int2 rc = ring_cell((t - NOWN) & 63);
+    const int pi = ringw ? rc.x : 1 + t % BX, pj = ringw ? rc.y : 1 + t / BX;
+    const int gi = gi0 - 1 + pi, gj = gj0 - 1 + pj;
+    const int rC = (pj + 1) * RX + (pi + 1), pC = pj * PX + pi;
+    const bool cell_ok = gj >= 1 && gj < jmt - 1 && gi >= 1 && gi < imt - 1;
+    const bool act = !ringw && gi < imt - 2 && gj < jmt - 2;
+    const size_t q = cell_ok ? (size_t)gj * imt + gi : 0;
 
     const double *T = a.at + n * n3;
     const double *TB = a.atb + n * (n3 + n2) + n2;   // level k at plane k
     const double *A = a.akt + n * n3;
     const double dts = p.dts;
 
-    PlaneSlots ps;
-#pragma unroll
-    for (int u = 0; u < NPF; ++u) {
-        const int r = t + u * NT;
-        const int rj = gj0 - 2 + r / RX, ri = gi0 - 2 + r % RX;
-        ps.in[u] = r < NR && rj >= 0 && rj < jmt && ri >= 0 && ri < imt;
-        ps.off[u] = ps.in[u] ? (size_t)rj * imt + ri : 0;
-    }
     for (int r = t; r < NR; r += NT) {
         const int rj = gj0 - 2 + r / RX, ri = gi0 - 2 + r % RX;
-        sK[r] = (rj >= 0 && rj < jmt && ri >= 0 && ri < imt) ? g.kmt[(size_t)rj * imt + ri] : 0;
+        sK[r] = (rj < jmt && ri < imt) ? g.kmt[(size_t)rj * imt + ri] : 0;
     }
-    // own cell and ring cell
-    const int rOwn = (lj + 2) * RX + (li + 2), pOwn = (lj + 1) * PX + (li + 1);
-    const bool own_ok = gj >= 1 && gj < jmt - 1 && gi >= 1 && gi < imt - 1;
     Metrics M{};
-    if (own_ok) M = load_metrics(g, gj, gi);
-    int rpj = 0, rpi = 0;
-    if (t < NRING) ring_cell(t, rpj, rpi);
-    const int rgj = gj0 - 1 + rpj, rgi = gi0 - 1 + rpi;
-    const bool rok = t < NRING && rgj >= 1 && rgj < jmt - 1 && rgi >= 1 && rgi < imt - 1;
-    const int rR = (rpj + 1) * RX + (rpi + 1), pR = rpj * PX + rpi;
-    if (rok) {
-        const Metrics m = load_metrics(g, rgj, rgi);
-        sRM[0][t] = m.tr; sRM[1][t] = m.D1; sRM[2][t] = m.D2; sRM[3][t] = m.D3; sRM[4][t] = m.D4;
-        sRM[5][t] = m.dtn; sRM[6][t] = m.dts; sRM[7][t] = m.dte; sRM[8][t] = m.dtw; sRM[9][t] = m.ahf;
-    }
+    if (cell_ok) M = load_metrics(g, gj, gi);
     const double sw = (n == 0 && act && a.swv) ? a.swv[q] : 0.0;
     const int kmt_own = act ? g.kmt[q] : 0;
     const double stf_n = act ? a.stf_n[n * n2 + q] : 0.0;
     const double enorm = (n == 1 && p.sssnorm && act) ? a.err_norm2[0] : 0.0;
     const bool rest = p.boundary_restore == 1 && a.restore != nullptr;
 
-    // prologue: levels 0 and 1 of at and ws, level 0 of atb, u_wface, v_sface
-    double v0[NPF], v1[NPF];
-    fetch(ps, T, v0);
-    fetch(ps, T + n2, v1);
-    stage(v0, sAT[0]);
-    stage(v1, sAT[1]);
-    fetch(ps, TB, v0);
-    stage(v0, sB[0]);
-    fetch(ps, a.uw, v0);
-    fetch(ps, a.vs, v1);
-    stage(v0, sU[0]);
-    stage(v1, sV[0]);
-    fetch(ps, a.ws, v0);
-    fetch(ps, a.ws + n2, v1);
-    stage(v0, sW[0]);
-    stage(v1, sW[1]);
-    // registers holding the next level's share of every plane
-    double fAT[NPF], fB[NPF], fU[NPF], fV[NPF], fW[NPF];
-    if (km > 2) fetch(ps, T + 2 * n2, fAT);
-    fetch(ps, TB + n2, fB);
-    fetch(ps, a.uw + n2, fU);
-    fetch(ps, a.vs + n2, fV);
-    if (km > 2) fetch(ps, a.ws + 2 * n2, fW);
-    double fA = act ? A[q] : 0.0;
-    double fR = 0.0;
-    __syncthreads();
+    if (ringw) {
+        // ---- wave 4: ring cells and plane staging
+        int off[NCI];
+#pragma unroll
+        for (int u = 0; u < NCI; ++u) {
+            const int c = min(u * 64 + lane, NCH - 1), row = c / (RX / 2), cp = c - row * (RX / 2);
+            off[u] = (int)min(e0 + (long)row * imt + 2 * cp, (long)n2 - 2);
+        }
+        // prologue: at 0..2, faces 0..1, ws 0..2, atb 0..1
+        for (int l = 0; l < 3 && l < km; ++l) {
+            stage_plane(T + (size_t)l * n2, sAT[l], off, lane);
+            stage_plane(a.ws + (size_t)l * n2, sW[l], off, lane);
+        }
+        for (int l = 0; l < 2 && l < km; ++l) {
+            stage_plane(a.uw + (size_t)l * n2, sU[l], off, lane);
+            stage_plane(a.vs + (size_t)l * n2, sV[l], off, lane);
+            stage_plane(TB + (size_t)l * n2, sB[l], off, lane);
+        }
+        wait_vmem();
+        bar_lds();
+        {
+            LW tmp;
+            const Lv L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
+            sFL[0][pC] = cell_ok ? pred_flag(g, rC, L0, sK, 0, dts, M.tr, M.D1, M.D2, M.D3, M.D4, tmp)
+                                 : false;
+        }
+        bar_lds();
+        for (int k = 0; k < km; ++k) {
+            const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3, s3 = (k + 3) & 3;
+            const int u0 = k % 3, u1 = (k + 1) % 3, u2 = (k + 2) % 3;
+            if (k + 1 < km) {
+                LW tmp;
+                const Lv L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
+                sFL[(k + 1) & 1][pC] =
+                    cell_ok ? pred_flag(g, rC, L1, sK, k + 1, dts, M.tr, M.D1, M.D2, M.D3, M.D4, tmp)
+                            : false;
+            }
+            if (p.hmix == LICOM_HMIX_DEL4)
+                sD2[pC] = cell_ok ? d2tk_at(rC, sB[u0], sK, k, M.dtn, M.dts, M.dte, M.dtw, M.ahf) : 0.0;
+            // planes for P(k+1) and C(k+1); their slots were last read in C(k-1)
+            if (k + 3 < km) {
+                stage_plane(T + (size_t)(k + 3) * n2, sAT[s3], off, lane);
+                stage_plane(a.ws + (size_t)(k + 3) * n2, sW[s3], off, lane);
+            }
+            if (k + 2 < km) {
+                stage_plane(a.uw + (size_t)(k + 2) * n2, sU[u2], off, lane);
+                stage_plane(a.vs + (size_t)(k + 2) * n2, sV[u2], off, lane);
+                stage_plane(TB + (size_t)(k + 2) * n2, sB[u2], off, lane);
+            }
+            bar_lds();     // end of P(k)
+            wait_vmem();   // C(k): the staged planes land
+            bar_lds();     // end of C(k)
+        }
+        return;
+    }
 
+    // ---- waves 0-3: the tile
+    double fA = act ? A[q] : 0.0;
+    double fR = (rest && act) ? a.restore[n * n3 + q] : 0.0;
+    bar_lds();
     LW lc{}, ln{};
     {
         const Lv L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
-        bool f0 = false;
-        if (own_ok) f0 = pred_flag(g, rOwn, L0, sK, 0, dts, M.tr, M.D1, M.D2, M.D3, M.D4, lc);
-        sFL[0][pOwn] = f0;
-        if (rok) {
-            LW tmp;
-            sFL[0][pR] = pred_flag(g, rR, L0, sK, 0, dts, sRM[0][t], sRM[1][t], sRM[2][t],
-                                   sRM[3][t], sRM[4][t], tmp);
-        }
+        sFL[0][pC] = cell_ok ? pred_flag(g, rC, L0, sK, 0, dts, M.tr, M.D1, M.D2, M.D3, M.D4, lc)
+                             : false;
     }
+    bar_lds();
+
     double Tprev = 0.0, Bprev = 0.0, Aprev = 0.0;
     bool fprev = false;
     const double lamda = 1.0 / (15.0 * 86400.0);
     const double aidif = 0.5;
 
     for (int k = 0; k < km; ++k) {
-        // ---- phase L: stage the prefetched planes, prefetch the next level
-        if (k + 2 < km) stage(fAT, sAT[(k + 2) % 3]);
-        if (k + 1 < km) {
-            stage(fB, sB[(k + 1) & 1]);
-            stage(fU, sU[(k + 1) & 1]);
-            stage(fV, sV[(k + 1) & 1]);
-        }
-        if (k + 2 < km) stage(fW, sW[(k + 2) % 3]);
+        const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3;
+        const int u0 = k % 3, u1 = (k + 1) % 3;
+        // ---- phase P(k)
         const double Acur = act ? dev::floor_akt(fA, k, n, p.hmix, p.dwndmix) : 0.0;
         const double Rk = fR;
-        if (k + 3 < km) fetch(ps, T + (size_t)(k + 3) * n2, fAT);
-        if (k + 2 < km) {
-            fetch(ps, TB + (size_t)(k + 2) * n2, fB);
-            fetch(ps, a.uw + (size_t)(k + 2) * n2, fU);
-            fetch(ps, a.vs + (size_t)(k + 2) * n2, fV);
-        }
-        if (k + 3 < km) fetch(ps, a.ws + (size_t)(k + 3) * n2, fW);
         if (k + 1 < km) {
             fA = act ? A[(size_t)(k + 1) * n2 + q] : 0.0;
             fR = (rest && act) ? a.restore[n * n3 + (size_t)(k + 1) * n2 + q] : 0.0;
+            const Lv L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
+            sFL[(k + 1) & 1][pC] =
+                cell_ok ? pred_flag(g, rC, L1, sK, k + 1, dts, M.tr, M.D1, M.D2, M.D3, M.D4, ln)
+                        : false;
         }
-        __syncthreads();
+        if (p.hmix == LICOM_HMIX_DEL4)
+            sD2[pC] = cell_ok ? d2tk_at(rC, sB[u0], sK, k, M.dtn, M.dts, M.dte, M.dtw, M.ahf) : 0.0;
+        bar_lds();
 
-        // ---- phase P: predictor of level k+1 (own + ring), d2tk of level k
-        const double *Pk = sAT[k % 3], *Pk1 = sAT[(k + 1) % 3];
-        const double *Bk = sB[k & 1], *Bk1 = sB[(k + 1) & 1];
-        if (k + 1 < km) {
-            const Lv L1{Pk1, Pk, sAT[(k + 2) % 3], sU[(k + 1) & 1], sV[(k + 1) & 1],
-                        sW[(k + 1) % 3], sW[(k + 2) % 3]};
-            bool f = false;
-            if (own_ok) f = pred_flag(g, rOwn, L1, sK, k + 1, dts, M.tr, M.D1, M.D2, M.D3, M.D4, ln);
-            sFL[(k + 1) & 1][pOwn] = f;
-            if (rok) {
-                LW tmp;
-                sFL[(k + 1) & 1][pR] = pred_flag(g, rR, L1, sK, k + 1, dts, sRM[0][t], sRM[1][t],
-                                                 sRM[2][t], sRM[3][t], sRM[4][t], tmp);
-            }
-        }
-        if (p.hmix == LICOM_HMIX_DEL4) {
-            sD2[pOwn] = own_ok ? d2tk_at(rOwn, Bk, sK, k, M.dtn, M.dts, M.dte, M.dtw, M.ahf) : 0.0;
-            if (rok)
-                sD2[pR] = d2tk_at(rR, Bk, sK, k, sRM[5][t], sRM[6][t], sRM[7][t], sRM[8][t], sRM[9][t]);
-        }
-        __syncthreads();
-
-        // ---- phase C: limited fluxes and the explicit update of level k
+        // ---- phase C(k): limited fluxes and the explicit update of level k
         if (act) {
+            const double *Pk = sAT[s0], *Pk1 = sAT[s1];
+            const double *Bk = sB[u0], *Bk1 = sB[u1];
             const unsigned char *F = sFL[k & 1];
-            const int r = rOwn, pp = pOwn;
+            const int r = rC, pp = pC;
             const double Tc = Pk[r], Te = Pk[r + 1], Tw = Pk[r - 1], Ts = Pk[r + RX], Tn = Pk[r - RX];
             const double Tu = Tprev, Td = (k + 1 < km) ? Pk1[r] : 0.0;
-            const double *U = sU[k & 1], *V = sV[k & 1];
+            const double *U = sU[u0], *V = sV[u0];
             const double Ue = U[r + 1], Uw = U[r], Vs = V[r], Vn = V[r - RX];
-            const double Wk = sW[k % 3][r], Wb = (k + 1 < km) ? sW[(k + 1) % 3][r] : 0.0;
+            const double Wk = sW[s0][r], Wb = (k + 1 < km) ? sW[s1][r] : 0.0;
             const bool fc = F[pp], fe = F[pp + 1], fw = F[pp - 1], fs = F[pp + PX], fn = F[pp - PX];
             const bool fd = (k + 1 < km) ? sFL[(k + 1) & 1][pp] : false;
             bool up1, up2, up3, up4;
@@ -532,291 +553,57 @@ __global__ __launch_bounds__(NT) void k_march(KGrid g, KPar p, MarchArgs a) {
         }
         Aprev = Acur;
         lc = ln;
-        __syncthreads();
+        bar_lds();
     }
 }
 
-// ---------------------------------------------------------------- solve
-// Two lanes per column: lane parity = tracer; 32 columns along i per wave.
-//
-// invtrit (tracer.cpp:923-978) eliminates from the bottom and substitutes
-// from the top.  Holding e(k), f(k) for the whole column would take 2*km
-// doubles per lane; instead the elimination keeps them only at every SEG-th
-// level, and the substitution recomputes each SEG-level segment from its
-// checkpoint with the same operations (identical bits).  Both sweeps load
-// the next segment's inputs while computing the current one.
-constexpr int SEG = 8;
-
-// convadj_cu (convadj.cpp:52-256) for one column, driven by the mask of
-// statically unstable interfaces found with the unmixed densities (bit k:
-// rholo(k-1) > rhoup(k)).  Every density the reference reads after mixing
-// starts is either recomputed by it or belongs to a level that mixing has not
-// touched, so the mask plus on-demand recomputation reproduces its decisions.
-// The two lanes of a column run this together: each holds its own tracer's
-// column X (the state just written, stride n2) and they swap values with
-// __shfl_xor.  Returns the mixed level range in lo/hi.
-__device__ __noinline__ void convect_sparse(const double *C, const double *to, const double *so,
-                                            const double *dzp, int km, int kcon, uint64_t mask,
-                                            int n, double *X, size_t n2, int &lo, int &hi) {
-    lo = km;
-    hi = -1;
-    int lcon = __ffsll((unsigned long long)mask) - 1;
-    for (;;) {   // CONV_1
-        int lcona = lcon, lconb = lcon + 1;
-        double dztsum = dzp[lcona - 1] + dzp[lconb - 1];
-        double trasum = X[(size_t)(lcona - 1) * n2] * dzp[lcona - 1] + X[(size_t)(lconb - 1) * n2] * dzp[lconb - 1];
-        double tramix = trasum / dztsum;
-        for (;;) {   // CONV_2
-            if (lconb != kcon) {
-                const double v = X[(size_t)lconb * n2];
-                const double vo = __shfl_xor(v, 1), mo = __shfl_xor(tramix, 1);
-                const double Tm = n == 0 ? tramix : mo, Sm = n == 0 ? mo : tramix;
-                const double Tv = n == 0 ? v : vo, Sv = n == 0 ? vo : v;
-                const double rl = dev::dens(C, km, lconb, Tm - to[lconb], Sm - so[lconb]);
-                const double ru = dev::dens(C, km, lconb, Tv - to[lconb], Sv - so[lconb]);
-                if (rl > ru) {
-                    lconb = lconb + 1;
-                    dztsum = dztsum + dzp[lconb - 1];
-                    trasum = trasum + v * dzp[lconb - 1];
-                    tramix = trasum / dztsum;
-                    continue;
-                }
-            }
-            if (lcona > 1) {
-                const double v = X[(size_t)(lcona - 2) * n2];
-                const double vo = __shfl_xor(v, 1), mo = __shfl_xor(tramix, 1);
-                const double Tm = n == 0 ? tramix : mo, Sm = n == 0 ? mo : tramix;
-                const double Tv = n == 0 ? v : vo, Sv = n == 0 ? vo : v;
-                const double rl = dev::dens(C, km, lcona - 1, Tv - to[lcona - 1], Sv - so[lcona - 1]);
-                const double ru = dev::dens(C, km, lcona - 1, Tm - to[lcona - 1], Sm - so[lcona - 1]);
-                if (rl > ru) {
-                    lcona = lcona - 1;
-                    dztsum = dztsum + dzp[lcona - 1];
-                    trasum = trasum + v * dzp[lcona - 1];
-                    tramix = trasum / dztsum;
-                    continue;
-                }
-            }
-            break;
-        }
-        for (int l = lcona - 1; l < lconb; ++l) X[(size_t)l * n2] = tramix;
-        lo = min(lo, lcona - 1);
-        hi = max(hi, lconb - 1);
-        if (lconb == kcon) return;
-        // CONV_3: next unstable interface below the mixed region
-        const uint64_t below = lconb >= 63 ? 0ull : (mask & ~((2ull << lconb) - 1ull));
-        if (below == 0) return;
-        lcon = __ffsll((unsigned long long)below) - 1;
+// Side effects of tracer_1_0 / tracer_1 / tracer_del4_2 (tracer.cpp:1088,
+// 1107-1108, 1187, 1315): utf, vtf *= oncc over the whole 3-D arrays (one
+// or two elements per lane, a pure streaming pass), then h0f *= onbc and the
+// AKT surface floor written back on the 2-D plane.
+template <typename V>
+__global__ __launch_bounds__(256) void k_side3(V *__restrict__ utf, V *__restrict__ vtf, size_t n,
+                                               double oncc) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    V a = utf[t], b = vtf[t];
+    if constexpr (sizeof(V) == 16) {
+        a.x *= oncc; a.y *= oncc;
+        b.x *= oncc; b.y *= oncc;
+    } else {
+        a *= oncc;
+        b *= oncc;
     }
+    utf[t] = a;
+    vtf[t] = b;
 }
 
-template <int KM>
-__global__ __launch_bounds__(256) void k_solve(KGrid g, KPar p, int flags, double *__restrict__ at,
-                                               double *__restrict__ atb,
-                                               const double *__restrict__ akt,
-                                               const double *__restrict__ ori,
-                                               const double *__restrict__ stf_n,
-                                               double *__restrict__ tend,
-                                               double *__restrict__ dt_diff,
-                                               double *__restrict__ dt_conv, int *mix_count) {
-    constexpr int NSEG = (KM + SEG - 1) / SEG;
-    const int t = threadIdx.x, n = t & 1, col = t >> 1;
-    const int gi = 2 + blockIdx.x * 32 + (col & 31);
-    const int gj = 2 + blockIdx.y * 4 + (col >> 5);
-    const int imt = g.imt, jmt = g.jmt, km = g.km;
-    const bool act = gi < imt - 2 && gj < jmt - 2;
-    const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
-    const size_t q = act ? (size_t)gj * imt + gi : 0;
-    const int kz = act ? g.kmt[q] : 0;
-    const double *O = ori + n * n3 + q;
-    const double *A = akt + n * n3 + q;
-    double *TT = at + n * n3 + q, *BB = atb + n * (n3 + n2) + n2 + q;
-    const double *odzp = g.odzp, *odzt = g.odzt;
-    const double dts = p.dts;
-    const double stf = act ? stf_n[n * n2 + q] : 0.0;
-    const int hmix = p.hmix;
-    const double dwm = p.dwndmix;
-
-    // ---- elimination from the bottom; checkpoints e,f at levels SEG*m (m >= 1)
-    double cke[NSEG + 1], ckf[NSEG + 1];
-#pragma unroll
-    for (int m = 0; m <= NSEG; ++m) cke[m] = ckf[m] = 0.0;
-    {
-        double pd[SEG], pa[SEG];
-#pragma unroll
-        for (int r = 0; r < SEG; ++r) {
-            const int k = SEG * (NSEG - 1) + r;
-            pd[r] = pa[r] = 0.0;
-            if (k < KM && k < kz) {
-                pd[r] = O[(size_t)k * n2];
-                pa[r] = k >= 1 ? A[(size_t)(k - 1) * n2] : 0.0;
-            }
-        }
-        double en = 0.0, fn = 0.0, wk_k = 0.0;
-#pragma unroll
-        for (int m = NSEG - 1; m >= 0; --m) {
-            double cd[SEG], ca[SEG];
-#pragma unroll
-            for (int r = 0; r < SEG; ++r) {
-                cd[r] = pd[r];
-                ca[r] = pa[r];
-            }
-            if (m > 0) {
-#pragma unroll
-                for (int r = 0; r < SEG; ++r) {
-                    const int k = SEG * (m - 1) + r;
-                    pd[r] = pa[r] = 0.0;
-                    if (k < kz) {
-                        pd[r] = O[(size_t)k * n2];
-                        pa[r] = k >= 1 ? A[(size_t)(k - 1) * n2] : 0.0;
-                    }
-                }
-            }
-#pragma unroll
-            for (int r = SEG - 1; r >= 0; --r) {
-                const int k = SEG * m + r;
-                if (k < KM && k < kz) {
-                    const double wkm1 = k >= 1 ? dev::floor_akt(ca[r], k - 1, n, hmix, dwm) : 0.0;
-                    double aa, bb, cc;
-                    dev::tri_row(k, kz, wkm1, wk_k, odzp, odzt, dts, aa, bb, cc);
-                    const double g0 = 1.0 / (bb - cc * en);
-                    en = aa * g0;
-                    fn = (cd[r] + cc * fn) * g0;
-                    wk_k = wkm1;
-                }
-            }
-            if (m > 0) {
-                cke[m] = en;
-                ckf[m] = fn;
-            }
-        }
-    }
-
-    // ---- substitution from the top, tracer_9, convadj stability scan
-    const bool conv = flags & LICOM_STEP_CONVADJ;
-    const double *C = g.eosc, *to = g.to, *so = g.so;
-    double xprev = 0.0, Tp = 0.0, Sp = 0.0;
-    uint64_t mask = 0;
-    double qd[SEG], qa[SEG], qt[SEG], qw = 0.0;
-    auto load_seg = [&](int m) {
-#pragma unroll
-        for (int r = 0; r < SEG; ++r) {
-            const int k = SEG * m + r;
-            qd[r] = qa[r] = qt[r] = 0.0;
-            if (k < KM && k < km && act) {
-                qd[r] = O[(size_t)k * n2];
-                qt[r] = TT[(size_t)k * n2];
-                if (k >= 1 && k < kz) qa[r] = A[(size_t)(k - 1) * n2];
-            }
-        }
-        const int kb = SEG * m + SEG - 1;
-        qw = (kb < KM && kb < kz - 1) ? A[(size_t)kb * n2] : 0.0;
-    };
-    load_seg(0);
-#pragma unroll
-    for (int m = 0; m < NSEG; ++m) {
-        double cd[SEG], ca[SEG], ct[SEG];
-#pragma unroll
-        for (int r = 0; r < SEG; ++r) {
-            cd[r] = qd[r];
-            ca[r] = qa[r];
-            ct[r] = qt[r];
-        }
-        const double cw = qw;
-        if (m + 1 < NSEG) load_seg(m + 1);
-        const int top = SEG * m;
-        double se[SEG], sf[SEG];
-        {
-            double ee = cke[m + 1], ff = ckf[m + 1];
-            double wk_k = dev::floor_akt(cw, top + SEG - 1, n, hmix, dwm);
-#pragma unroll
-            for (int r = SEG - 1; r >= 0; --r) {
-                const int k = top + r;
-                se[r] = sf[r] = 0.0;
-                if (k < KM && k < kz) {
-                    const double wkm1 = k >= 1 ? dev::floor_akt(ca[r], k - 1, n, hmix, dwm) : 0.0;
-                    double aa, bb, cc;
-                    dev::tri_row(k, kz, wkm1, wk_k, odzp, odzt, dts, aa, bb, cc);
-                    const double g0 = 1.0 / (bb - cc * ee);
-                    ee = aa * g0;
-                    ff = (cd[r] + cc * ff) * g0;
-                    se[r] = ee;
-                    sf[r] = ff;
-                    wk_k = wkm1;
-                }
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < SEG; ++r) {
-            const int k = top + r;
-            if (k < KM && k < km) {
-                const size_t c = (size_t)k * n2;
-                const bool wet = k < kz;
-                const double o = cd[r];
-                double x = o;
-                if (wet) x = (k == 0) ? (se[0] * stf + sf[0]) * 1.0 : (se[r] * xprev + sf[r]) * 1.0;
-                xprev = x;
-                const double vk = wet ? 1.0 : 0.0;
-                if (act) {
-                    if (dt_diff) {
-                        double v = (x - o) / dts * vk;
-                        if (k == 0) v = v - stf * 0.5 * odzp[0];
-                        dt_diff[n * n3 + q + c] = v;
-                    }
-                    if (tend) tend[n * n3 + q + c] = (x - ct[r]) / dts * vk;
-                    if (conv && dt_conv) dt_conv[n * n3 + q + c] = 0.0;
-                    TT[c] = x;
-                    BB[c] = x;
-                }
-                if (conv) {
-                    const double xo = __shfl_xor(x, 1);
-                    const double Tk = n == 0 ? x : xo, Sk = n == 0 ? xo : x;
-                    if (k >= 1 && k <= kz - 1) {
-                        const double up = dev::dens(C, km, k, Tk - to[k], Sk - so[k]);
-                        const double lo = dev::dens(C, km, k, Tp - to[k], Sp - so[k]);
-                        if (lo > up) mask |= 1ull << k;
-                    }
-                    Tp = Tk;
-                    Sp = Sk;
-                }
-            }
-        }
-    }
-
-    // ---- convadj for statically unstable columns (both lanes of the pair)
-    if (conv && act && mask != 0) {
-        if (n == 0) atomicAdd(mix_count, 1);
-        int lo, hi;
-        convect_sparse(C, to, so, g.dzp, km, kz, mask, n, TT, n2, lo, hi);
-        for (int l = lo; l <= hi; ++l) {
-            const size_t c = (size_t)l * n2;
-            const double y = TT[c], b = BB[c];
-            if (y != b) {
-                const double dc = (y - b) / dts * 1.0;
-                if (dt_conv) dt_conv[n * n3 + q + c] = dc;
-                if (tend) tend[n * n3 + q + c] += dc;
-                BB[c] = y;
-            }
-        }
-    }
-}
-
-// In-place side effects of the reference on its inputs
-// (tracer.cpp:1088, 1107-1108, 1187, 1315).
-__global__ void k_side(KGrid g, KPar p, double *utf, double *vtf, double *h0f, double *akt) {
+__global__ __launch_bounds__(256) void k_side2(KGrid g, KPar p, double *h0f, double *akt) {
     const size_t n2 = (size_t)g.imt * g.jmt, n3 = n2 * g.km;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n3;
-         t += (size_t)gridDim.x * blockDim.x) {
-        utf[t] *= p.oncc;
-        vtf[t] *= p.oncc;
-        if (t < n2) {
-            h0f[t] = h0f[t] * p.onbc;
-            const int j = (int)(t / g.imt), i = (int)(t % g.imt);
-            if (i >= 1 && i < g.imt - 1 && j >= 1 && j < g.jmt - 1)
-                for (int n = 0; n < 2; ++n)
-                    akt[n * n3 + t] = dev::floor_akt(akt[n * n3 + t], 0, n, p.hmix, p.dwndmix);
-        }
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n2) return;
+    h0f[t] = h0f[t] * p.onbc;
+    const int j = (int)(t / g.imt), i = (int)(t % g.imt);
+    if (i >= 1 && i < g.imt - 1 && j >= 1 && j < g.jmt - 1)
+        for (int n = 0; n < 2; ++n)
+            akt[n * n3 + t] = dev::floor_akt(akt[n * n3 + t], 0, n, p.hmix, p.dwndmix);
+}
+
+void launch_side(licom_ctx *ctx, const licom_fields *f) {
+    const KGrid &g = ctx->g;
+    hipStream_t st = ctx->stream;
+    Timer t(ctx, "k_side");
+    const size_t n3 = ctx->n3;
+    const bool vec = ((reinterpret_cast<uintptr_t>(f->utf) | reinterpret_cast<uintptr_t>(f->vtf)) & 15) == 0 &&
+                     n3 % 2 == 0;
+    if (vec) {
+        const size_t n = n3 / 2;
+        k_side3<double2><<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
+            reinterpret_cast<double2 *>(f->utf), reinterpret_cast<double2 *>(f->vtf), n, ctx->kp.oncc);
+    } else {
+        k_side3<double><<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(f->utf, f->vtf, n3, ctx->kp.oncc);
     }
+    k_side2<<<(unsigned)((ctx->n2 + 255) / 256), 256, 0, st>>>(g, ctx->kp, f->h0f, f->akt);
 }
 
 template <int KM>
@@ -828,15 +615,18 @@ void launch_prelude_solve(licom_ctx *ctx, const licom_fields *f, const licom_dia
     hipStream_t st = ctx->stream;
     if (prelude) {
         Timer t(ctx, "k_prelude");
+        // the S half of ori holds wka until k_march writes it
+        double *wka = s.ori + ctx->n3;
         const dim3 b(64, 4), gr((g.imt + 63) / 64, (g.jmt + 3) / 4);
-        k_prelude<KM><<<gr, b, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl, s.work_u, s.stf_pre,
-                                       f->ws, s.uw, s.vs);
+        k_facediv<<<dim3(gr.x, gr.y, g.km), b, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl,
+                                                        s.work_u, s.uw, s.vs, wka);
+        k_wcol<KM><<<gr, b, 0, st>>>(g, wka, s.stf_pre, f->ws);
     } else {
         Timer t(ctx, "k_solve");
-        const dim3 gr((g.imt - 4 + 31) / 32, (g.jmt - 4 + 3) / 4);
-        k_solve<KM><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
+        const dim3 gr = solve_grid(g.imt, g.jmt, 0);
+        k_solve<KM, double, double><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
                                        d ? d->tend : nullptr, d ? d->dt_diff : nullptr,
-                                       d ? d->dt_conv : nullptr, s.mix_count);
+                                       d ? d->dt_conv : nullptr, s.mix_count, 0);
     }
 }
 
@@ -883,10 +673,7 @@ int launch_step_fused(licom_ctx *ctx, const licom_fields *f, const licom_diag *d
         k_march<<<16 * a.tiles_per_xcd, NT, 0, st>>>(g, ctx->kp, a);
     }
     launch_column(ctx, f, d, flags, false);
-    if (flags & LICOM_STEP_SIDE_EFFECTS) {
-        Timer t(ctx, "k_side");
-        k_side<<<4096, 256, 0, st>>>(g, ctx->kp, f->utf, f->vtf, f->h0f, f->akt);
-    }
+    if (flags & LICOM_STEP_SIDE_EFFECTS) launch_side(ctx, f);
     return launch_halo(ctx, f, flags);
 }
 

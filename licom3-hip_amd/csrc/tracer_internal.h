@@ -88,6 +88,9 @@ int launch_surface(licom_ctx *ctx, const licom_fields *f, const licom_diag *d);
 int launch_halo(licom_ctx *ctx, const licom_fields *f, int flags);
 // tracer_fused.hip: prelude / k-march / column solve (production path)
 int launch_step_fused(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags);
+// tracer_mixed.hip: the mixed-precision form
+int launch_step_mixed(licom_ctx *ctx, const licom_fields_mixed *f, const licom_diag_mixed *d,
+                      int flags);
 
 int launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 

@@ -21,6 +21,7 @@
 
 #include "tracer_device.h"
 #include "tracer_internal.h"
+#include "tracer_surface.h"
 
 namespace licom {
 namespace {
@@ -38,123 +39,20 @@ __device__ __forceinline__ double vitf(const KGrid &g, int k, int j, int i) {
 }
 
 // ---------------------------------------------------------------- surface
-// tracer_1_0 (tracer.cpp:1081-1093), T->U average of tracer_1
-// (:1131-1149), surface fluxes of tracer_5 n=1 (:1479-1524, FRC_CORE) and
-// tracer_6 n=0 (:1543-1566, FRC_CORE), SSSNORM integrand (:1517-1522).
-__device__ __forceinline__ double stf0(const double *h0f, const double *h0l, double onbc, size_t q) {
-    return 0.5 * (h0f[q] * onbc) + 0.5 * h0l[q];
+// (bodies in tracer_surface.h)
+__global__ void k_surface(KGrid g, KPar p, SurfaceIO o) {
+    surface_cell(g, p, o, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y);
 }
 
-__global__ void k_surface(KGrid g, KPar p, const double *__restrict__ h0f,
-                          const double *__restrict__ h0l, const double *__restrict__ atb,
-                          const double *swv, const double *nswv, const double *sst,
-                          const double *seaice, const double *sss, const double *fresh,
-                          double *stf_pre, double *work_u, double *stf_n, double *temp11,
-                          double *net, int *mix_count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    if (i == 0 && j == 0 && mix_count) *mix_count = 0;
-    if (i >= g.imt || j >= g.jmt) return;
-    const size_t q = i2(g, j, i), n2 = (size_t)g.imt * g.jmt;
-    const double s0 = stf0(h0f, h0l, p.onbc, q);
-    stf_pre[q] = s0;
-    double wu = 0.0;
-    if (j < g.jmt - 1 && i >= 1) {
-        wu = g.au0[q] * s0 + g.aus[q] * stf0(h0f, h0l, p.onbc, i2(g, j + 1, i)) +
-             g.auw[q] * stf0(h0f, h0l, p.onbc, i2(g, j, i - 1)) +
-             g.ausw[q] * stf0(h0f, h0l, p.onbc, i2(g, j + 1, i - 1));
-    }
-    work_u[q] = wu;
-    double sT = s0, sS = s0, t11 = 0.0;
-    if (g.kmt[q] > 0) {
-        const double odzp0 = g.odzp[0];
-        const double tb = atb[n2 + q];                       // atb(:,:,1,1)
-        const double sb = atb[(size_t)(g.km + 1) * n2 + n2 + q];  // atb(:,:,1,2)
-        const double fw = fresh ? fresh[q] : 0.0, si = seaice ? seaice[q] : 0.0;
-        const double ss = sss ? sss[q] : 0.0, st = sst ? sst[q] : 0.0;
-        const double sw = swv ? swv[q] : 0.0, nsw = nswv ? nswv[q] : 0.0;
-        sS = fw * 34.7 * p.od0 / 1000.0 + p.gamma * (ss - sb) * si / odzp0 +
-             p.gamma * 30.0 / 365.0 / 4.0 * 50.0 * (ss - sb) / odzp0 * (1.0 - si);
-        sT = (sw + nsw) * p.od0cp + si * p.gamma * (st - tb) / odzp0;
-        const double n1 = sS * odzp0;
-        t11 = g.tarea[q] * n1 * 1.0;
-        if (net) {
-            net[q] = sT * odzp0;
-            net[n2 + q] = n1;
-        }
-    }
-    stf_n[q] = sT;
-    stf_n[n2 + q] = sS;
-    temp11[q] = t11;
+__global__ __launch_bounds__(kRedThreads) void k_sss_partial(KGrid g, const double *__restrict__ temp11,
+                                                             double *partials) {
+    sss_partial_block(g, temp11, partials, blockIdx.x, gridDim.x);
 }
 
-// Compensated (TwoSum) accumulation of the SSSNORM integrand over the
-// physical cells; the reference sums on the host in j-major order and then
-// MPI_ALLREDUCEs (tracer.cpp:1849-1868, mpi3.F90:26).  The order here is
-// fixed by the launch shape, so repeated runs give identical bits.
-using dev::two_sum;
-
-constexpr int kRedThreads = 256;
-
-__global__ void k_sss_partial(KGrid g, const double *__restrict__ temp11, double *partials) {
-    __shared__ double sh[kRedThreads], sl[kRedThreads];
-    const int nx = g.imt - 4, ny = g.jmt - 4;
-    const size_t total = (size_t)nx * ny;
-    double hi = 0.0, lo = 0.0;
-    for (size_t t = (size_t)blockIdx.x * kRedThreads + threadIdx.x; t < total;
-         t += (size_t)gridDim.x * kRedThreads) {
-        const int jj = (int)(t / nx), ii = (int)(t % nx);
-        double s, e;
-        two_sum(hi, temp11[i2(g, jj + 2, ii + 2)], s, e);
-        hi = s;
-        lo += e;
-    }
-    sh[threadIdx.x] = hi;
-    sl[threadIdx.x] = lo;
-    __syncthreads();
-    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            double s, e;
-            two_sum(sh[threadIdx.x], sh[threadIdx.x + w], s, e);
-            sh[threadIdx.x] = s;
-            sl[threadIdx.x] = sl[threadIdx.x] + sl[threadIdx.x + w] + e;
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = sh[0];
-        partials[2 * blockIdx.x + 1] = sl[0];
-    }
-}
-
-__global__ void k_sss_final(const double *__restrict__ partials, int nblk, double area_t,
-                            int enabled, double *err_norm2, double *out_copy) {
-    __shared__ double sh[kRedThreads], sl[kRedThreads];
-    double hi = 0.0, lo = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += kRedThreads) {
-        double s, e;
-        two_sum(hi, partials[2 * b], s, e);
-        hi = s;
-        lo += e + partials[2 * b + 1];
-    }
-    sh[threadIdx.x] = hi;
-    sl[threadIdx.x] = lo;
-    __syncthreads();
-    for (int w = kRedThreads / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            double s, e;
-            two_sum(sh[threadIdx.x], sh[threadIdx.x + w], s, e);
-            sh[threadIdx.x] = s;
-            sl[threadIdx.x] = sl[threadIdx.x] + sl[threadIdx.x + w] + e;
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double total = sh[0] + sl[0];
-        const double v = enabled ? -total / area_t : 0.0;
-        *err_norm2 = v;
-        if (out_copy) *out_copy = v;
-    }
+__global__ __launch_bounds__(kRedThreads) void k_sss_final(const double *__restrict__ partials, int nblk,
+                                                           double area_t, int enabled, double *err_norm2,
+                                                           double *out_copy) {
+    sss_final_block(partials, nblk, area_t, enabled, err_norm2, out_copy);
 }
 
 // ------------------------------------------------------------ velocities
@@ -588,28 +486,8 @@ __global__ void k_column(KGrid g, KPar p, int flags, double *at, double *atb, do
 // zero in the closed rows (POP_HaloUpdate with fillValue 0, as
 // pop_haloupdate_tracer2.F90:10 followed by tracer_9's copy).
 __global__ void k_halo(int imt, int jmt, int nplanes, double *at, double *atb, int km) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int plane = blockIdx.y;   // 0..2*km-1 at, 2*km..4*km-1 atb
-    const int ncol = 4 * (jmt - 4), nrow = 4 * imt;
-    if (t >= ncol + nrow || plane >= nplanes) return;
-    const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
-    double *P;
-    if (plane < 2 * km) {
-        P = at + (size_t)plane * n2;
-    } else {
-        const int pl = plane - 2 * km, n = pl / km, k = pl % km;
-        P = atb + n * (n3 + n2) + n2 + (size_t)k * n2;
-    }
-    if (t < ncol) {
-        const int j = 2 + t / 4, w = t % 4;
-        const int dst = (w == 0) ? 0 : (w == 1) ? 1 : (w == 2) ? imt - 2 : imt - 1;
-        const int src = (w == 0) ? imt - 4 : (w == 1) ? imt - 3 : (w == 2) ? 2 : 3;
-        P[(size_t)j * imt + dst] = P[(size_t)j * imt + src];
-    } else {
-        const int r = t - ncol, w = r / imt, i = r % imt;
-        const int j = (w == 0) ? 0 : (w == 1) ? 1 : (w == 2) ? jmt - 2 : jmt - 1;
-        P[(size_t)j * imt + i] = 0.0;
-    }
+    (void)nplanes;
+    halo_cell(imt, jmt, km, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x, at, atb);
 }
 
 // convadj_cu over every column, ghost columns included (convadj.cpp:29-288).
@@ -640,11 +518,12 @@ __global__ void k_convadj(KGrid g, KPar p, double *at, double *atb, double *tend
         }
 }
 
-// 16-byte vector copy used for bandwidth calibration.
-__global__ void k_stream_copy(const double2 *__restrict__ src, double2 *__restrict__ dst, size_t n) {
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (size_t)gridDim.x * blockDim.x)
-        dst[t] = src[t];
+// 16-byte vector copy used for bandwidth calibration: one element per lane,
+// one pass over the grid (faster on gfx950 than a grid-stride loop).
+__global__ __launch_bounds__(256) void k_stream_copy(const double2 *__restrict__ src,
+                                                     double2 *__restrict__ dst, size_t n) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) dst[t] = src[t];
 }
 
 __global__ void k_copy_plane(const double *src, double *dst, size_t n) {
@@ -683,9 +562,9 @@ int launch_surface(licom_ctx *ctx, const licom_fields *f, const licom_diag *d) {
     const dim3 b2(64, 4);
     {
         Timer t(ctx, "k_surface");
-        k_surface<<<grid2(g.imt, g.jmt, b2), b2, 0, st>>>(
-            g, p, f->h0f, f->h0l, f->atb, f->swv, f->nswv, f->sst, f->seaice, f->sss, f->fresh,
-            s.stf_pre, s.work_u, s.stf_n, s.temp11, d ? d->net : nullptr, s.mix_count);
+        SurfaceIO o{f->h0f, f->h0l, f->atb, f->swv, f->nswv, f->sst, f->seaice, f->sss, f->fresh,
+                    s.stf_pre, s.work_u, s.stf_n, s.temp11, d ? d->net : nullptr, s.mix_count};
+        k_surface<<<grid2(g.imt, g.jmt, b2), b2, 0, st>>>(g, p, o);
     }
     {
         Timer t(ctx, "k_sss");
@@ -757,7 +636,13 @@ int launch_step_simple(licom_ctx *ctx, const licom_fields *f, const licom_diag *
 
 int launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {
     const size_t n = bytes / 16;
-    k_stream_copy<<<8192, 256, 0, st>>>(static_cast<const double2 *>(src), static_cast<double2 *>(dst), n);
+    if (n == 0) return LICOM_OK;
+    if ((n + 255) / 256 > 0x7fffffffull) {
+        set_error("licom_stream_copy: %zu bytes exceed one launch", bytes);
+        return LICOM_EINVAL;
+    }
+    k_stream_copy<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(static_cast<const double2 *>(src),
+                                                               static_cast<double2 *>(dst), n);
     return hip_status(hipGetLastError(), "stream copy launch");
 }
 

@@ -1,0 +1,1177 @@
+// tracer_mixed.hip -- mixed-precision form of the tracer step on gfx950.
+//
+// The tracer state (at, atb), the surface fields, the implicit vertical solve,
+// convadj and the SSSNORM sum stay FP64; velocities, ws, akt, the face
+// velocities, the divergence and the 3-D diagnostics are FP32.  The k-march
+// forms every difference of the tracer (neighbour minus the cell's own value)
+// in FP64 and runs the tspas fluxes, the limiter test, the del4/del2 stencils
+// and the explicit vertical diffusion in FP32 on those anomalies; its FP32
+// tendency is added to the FP64 state (ori = at + dts * tf).  Written in this
+// anomaly form, the sums of tspas (x0 + c1, y0 + c2, za + zc of
+// tracer.cpp:311-414) reduce to flux differences with no cancellation of the
+// O(10 K) tracer values, so FP32 keeps ~1e-7 relative accuracy on the
+// increments.
+//
+//   k_facediv_mx / k_wcol_mx<KM>  face velocities, divergence, continuity w
+//   k_march_mx                    five-wave k-march (as the FP64 k_march)
+//   k_solve<KM, float, float>     FP64 invtrit + convadj (tracer_solve.h)
+//   k_side3<float4> / k_side2_mx  in-place side effects
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "tracer_device.h"
+#include "tracer_internal.h"
+#include "tracer_solve.h"
+#include "tracer_surface.h"
+
+namespace licom {
+namespace {
+
+constexpr int KMAXM = LICOM_MAX_KM;
+
+__device__ __forceinline__ float wkf(float uf, float ul, float oncc) {
+    return 0.5f * (uf * oncc) + 0.5f * ul;
+}
+
+// ------------------------------------------------------------- prelude
+// Launch shapes of k_facediv_mx: a 256-lane workgroup covers FX x (256/FX)
+// cells of one level; shape 0: 64 x 4, 1: 32 x 8, 2: 128 x 2.
+__host__ __device__ constexpr int fd_cols(int shape) { return shape == 0 ? 64 : (shape == 1 ? 32 : 128); }
+__host__ __device__ constexpr int fd_rows(int shape) { return 256 / fd_cols(shape); }
+
+__device__ __forceinline__ void facediv_body(const KGrid &g, float oncc, const float *__restrict__ utf,
+                                             const float *__restrict__ utl,
+                                             const float *__restrict__ vtf,
+                                             const float *__restrict__ vtl,
+                                             const double *__restrict__ work_u,
+                                             float *__restrict__ uw, float *__restrict__ vs,
+                                             float *__restrict__ wka, int bx, int by, int k,
+                                             int shape) {
+    // wkd / wkb of the (FX+1) x (FY+1) window (the block plus its east column
+    // and north row), each element read from HBM once
+    __shared__ float sD[400], sBv[400];
+    const int FX = fd_cols(shape), FY = fd_rows(shape), WX = FX + 1;
+    const int t = threadIdx.x, tx = t % FX, ty = t / FX;
+    const int i0 = bx * FX, j0 = by * FY;
+    const int i = i0 + tx, j = j0 + ty;
+    const int imt = g.imt, jmt = g.jmt;
+    const size_t n2 = (size_t)imt * jmt, o = (size_t)k * n2;
+    for (int e = t; e < WX * (FY + 1); e += 256) {
+        const int wr = e / WX, wc = e - wr * WX;
+        const int jj = j0 - 1 + wr, ii = i0 + wc;
+        float d = 0.0f, b = 0.0f;
+        if (jj >= 0 && jj < jmt && ii < imt) {
+            const size_t c = o + (size_t)jj * imt + ii;
+            d = wkf(utf[c], utl[c], oncc);
+            b = wkf(vtf[c], vtl[c], oncc);
+        }
+        sD[e] = d;
+        sBv[e] = b;
+    }
+    __syncthreads();
+    if (i >= imt || j >= jmt) return;
+    const size_t q = (size_t)j * imt + i, c = o + q;
+    const int w0 = (ty + 1) * WX + tx;       // window position of (j, i)
+    const float dc = sD[w0], bc = sBv[w0];
+    float u = 0.0f, v = 0.0f, d = 0.0f;
+    if (j >= 1 && j < jmt - 1) u = (sD[w0 - WX] + dc) * (float)g.htw[q] * 0.25f;
+    if (i >= 1 && i < imt - 1) {
+        const float be = sBv[w0 + 1];
+        v = (bc + be) * (float)g.hts[q] * 0.25f;
+        if (j >= 1 && j < jmt - 1 && k <= g.kmt[q] - 1) {
+            const size_t qe = q + 1, qn = q - imt, qne = q - imt + 1;
+            const float fc = (float)(1.0 + work_u[q] * g.ohbu[q]);
+            const float fe = (float)(1.0 + work_u[qe] * g.ohbu[qe]);
+            const float fn = (float)(1.0 + work_u[qn] * g.ohbu[qn]);
+            const float fne = (float)(1.0 + work_u[qne] * g.ohbu[qne]);
+            const float de = sD[w0 + 1], dn = sD[w0 - WX], dne = sD[w0 - WX + 1];
+            const float bn = sBv[w0 - WX], bne = sBv[w0 - WX + 1];
+            d = 0.5f *
+                ((fe * de + fne * dne) * (float)g.htw[qe] - (fc * dc + fn * dn) * (float)g.htw[q] +
+                 (fe * be + fc * bc) * (float)g.hts[q] - (fne * bne + fn * bn) * (float)g.hts[qn]) *
+                (float)g.tarea_r[q];
+        }
+    }
+    uw[c] = u;
+    vs[c] = v;
+    wka[c] = d;
+}
+
+__global__ __launch_bounds__(256) void k_facediv_mx(KGrid g, float oncc, const float *__restrict__ utf,
+                                                    const float *__restrict__ utl,
+                                                    const float *__restrict__ vtf,
+                                                    const float *__restrict__ vtl,
+                                                    const double *__restrict__ work_u,
+                                                    float *__restrict__ uw, float *__restrict__ vs,
+                                                    float *__restrict__ wka, int shape) {
+    facediv_body(g, oncc, utf, utl, vtf, vtl, work_u, uw, vs, wka, blockIdx.x, blockIdx.y, blockIdx.z,
+                 shape);
+}
+
+// Continuity w (upwell_3/4) with the column sum and the recursion in FP64.
+template <int KM>
+__device__ __forceinline__ void wcol_body(const KGrid &g, const float *__restrict__ wka,
+                                          const double *__restrict__ stf_pre, float *__restrict__ ws,
+                                          int bx, int by) {
+    const int i = bx * 64 + (int)threadIdx.x % 64;
+    const int j = by * 4 + (int)threadIdx.x / 64;
+    const int imt = g.imt, jmt = g.jmt, km = g.km;
+    if (i < 1 || i >= imt - 1 || j < 1 || j >= jmt - 1) return;
+    const size_t n2 = (size_t)imt * jmt, q = (size_t)j * imt + i;
+    const int kmt = g.kmt[q];
+    float d[KM];
+    double w = 0.0;
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+        d[k] = 0.0f;
+        if (k < km && k <= kmt - 1) {
+            d[k] = wka[(size_t)k * n2 + q];
+            w = w - g.dzp[k] * (double)d[k];
+        }
+    }
+    const double ohbt = g.ohbt[q];
+    const double sc = 1.0 / (1.0 + stf_pre[q] * ohbt);
+    double prev = (double)ws[q];
+#pragma unroll
+    for (int k = 1; k < KM; ++k) {
+        if (k < km) {
+            const double cur = (k < kmt ? 1.0 : 0.0) * (prev + g.dzp[k - 1] * (w * ohbt + (double)d[k - 1]));
+            ws[(size_t)k * n2 + q] = (float)(cur * sc);
+            prev = cur;
+        }
+    }
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void k_wcol_mx(KGrid g, const float *__restrict__ wka,
+                                                 const double *__restrict__ stf_pre,
+                                                 float *__restrict__ ws) {
+    wcol_body<KM>(g, wka, stf_pre, ws, blockIdx.x, blockIdx.y);
+}
+
+// --------------------------------------------------------------- march
+constexpr int BX = 16, BY = 16;
+constexpr int PX = BX + 2, PY = BY + 2;
+constexpr int RX = BX + 4, RY = BY + 4;
+constexpr int NOWN = BX * BY, NRING = 2 * (BX + BY), NT = NOWN + NRING;
+constexpr int NP = PX * PY, NR = RX * RY;
+constexpr int NCH = NR / 2, NCI = (NCH + 63) / 64;   // 16-byte chunks of an FP64 window
+constexpr int NCF = (NR + 63) / 64;                  // dword loads of an FP32 window
+constexpr float BIG = 1.0e10f;
+
+struct MarchArgsMx {
+    const double *at, *atb, *stf_n, *err_norm2, *swv, *restore;
+    const float *akt, *uw, *vs, *ws;
+    double *ori;
+    float *ax, *ay, *az, *dz, *pen;
+    int ntx, ntiles, tiles_per_xcd;
+};
+
+static_assert(BX == 16 && BY == 16, "ring_cell assumes a 16x16 tile");
+__device__ __forceinline__ int2 ring_cell(int r) {
+    const int side = r >> 4, o = 1 + (r & 15);
+    const int pj = side == 0 ? 0 : (side == 1 ? PY - 1 : o);
+    const int pi = side <= 1 ? o : (side == 2 ? 0 : PX - 1);
+    return make_int2(pi, pj);
+}
+
+// FP32 metrics of a cell: tarea_r, the LW coefficients 2 dts tarea_r / D
+// (D = htw*hun at i+1, i; hts*hue at j, j-1) and the del2 stencil.
+struct MetricsF {
+    float tr, cx1, cx2, cx3, cx4, dtn, dts, dte, dtw, ahf;
+};
+
+__device__ __forceinline__ MetricsF load_metrics_f(const KGrid &g, int gj, int gi, double dts) {
+    const size_t q = (size_t)gj * g.imt + gi, sj = g.imt;
+    const double tr = g.tarea_r[q];
+    MetricsF m;
+    m.tr = (float)tr;
+    m.cx1 = (float)(2.0 * dts * tr / (g.htw[q + 1] * g.hun[q + 1]));
+    m.cx2 = (float)(2.0 * dts * tr / (g.htw[q] * g.hun[q]));
+    m.cx3 = (float)(2.0 * dts * tr / (g.hts[q] * g.hue[q]));
+    m.cx4 = (float)(2.0 * dts * tr / (g.hts[q - sj] * g.hue[q - sj]));
+    m.dtn = (float)g.dtn[q];
+    m.dts = (float)g.dts[q];
+    m.dte = (float)g.dte[q];
+    m.dtw = (float)g.dtw[q];
+    m.ahf = (float)g.ahf[q];
+    return m;
+}
+
+struct LWf {
+    float xy1, xy2, xy3, xy4, zb1, zb2;
+};
+
+struct LvM {
+    const double *Pk, *Pu, *Pd;
+    const float *U, *V, *Wk, *Wb;
+};
+
+// Anomalies of the six neighbours about the cell value (FP64 differences).
+struct Anom {
+    float e, w, s, n, u, d;
+};
+
+// The advective terms of advection_tracer_2 in anomaly form: xc = x0 + c1,
+// yc = y0 + c2, zc = za + zc (tracer.cpp:311-414).
+__device__ __forceinline__ void cheap_mx(const Anom &a, float Ue, float Uw, float Vs, float Vn,
+                                         float Wk, float Wb, float tr, float odzpk, float &xc,
+                                         float &yc, float &zc) {
+    xc = (a.e * Ue - a.w * Uw) * tr;
+    yc = (a.s * Vs - a.n * Vn) * tr;
+    zc = 0.5f * odzpk * (Wk * a.u - Wb * a.d);
+}
+
+// Predictor and limiter flag of level K at R-plane index r (mixed form).
+__device__ __forceinline__ bool pred_flag_mx(const KGrid &g, int r, const LvM &L, const int *sK,
+                                             int K, float dts, const MetricsF &M, LWf &lw) {
+    const int km = g.km;
+    const double Tc = L.Pk[r];
+    Anom a;
+    a.e = (float)(L.Pk[r + 1] - Tc);
+    a.w = (float)(L.Pk[r - 1] - Tc);
+    a.s = (float)(L.Pk[r + RX] - Tc);
+    a.n = (float)(L.Pk[r - RX] - Tc);
+    a.u = K > 0 ? (float)(L.Pu[r] - Tc) : 0.0f;
+    a.d = K < km - 1 ? (float)(L.Pd[r] - Tc) : 0.0f;
+    const float Ue = L.U[r + 1], Uw = L.U[r], Vs = L.V[r], Vn = L.V[r - RX];
+    const float Wk = K > 0 ? L.Wk[r] : 0.0f, Wb = K < km - 1 ? L.Wb[r] : 0.0f;
+    const float odzpk = (float)g.odzp[K];
+    float xc, yc, zc;
+    cheap_mx(a, Ue, Uw, Vs, Vn, Wk, Wb, M.tr, odzpk, xc, yc, zc);
+    lw.xy1 = -(a.e * Ue * Ue) * M.cx1;
+    lw.xy2 = -(a.w * Uw * Uw) * M.cx2;
+    lw.xy3 = -(a.s * Vs * Vs) * M.cx3;
+    lw.xy4 = -(a.n * Vn * Vn) * M.cx4;
+    lw.zb1 = K > 0 ? -0.5f * odzpk * Wk * Wk * (float)g.odzt[K] * a.u * dts : 0.0f;
+    lw.zb2 = K < km - 1 ? -0.5f * odzpk * Wb * Wb * (float)g.odzt[K + 1] * a.d * dts : 0.0f;
+    const float xx = -(xc + lw.xy1 + lw.xy2);
+    const float yy = -(yc + lw.xy3 + lw.xy4);
+    const float zz = -(lw.zb1 + lw.zb2 + zc);
+    const float d00 = (xx + yy + zz) * dts;
+    // 7-point bounds about the cell value, land at -+1e10 (tracer.cpp:416-545)
+    float mx = sK[r] > K ? 0.0f : -BIG, mn = sK[r] > K ? 0.0f : BIG;
+    const int nb[4] = {r - RX, r + RX, r - 1, r + 1};
+    const float tv[4] = {a.n, a.s, a.w, a.e};
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const bool wet = sK[nb[m]] > K;
+        mx = fmaxf(mx, wet ? tv[m] : -BIG);
+        mn = fminf(mn, wet ? tv[m] : BIG);
+    }
+    if (K > 0) {
+        const bool wet = sK[r] > K - 1;
+        mx = fmaxf(mx, wet ? a.u : -BIG);
+        mn = fminf(mn, wet ? a.u : BIG);
+    }
+    if (K < km - 1) {
+        const bool wet = sK[r] > K + 1;
+        mx = fmaxf(mx, wet ? a.d : -BIG);
+        mn = fminf(mn, wet ? a.d : BIG);
+    }
+    return d00 > mx || d00 < mn;
+}
+
+// del4 first pass, anomaly form: AHF * sum over wet neighbours of DT*(B_nbr - B).
+__device__ __forceinline__ float d2tk_mx(int r, const double *B, const int *sK, int k,
+                                         const MetricsF &M) {
+    float h = 0.0f;
+    if (k <= sK[r] - 1) {
+        const double Bc = B[r];
+        if (k <= sK[r - RX] - 1) h += M.dtn * (float)(B[r - RX] - Bc);
+        if (k <= sK[r + RX] - 1) h += M.dts * (float)(B[r + RX] - Bc);
+        if (k <= sK[r + 1] - 1) h += M.dte * (float)(B[r + 1] - Bc);
+        if (k <= sK[r - 1] - 1) h += M.dtw * (float)(B[r - 1] - Bc);
+    }
+    return M.ahf * h;
+}
+
+#define LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
+
+__device__ __forceinline__ void stage_d(const double *src, double *dst, const int (&off)[NCI],
+                                        int lane) {
+#pragma unroll
+    for (int u = 0; u < NCI; ++u)
+        if (u * 64 + lane < NCH)
+            __builtin_amdgcn_global_load_lds(src + off[u], LDS_PTR(dst + 128 * u), 16, 0, 0);
+}
+
+__device__ __forceinline__ void stage_f(const float *src, float *dst, const int (&off)[NCF],
+                                        int lane) {
+#pragma unroll
+    for (int u = 0; u < NCF; ++u)
+        if (u * 64 + lane < NR)
+            __builtin_amdgcn_global_load_lds(src + off[u], LDS_PTR(dst + 64 * u), 4, 0, 0);
+}
+
+__device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+__device__ __forceinline__ void bar_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+__device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const MarchArgsMx &a, int tile,
+                                           int n) {
+    __shared__ __align__(16) double sAT[4][NR];
+    __shared__ __align__(16) double sB[3][NR];
+    __shared__ __align__(16) float sU[3][NR];
+    __shared__ __align__(16) float sV[3][NR];
+    __shared__ __align__(16) float sW[4][NR];
+    __shared__ float sD2[NP];
+    __shared__ unsigned char sFL[2][NP];
+    __shared__ int sK[NR];
+
+    const int imt = g.imt, jmt = g.jmt, km = g.km;
+    const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
+    const int gi0 = 2 + (tile % a.ntx) * BX, gj0 = 2 + (tile / a.ntx) * BY;
+    const long e0 = (long)(gj0 - 2) * imt + (gi0 - 2);
+
+    const int t = threadIdx.x, lane = t & 63;
+    const bool ringw = t >= NOWN;
+    const int2 rc = ring_cell((t - NOWN) & 63);
+    const int pi = ringw ? rc.x : 1 + t % BX, pj = ringw ? rc.y : 1 + t / BX;
+    const int gi = gi0 - 1 + pi, gj = gj0 - 1 + pj;
+    const int rC = (pj + 1) * RX + (pi + 1), pC = pj * PX + pi;
+    const bool cell_ok = gj >= 1 && gj < jmt - 1 && gi >= 1 && gi < imt - 1;
+    const bool act = !ringw && gi < imt - 2 && gj < jmt - 2;
+    const size_t q = cell_ok ? (size_t)gj * imt + gi : 0;
+
+    const double *T = a.at + n * n3;
+    const double *TB = a.atb + n * (n3 + n2) + n2;
+    const float *A = a.akt + n * n3;
+    const double dts = p.dts;
+    const float dtsf = (float)dts;
+
+    for (int r = t; r < NR; r += NT) {
+        const int rj = gj0 - 2 + r / RX, ri = gi0 - 2 + r % RX;
+        sK[r] = (rj < jmt && ri < imt) ? g.kmt[(size_t)rj * imt + ri] : 0;
+    }
+    MetricsF M{};
+    if (cell_ok) M = load_metrics_f(g, gj, gi, dts);
+
+    if (ringw) {
+        int offd[NCI], offf[NCF];
+#pragma unroll
+        for (int u = 0; u < NCI; ++u) {
+            const int c = min(u * 64 + lane, NCH - 1), row = c / (RX / 2), cp = c - row * (RX / 2);
+            offd[u] = (int)min(e0 + (long)row * imt + 2 * cp, (long)n2 - 2);
+        }
+#pragma unroll
+        for (int u = 0; u < NCF; ++u) {
+            const int c = min(u * 64 + lane, NR - 1), row = c / RX, col = c - row * RX;
+            offf[u] = (int)min(e0 + (long)row * imt + col, (long)n2 - 1);
+        }
+        for (int l = 0; l < 3 && l < km; ++l) {
+            stage_d(T + (size_t)l * n2, sAT[l], offd, lane);
+            stage_f(a.ws + (size_t)l * n2, sW[l], offf, lane);
+        }
+        for (int l = 0; l < 2 && l < km; ++l) {
+            stage_f(a.uw + (size_t)l * n2, sU[l], offf, lane);
+            stage_f(a.vs + (size_t)l * n2, sV[l], offf, lane);
+            stage_d(TB + (size_t)l * n2, sB[l], offd, lane);
+        }
+        wait_vmem();
+        bar_lds();
+        {
+            LWf tmp;
+            const LvM L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
+            sFL[0][pC] = cell_ok ? pred_flag_mx(g, rC, L0, sK, 0, dtsf, M, tmp) : false;
+        }
+        bar_lds();
+        for (int k = 0; k < km; ++k) {
+            const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3, s3 = (k + 3) & 3;
+            const int u0 = k % 3, u1 = (k + 1) % 3, u2 = (k + 2) % 3;
+            if (k + 1 < km) {
+                LWf tmp;
+                const LvM L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
+                sFL[(k + 1) & 1][pC] = cell_ok ? pred_flag_mx(g, rC, L1, sK, k + 1, dtsf, M, tmp) : false;
+            }
+            if (p.hmix == LICOM_HMIX_DEL4) sD2[pC] = cell_ok ? d2tk_mx(rC, sB[u0], sK, k, M) : 0.0f;
+            if (k + 3 < km) {
+                stage_d(T + (size_t)(k + 3) * n2, sAT[s3], offd, lane);
+                stage_f(a.ws + (size_t)(k + 3) * n2, sW[s3], offf, lane);
+            }
+            if (k + 2 < km) {
+                stage_f(a.uw + (size_t)(k + 2) * n2, sU[u2], offf, lane);
+                stage_f(a.vs + (size_t)(k + 2) * n2, sV[u2], offf, lane);
+                stage_d(TB + (size_t)(k + 2) * n2, sB[u2], offd, lane);
+            }
+            bar_lds();
+            wait_vmem();
+            bar_lds();
+        }
+        return;
+    }
+
+    const double sw = (n == 0 && act && a.swv) ? a.swv[q] : 0.0;
+    const int kmt_own = act ? g.kmt[q] : 0;
+    const double stf_n = act ? a.stf_n[n * n2 + q] : 0.0;
+    const double enorm = (n == 1 && p.sssnorm && act) ? a.err_norm2[0] : 0.0;
+    const bool rest = p.boundary_restore == 1 && a.restore != nullptr;
+    float fA = act ? A[q] : 0.0f;
+    double fR = (rest && act) ? a.restore[n * n3 + q] : 0.0;
+    bar_lds();
+    LWf lc{}, ln{};
+    {
+        const LvM L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
+        sFL[0][pC] = cell_ok ? pred_flag_mx(g, rC, L0, sK, 0, dtsf, M, lc) : false;
+    }
+    bar_lds();
+
+    double Tprev = 0.0, Bprev = 0.0;
+    float Aprev = 0.0f;
+    bool fprev = false;
+    const double lamda = 1.0 / (15.0 * 86400.0);
+    const float ah = (float)p.ah;
+
+    for (int k = 0; k < km; ++k) {
+        const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3;
+        const int u0 = k % 3, u1 = (k + 1) % 3;
+        const float Acur = act ? (float)dev::floor_akt((double)fA, k, n, p.hmix, p.dwndmix) : 0.0f;
+        const double Rk = fR;
+        if (k + 1 < km) {
+            fA = act ? A[(size_t)(k + 1) * n2 + q] : 0.0f;
+            fR = (rest && act) ? a.restore[n * n3 + (size_t)(k + 1) * n2 + q] : 0.0;
+            const LvM L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
+            sFL[(k + 1) & 1][pC] = cell_ok ? pred_flag_mx(g, rC, L1, sK, k + 1, dtsf, M, ln) : false;
+        }
+        if (p.hmix == LICOM_HMIX_DEL4) sD2[pC] = cell_ok ? d2tk_mx(rC, sB[u0], sK, k, M) : 0.0f;
+        bar_lds();
+
+        if (act) {
+            const double *Pk = sAT[s0], *Pk1 = sAT[s1];
+            const double *Bk = sB[u0], *Bk1 = sB[u1];
+            const unsigned char *F = sFL[k & 1];
+            const int r = rC, pp = pC;
+            const double Tc = Pk[r];
+            Anom an;
+            an.e = (float)(Pk[r + 1] - Tc);
+            an.w = (float)(Pk[r - 1] - Tc);
+            an.s = (float)(Pk[r + RX] - Tc);
+            an.n = (float)(Pk[r - RX] - Tc);
+            an.u = k > 0 ? (float)(Tprev - Tc) : 0.0f;
+            an.d = k + 1 < km ? (float)(Pk1[r] - Tc) : 0.0f;
+            const float *U = sU[u0], *V = sV[u0];
+            const float Ue = U[r + 1], Uw = U[r], Vs = V[r], Vn = V[r - RX];
+            const float Wk = k > 0 ? sW[s0][r] : 0.0f, Wb = (k + 1 < km) ? sW[s1][r] : 0.0f;
+            const bool fc = F[pp], fe = F[pp + 1], fw = F[pp - 1], fs = F[pp + PX], fn = F[pp - PX];
+            const bool fd = (k + 1 < km) ? sFL[(k + 1) & 1][pp] : false;
+            bool up1, up2, up3, up4;
+            if (k == 0 || k == km - 1) {
+                up1 = fc || (fe && gi <= imt - 3);
+                up2 = fc || (fw && gi >= 2);
+                up3 = fc || (fs && gj <= jmt - 3);
+                up4 = fc || (fn && gj >= 2);
+            } else {
+                up1 = (fc || fe) && gi <= imt - 3;
+                up2 = (fc || fw) && gi >= 2;
+                up3 = (fc || fs) && gj <= jmt - 3;
+                up4 = (fc || fn) && gj >= 2;
+            }
+            const float tr = M.tr;
+            const float odzpk = (float)g.odzp[k];
+            float xc, yc, zc;
+            cheap_mx(an, Ue, Uw, Vs, Vn, Wk, Wb, tr, odzpk, xc, yc, zc);
+            const float xy1 = up1 ? -an.e * fabsf(Ue) * tr : lc.xy1;
+            const float xy2 = up2 ? -an.w * fabsf(Uw) * tr : lc.xy2;
+            const float xy3 = up3 ? -an.s * fabsf(Vs) * tr : lc.xy3;
+            const float xy4 = up4 ? -an.n * fabsf(Vn) * tr : lc.xy4;
+            float zb1 = 0.0f, zb2 = 0.0f;
+            if (k > 0) zb1 = (fc || fprev) ? -0.5f * fabsf(Wk) * odzpk * an.u : lc.zb1;
+            if (k + 1 < km) zb2 = (fc || fd) ? -0.5f * fabsf(Wb) * odzpk * an.d : lc.zb2;
+            const float xx = -(xc + xy1 + xy2);
+            const float yy = -(yc + xy3 + xy4);
+            const float zz = -(zc + zb1 + zb2);
+            const size_t c = (size_t)k * n2 + q;
+            if (a.ax) a.ax[n * n3 + c] = xx;
+            if (a.ay) a.ay[n * n3 + c] = yy;
+            if (a.az) a.az[n * n3 + c] = zz;
+            const float vk = k < kmt_own ? 1.0f : 0.0f;
+            float tf = (xx + yy + zz) * vk;
+            // horizontal mixing (tracer_del4_2 / tracer_4)
+            {
+                float h = 0.0f;
+                if (p.hmix == LICOM_HMIX_DEL4) {
+                    if (k <= sK[r] - 1) {
+                        float cc = 0.0f;
+                        if (k <= sK[r - RX] - 1) { h += M.dtn * sD2[pp - PX]; cc -= M.dtn; }
+                        if (k <= sK[r + RX] - 1) { h += M.dts * sD2[pp + PX]; cc -= M.dts; }
+                        if (k <= sK[r + 1] - 1) { h += M.dte * sD2[pp + 1]; cc -= M.dte; }
+                        if (k <= sK[r - 1] - 1) { h += M.dtw * sD2[pp - 1]; cc -= M.dtw; }
+                        h = cc * sD2[pp] + h;
+                    }
+                } else if (k <= sK[r] - 1) {
+                    const double Bc0 = Bk[r];
+                    if (k <= sK[r - RX] - 1) h += M.dtn * (float)(Bk[r - RX] - Bc0);
+                    if (k <= sK[r + RX] - 1) h += M.dts * (float)(Bk[r + RX] - Bc0);
+                    if (k <= sK[r + 1] - 1) h += M.dte * (float)(Bk[r + 1] - Bc0);
+                    if (k <= sK[r - 1] - 1) h += M.dtw * (float)(Bk[r - 1] - Bc0);
+                }
+                tf += ah * h;
+            }
+            // shortwave penetration (T) and explicit vertical diffusion
+            const float v1 = (k + 1 < km && k + 1 < kmt_own) ? 1.0f : 0.0f;
+            if (n == 0) {
+                float pv;
+                const float swf = (float)sw;
+                if (k == 0) {
+                    pv = -swf * (float)g.pen[0] * (1 < kmt_own ? 1.0f : 0.0f) * odzpk;
+                } else if (k == km - 1) {
+                    pv = swf * (float)g.pen[km - 2] * vk * odzpk;
+                } else {
+                    pv = (swf * (float)g.pen[k - 1] * vk - swf * (float)g.pen[k] * v1) * odzpk;
+                }
+                tf += pv;
+                if (a.pen) a.pen[c] = pv;
+            }
+            const double Bc = Bk[r];
+            float dzv;
+            if (k == 0) {
+                dzv = -odzpk * Acur * (float)(Bc - Bk1[r]) * (float)g.odzt[1] * v1 * 0.5f;
+            } else if (k == km - 1) {
+                dzv = odzpk * Aprev * (float)(Bprev - Bc) * (float)g.odzt[km - 1] * vk * 0.5f;
+            } else {
+                const float wt1 = Aprev * (float)(Bprev - Bc) * (float)g.odzt[k] * vk;
+                const float wt2 = Acur * (float)(Bc - Bk1[r]) * (float)g.odzt[k + 1] * v1;
+                dzv = odzpk * (wt1 - wt2) * 0.5f;
+            }
+            tf += dzv;
+            if (a.dz) a.dz[n * n3 + c] = dzv;
+            if (k == 0 && kmt_own > 0) {
+                tf += (float)(stf_n * 0.5 * g.odzp[0]);
+                if (n == 1 && p.sssnorm) tf += (float)enorm * vk;
+            } else if (k == 0 && n == 1 && p.sssnorm) {
+                tf += (float)enorm * vk;
+            }
+            if (rest && k >= 1) tf += vk * (float)((Rk - Bc) * lamda);
+            a.ori[n * n3 + c] = Tc + dts * (double)tf;
+            Tprev = Tc;
+            Bprev = Bc;
+            fprev = fc;
+        }
+        Aprev = Acur;
+        lc = ln;
+        bar_lds();
+    }
+}
+
+__global__ __launch_bounds__(NT, 3) void k_march_mx(KGrid g, KPar p, MarchArgsMx a) {
+    // XCD-aware placement: consecutive blocks of one XCD take the T and S
+    // halves of one tile, then the next tile of the same row of tiles.
+    const int Lb = blockIdx.x, xcd = Lb & 7, slot = Lb >> 3;
+    const int tile = xcd * a.tiles_per_xcd + (slot >> 1);
+    if (tile >= a.ntiles) return;
+    march_body(g, p, a, tile, slot & 1);
+}
+
+// --------------------------------------------------------------- side
+__global__ __launch_bounds__(256) void k_side3_mx(float4 *__restrict__ utf, float4 *__restrict__ vtf,
+                                                  size_t n, float oncc) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    float4 a = utf[t], b = vtf[t];
+    a.x *= oncc; a.y *= oncc; a.z *= oncc; a.w *= oncc;
+    b.x *= oncc; b.y *= oncc; b.z *= oncc; b.w *= oncc;
+    utf[t] = a;
+    vtf[t] = b;
+}
+
+__global__ __launch_bounds__(256) void k_side1_mx(float *__restrict__ utf, float *__restrict__ vtf,
+                                                  size_t n, float oncc) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    utf[t] *= oncc;
+    vtf[t] *= oncc;
+}
+
+__global__ __launch_bounds__(256) void k_side2_mx(KGrid g, KPar p, double *h0f, float *akt) {
+    const size_t n2 = (size_t)g.imt * g.jmt, n3 = n2 * g.km;
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n2) return;
+    h0f[t] = h0f[t] * p.onbc;
+    const int j = (int)(t / g.imt), i = (int)(t % g.imt);
+    if (i >= 1 && i < g.imt - 1 && j >= 1 && j < g.jmt - 1)
+        for (int n = 0; n < 2; ++n)
+            akt[n * n3 + t] = (float)dev::floor_akt((double)akt[n * n3 + t], 0, n, p.hmix, p.dwndmix);
+}
+
+template <int KM>
+void launch_prelude_solve_mx(licom_ctx *ctx, const licom_fields_mixed *f, const licom_diag_mixed *d,
+                             int flags, bool prelude) {
+    const KGrid &g = ctx->g;
+    const KPar &p = ctx->kp;
+    Scratch &s = ctx->s;
+    hipStream_t st = ctx->stream;
+    float *uw = reinterpret_cast<float *>(s.uw), *vs = reinterpret_cast<float *>(s.vs);
+    if (prelude) {
+        Timer t(ctx, "k_prelude");
+        float *wka = reinterpret_cast<float *>(s.ori + ctx->n3);
+        const dim3 gr((g.imt + 63) / 64, (g.jmt + 3) / 4);
+        k_facediv_mx<<<dim3(gr.x, gr.y, g.km), 256, 0, st>>>(g, (float)p.oncc, f->utf, f->utl, f->vtf,
+                                                             f->vtl, s.work_u, uw, vs, wka, 0);
+        k_wcol_mx<KM><<<gr, 256, 0, st>>>(g, wka, s.stf_pre, f->ws);
+    } else {
+        Timer t(ctx, "k_solve");
+        const dim3 gr = solve_grid(g.imt, g.jmt, 0);
+        k_solve<KM, float, float><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
+                                                      d ? d->tend : nullptr, d ? d->dt_diff : nullptr,
+                                                      d ? d->dt_conv : nullptr, s.mix_count, 0);
+    }
+}
+
+void launch_column_mx(licom_ctx *ctx, const licom_fields_mixed *f, const licom_diag_mixed *d, int flags,
+                      bool prelude) {
+    switch (ctx->km) {
+        case 30: launch_prelude_solve_mx<30>(ctx, f, d, flags, prelude); break;
+        case 55: launch_prelude_solve_mx<55>(ctx, f, d, flags, prelude); break;
+        default: launch_prelude_solve_mx<KMAXM>(ctx, f, d, flags, prelude); break;
+    }
+}
+
+}  // namespace
+
+int launch_step_mixed(licom_ctx *ctx, const licom_fields_mixed *f, const licom_diag_mixed *d, int flags) {
+    const KGrid &g = ctx->g;
+    Scratch &s = ctx->s;
+    hipStream_t st = ctx->stream;
+    // the FP64 surface, SSSNORM and halo kernels only touch FP64 fields
+    licom_fields f64{};
+    f64.at = f->at;
+    f64.atb = f->atb;
+    f64.h0f = f->h0f;
+    f64.h0l = f->h0l;
+    f64.swv = f->swv;
+    f64.nswv = f->nswv;
+    f64.sst = f->sst;
+    f64.seaice = f->seaice;
+    f64.sss = f->sss;
+    f64.fresh = f->fresh;
+    f64.restore = f->restore;
+    licom_diag d64{};
+    if (d) {
+        d64.net = d->net;
+        d64.fw_norm2 = d->fw_norm2;
+    }
+    if (int rc = launch_surface(ctx, &f64, &d64)) return rc;
+    launch_column_mx(ctx, f, d, flags, true);
+    {
+        Timer t(ctx, "k_march");
+        MarchArgsMx a{};
+        a.at = f->at;
+        a.atb = f->atb;
+        a.akt = f->akt;
+        a.uw = reinterpret_cast<const float *>(s.uw);
+        a.vs = reinterpret_cast<const float *>(s.vs);
+        a.ws = f->ws;
+        a.stf_n = s.stf_n;
+        a.err_norm2 = s.err_norm2;
+        a.swv = f->swv;
+        a.restore = f->restore;
+        a.ori = s.ori;
+        a.ax = d ? d->ax : nullptr;
+        a.ay = d ? d->ay : nullptr;
+        a.az = d ? d->az : nullptr;
+        a.dz = d ? d->dz : nullptr;
+        a.pen = d ? d->penetrate : nullptr;
+        a.ntx = (g.imt - 4 + BX - 1) / BX;
+        const int nty = (g.jmt - 4 + BY - 1) / BY;
+        a.ntiles = a.ntx * nty;
+        a.tiles_per_xcd = (a.ntiles + 7) / 8;
+        k_march_mx<<<16 * a.tiles_per_xcd, NT, 0, st>>>(g, ctx->kp, a);
+    }
+    launch_column_mx(ctx, f, d, flags, false);
+    if (flags & LICOM_STEP_SIDE_EFFECTS) {
+        Timer t(ctx, "k_side");
+        const size_t n3 = ctx->n3;
+        const float oncc = (float)ctx->kp.oncc;
+        const bool vec = ((reinterpret_cast<uintptr_t>(f->utf) | reinterpret_cast<uintptr_t>(f->vtf)) & 15) == 0 &&
+                         n3 % 4 == 0;
+        if (vec) {
+            const size_t n = n3 / 4;
+            k_side3_mx<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(
+                reinterpret_cast<float4 *>(f->utf), reinterpret_cast<float4 *>(f->vtf), n, oncc);
+        } else {
+            k_side1_mx<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(f->utf, f->vtf, n3, oncc);
+        }
+        k_side2_mx<<<(unsigned)((ctx->n2 + 255) / 256), 256, 0, st>>>(g, ctx->kp, f->h0f, f->akt);
+    }
+    return launch_halo(ctx, &f64, flags & ~LICOM_STEP_VTL_SCRATCH);
+}
+
+// ================================================================ ensemble
+namespace {
+
+// One member of a batch: its grid, parameters, fields, diagnostics, scratch
+// and launch decomposition, read by every batched kernel.
+struct MemberDev {
+    KGrid g;
+    KPar p;
+    double *at, *atb, *h0f;
+    const double *h0l, *swv, *nswv, *sst, *seaice, *sss, *fresh, *restore;
+    float *akt, *utf, *vtf;
+    const float *utl, *vtl;
+    float *ws;
+    float *ax, *ay, *az, *dz, *tend, *dt_diff, *dt_conv, *pen;
+    double *net, *fw_norm2;
+    double *stf_pre, *work_u, *stf_n, *temp11, *partials, *err_norm2, *ori;
+    float *uw, *vs, *wka;
+    int *mix_count;
+    double area_t;
+    int nblk;
+    int sf_bx, sf_by;                 // 64 x 4 blocks of the 2-D kernels
+    int fd_shape, fd_bx, fd_by;       // faces/divergence blocks per level
+    int sv_shape, sv_bx, sv_by;       // column-solve blocks
+    int ntx, ntiles;                  // 16 x 16 k-march tiles
+    int side_vec, side_blocks, side2_blocks;
+    int halo_bx;                      // halo blocks per plane
+};
+
+enum { ST_SURF, ST_SSSP, ST_SSSF, ST_FD, ST_WC, ST_MARCH, ST_SOLVE, ST_SIDE3, ST_SIDE2, ST_HALO,
+       NSTAGE };
+
+// Member owning global block b of a stage (starts: nm+1 prefix sums).
+__device__ __forceinline__ int find_member(const int *st, int nm, int b) {
+    int lo = 0, hi = nm - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (st[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+#define LICOM_MEMBER(stage)                                        \
+    const int *st_ = starts + (stage) * (nm + 1);                  \
+    const int m_ = find_member(st_, nm, (int)blockIdx.x);          \
+    const MemberDev &d = M[m_];                                    \
+    const int l = (int)blockIdx.x - st_[m_];
+
+__global__ __launch_bounds__(256) void kb_surface(const MemberDev *M, const int *starts, int nm) {
+    LICOM_MEMBER(ST_SURF)
+    const SurfaceIO o{d.h0f, d.h0l, d.atb, d.swv, d.nswv, d.sst, d.seaice, d.sss, d.fresh,
+                      d.stf_pre, d.work_u, d.stf_n, d.temp11, d.net, d.mix_count};
+    const int bx = l % d.sf_bx, by = l / d.sf_bx;
+    surface_cell(d.g, d.p, o, bx * 64 + (int)threadIdx.x % 64, by * 4 + (int)threadIdx.x / 64);
+}
+
+__global__ __launch_bounds__(kRedThreads) void kb_sss_partial(const MemberDev *M, const int *starts, int nm) {
+    LICOM_MEMBER(ST_SSSP)
+    sss_partial_block(d.g, d.temp11, d.partials, l, d.nblk);
+}
+
+__global__ __launch_bounds__(kRedThreads) void kb_sss_final(const MemberDev *M, const int *starts, int nm) {
+    LICOM_MEMBER(ST_SSSF)
+    (void)l;
+    sss_final_block(d.partials, d.nblk, d.area_t, d.p.sssnorm, d.err_norm2, d.fw_norm2);
+}
+
+__global__ __launch_bounds__(256) void kb_facediv(const MemberDev *M, const int *starts, int nm) {
+    LICOM_MEMBER(ST_FD)
+    const int per = d.fd_bx * d.fd_by, k = l / per, r = l - k * per;
+    facediv_body(d.g, (float)d.p.oncc, d.utf, d.utl, d.vtf, d.vtl, d.work_u, d.uw, d.vs, d.wka,
+                 r % d.fd_bx, r / d.fd_bx, k, d.fd_shape);
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void kb_wcol(const MemberDev *M, const int *starts, int nm) {
+    LICOM_MEMBER(ST_WC)
+    wcol_body<KM>(d.g, d.wka, d.stf_pre, d.ws, l % d.sf_bx, l / d.sf_bx);
+}
+
+// Persistent k-march: each workgroup walks the (member, tile, tracer) list.
+__global__ __launch_bounds__(NT, 3) void kb_march(const MemberDev *M, const int *starts, int nm,
+                                                  int total) {
+    const int *st_ = starts + ST_MARCH * (nm + 1);
+    for (int it = blockIdx.x; it < total; it += gridDim.x) {
+        const int m = find_member(st_, nm, it);
+        const MemberDev &d = M[m];
+        const int l = it - st_[m];
+        MarchArgsMx a;
+        a.at = d.at;
+        a.atb = d.atb;
+        a.stf_n = d.stf_n;
+        a.err_norm2 = d.err_norm2;
+        a.swv = d.swv;
+        a.restore = d.restore;
+        a.akt = d.akt;
+        a.uw = d.uw;
+        a.vs = d.vs;
+        a.ws = d.ws;
+        a.ori = d.ori;
+        a.ax = d.ax;
+        a.ay = d.ay;
+        a.az = d.az;
+        a.dz = d.dz;
+        a.pen = d.pen;
+        a.ntx = d.ntx;
+        a.ntiles = d.ntiles;
+        a.tiles_per_xcd = 0;
+        march_body(d.g, d.p, a, l >> 1, l & 1);
+    }
+}
+
+template <int KM>
+__global__ __launch_bounds__(256) void kb_solve(const MemberDev *M, const int *starts, int nm, int flags) {
+    LICOM_MEMBER(ST_SOLVE)
+    solve_body<KM, float, float>(d.g, d.p, flags, d.at, d.atb, d.akt, d.ori, d.stf_n, d.tend,
+                                 d.dt_diff, d.dt_conv, d.mix_count, l % d.sv_bx, l / d.sv_bx,
+                                 d.sv_shape);
+}
+
+__global__ __launch_bounds__(256) void kb_side3(const MemberDev *M, const int *starts, int nm) {
+    LICOM_MEMBER(ST_SIDE3)
+    const size_t n3 = (size_t)d.g.imt * d.g.jmt * d.g.km;
+    const size_t t = (size_t)l * 256 + threadIdx.x;
+    const float oncc = (float)d.p.oncc;
+    if (d.side_vec) {
+        if (t >= n3 / 4) return;
+        float4 *u = reinterpret_cast<float4 *>(d.utf), *v = reinterpret_cast<float4 *>(d.vtf);
+        float4 a = u[t], b = v[t];
+        a.x *= oncc; a.y *= oncc; a.z *= oncc; a.w *= oncc;
+        b.x *= oncc; b.y *= oncc; b.z *= oncc; b.w *= oncc;
+        u[t] = a;
+        v[t] = b;
+    } else {
+        if (t >= n3) return;
+        d.utf[t] *= oncc;
+        d.vtf[t] *= oncc;
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_side2(const MemberDev *M, const int *starts, int nm) {
+    LICOM_MEMBER(ST_SIDE2)
+    const KGrid &g = d.g;
+    const size_t n2 = (size_t)g.imt * g.jmt, n3 = n2 * g.km;
+    const size_t t = (size_t)l * 256 + threadIdx.x;
+    if (t >= n2) return;
+    d.h0f[t] = d.h0f[t] * d.p.onbc;
+    const int j = (int)(t / g.imt), i = (int)(t % g.imt);
+    if (i >= 1 && i < g.imt - 1 && j >= 1 && j < g.jmt - 1)
+        for (int n = 0; n < 2; ++n)
+            d.akt[n * n3 + t] = (float)dev::floor_akt((double)d.akt[n * n3 + t], 0, n, d.p.hmix, d.p.dwndmix);
+}
+
+__global__ __launch_bounds__(256) void kb_halo(const MemberDev *M, const int *starts, int nm) {
+    LICOM_MEMBER(ST_HALO)
+    halo_cell(d.g.imt, d.g.jmt, d.g.km, l / d.halo_bx, (l % d.halo_bx) * 256 + (int)threadIdx.x, d.at,
+              d.atb);
+}
+
+#undef LICOM_MEMBER
+
+}  // namespace
+}  // namespace licom
+
+using licom::MemberDev;
+
+struct licom_batch {
+    int n = 0;
+    int kmax = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    MemberDev *d_members = nullptr;
+    int *d_starts = nullptr;
+    int totals[licom::NSTAGE] = {};
+    int march_grid = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int batch_launch_step(licom_batch *b, int flags) {
+    using namespace licom;
+    const int nm = b->n;
+    hipStream_t st = b->stream;
+    const MemberDev *M = b->d_members;
+    const int *S = b->d_starts;
+    auto grid = [&](int stage) { return (unsigned)b->totals[stage]; };
+    kb_surface<<<grid(ST_SURF), 256, 0, st>>>(M, S, nm);
+    kb_sss_partial<<<grid(ST_SSSP), kRedThreads, 0, st>>>(M, S, nm);
+    kb_sss_final<<<grid(ST_SSSF), kRedThreads, 0, st>>>(M, S, nm);
+    kb_facediv<<<grid(ST_FD), 256, 0, st>>>(M, S, nm);
+    if (b->kmax <= 30) kb_wcol<30><<<grid(ST_WC), 256, 0, st>>>(M, S, nm);
+    else if (b->kmax <= 55) kb_wcol<55><<<grid(ST_WC), 256, 0, st>>>(M, S, nm);
+    else kb_wcol<LICOM_MAX_KM><<<grid(ST_WC), 256, 0, st>>>(M, S, nm);
+    const int items = b->totals[ST_MARCH];
+    const int mg = b->march_grid > 0 ? std::min(b->march_grid, items) : items;
+    kb_march<<<(unsigned)mg, NT, 0, st>>>(M, S, nm, items);
+    if (b->kmax <= 30) kb_solve<30><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
+    else if (b->kmax <= 55) kb_solve<55><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
+    else kb_solve<LICOM_MAX_KM><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
+    if (flags & LICOM_STEP_SIDE_EFFECTS) {
+        kb_side3<<<grid(ST_SIDE3), 256, 0, st>>>(M, S, nm);
+        kb_side2<<<grid(ST_SIDE2), 256, 0, st>>>(M, S, nm);
+    }
+    kb_halo<<<grid(ST_HALO), 256, 0, st>>>(M, S, nm);
+    return hip_status(hipGetLastError(), "batch launch");
+}
+
+int check_batch_flags(int flags) {
+    if (flags & (LICOM_STEP_VTL_SCRATCH | LICOM_STEP_STAGED)) {
+        licom::set_error("batch: VTL_SCRATCH and STAGED are FP64-only");
+        return LICOM_ENOTSUP;
+    }
+    return LICOM_OK;
+}
+
+}  // namespace
+
+extern "C" int licom_batch_create(licom_ctx *const *members, const licom_fields_mixed *fields,
+                                  const licom_diag_mixed *diags, const licom_member_tuning *tuning,
+                                  int n, licom_batch **out) {
+    using namespace licom;
+    if (!members || !fields || !out || n <= 0) {
+        set_error("licom_batch_create: NULL argument or n <= 0");
+        return LICOM_EINVAL;
+    }
+    *out = nullptr;
+    std::vector<MemberDev> hm(n);
+    std::vector<int> starts((size_t)NSTAGE * (n + 1), 0);
+    int kmax = 0;
+    for (int m = 0; m < n; ++m) {
+        licom_ctx *c = members[m];
+        const licom_fields_mixed &f = fields[m];
+        if (!c || !f.at || !f.atb || !f.akt || !f.utf || !f.vtf || !f.utl || !f.vtl || !f.ws ||
+            !f.h0f || !f.h0l) {
+            set_error("licom_batch_create: member %d has a NULL context or field", m);
+            return LICOM_EINVAL;
+        }
+        const uintptr_t al = reinterpret_cast<uintptr_t>(f.at) | reinterpret_cast<uintptr_t>(f.atb);
+        if (c->imt % 2 != 0 || (al & 15) != 0) {
+            set_error("licom_batch_create: member %d needs an even imt and 16-byte aligned at/atb", m);
+            return LICOM_ENOTSUP;
+        }
+        if (c->p.boundary_restore == 1 && !f.restore) {
+            set_error("licom_batch_create: member %d has boundary_restore=1 but no restore field", m);
+            return LICOM_EINVAL;
+        }
+        const licom_member_tuning tu = tuning ? tuning[m] : licom_member_tuning{0, 0};
+        if (tu.facediv_shape < 0 || tu.facediv_shape > 2 || tu.solve_shape < 0 || tu.solve_shape > 2) {
+            set_error("licom_batch_create: member %d has an unknown launch shape", m);
+            return LICOM_EINVAL;
+        }
+        MemberDev &d = hm[m];
+        d = MemberDev{};
+        d.g = c->g;
+        d.p = c->kp;
+        d.at = f.at;
+        d.atb = f.atb;
+        d.h0f = f.h0f;
+        d.h0l = f.h0l;
+        d.swv = f.swv;
+        d.nswv = f.nswv;
+        d.sst = f.sst;
+        d.seaice = f.seaice;
+        d.sss = f.sss;
+        d.fresh = f.fresh;
+        d.restore = f.restore;
+        d.akt = f.akt;
+        d.utf = f.utf;
+        d.vtf = f.vtf;
+        d.utl = f.utl;
+        d.vtl = f.vtl;
+        d.ws = f.ws;
+        if (diags) {
+            const licom_diag_mixed &dg = diags[m];
+            d.ax = dg.ax;
+            d.ay = dg.ay;
+            d.az = dg.az;
+            d.dz = dg.dz;
+            d.tend = dg.tend;
+            d.dt_diff = dg.dt_diff;
+            d.dt_conv = dg.dt_conv;
+            d.pen = dg.penetrate;
+            d.net = dg.net;
+            d.fw_norm2 = dg.fw_norm2;
+        }
+        Scratch &s = c->s;
+        d.stf_pre = s.stf_pre;
+        d.work_u = s.work_u;
+        d.stf_n = s.stf_n;
+        d.temp11 = s.temp11;
+        d.partials = s.partials;
+        d.err_norm2 = s.err_norm2;
+        d.ori = s.ori;
+        d.uw = reinterpret_cast<float *>(s.uw);
+        d.vs = reinterpret_cast<float *>(s.vs);
+        d.wka = reinterpret_cast<float *>(s.ori + c->n3);
+        d.mix_count = s.mix_count;
+        d.area_t = c->area_t;
+        d.nblk = s.nblk;
+        const int imt = c->imt, jmt = c->jmt, km = c->km;
+        kmax = std::max(kmax, km);
+        d.sf_bx = (imt + 63) / 64;
+        d.sf_by = (jmt + 3) / 4;
+        d.fd_shape = tu.facediv_shape;
+        d.fd_bx = (imt + fd_cols(d.fd_shape) - 1) / fd_cols(d.fd_shape);
+        d.fd_by = (jmt + fd_rows(d.fd_shape) - 1) / fd_rows(d.fd_shape);
+        d.sv_shape = tu.solve_shape;
+        const dim3 sg = solve_grid(imt, jmt, d.sv_shape);
+        d.sv_bx = (int)sg.x;
+        d.sv_by = (int)sg.y;
+        d.ntx = (imt - 4 + BX - 1) / BX;
+        d.ntiles = d.ntx * ((jmt - 4 + BY - 1) / BY);
+        const size_t n3 = c->n3, n2 = c->n2;
+        d.side_vec = ((reinterpret_cast<uintptr_t>(f.utf) | reinterpret_cast<uintptr_t>(f.vtf)) & 15) == 0 &&
+                     n3 % 4 == 0;
+        d.side_blocks = (int)(((d.side_vec ? n3 / 4 : n3) + 255) / 256);
+        d.side2_blocks = (int)((n2 + 255) / 256);
+        d.halo_bx = (4 * (jmt - 4) + 4 * imt + 255) / 256;
+        const long cnt[NSTAGE] = {(long)d.sf_bx * d.sf_by,
+                                  d.nblk,
+                                  1,
+                                  (long)d.fd_bx * d.fd_by * km,
+                                  (long)d.sf_bx * d.sf_by,
+                                  2L * d.ntiles,
+                                  (long)d.sv_bx * d.sv_by,
+                                  d.side_blocks,
+                                  d.side2_blocks,
+                                  (long)d.halo_bx * 4 * km};
+        for (int st = 0; st < NSTAGE; ++st) {
+            const long next = starts[(size_t)st * (n + 1) + m] + cnt[st];
+            if (next > 0x7fffffffL) {
+                set_error("licom_batch_create: too many workgroups in stage %d", st);
+                return LICOM_EINVAL;
+            }
+            starts[(size_t)st * (n + 1) + m + 1] = (int)next;
+        }
+    }
+    if (kmax > LICOM_MAX_KM) {
+        set_error("licom_batch_create: km %d exceeds %d", kmax, LICOM_MAX_KM);
+        return LICOM_EINVAL;
+    }
+    licom_batch *b = new (std::nothrow) licom_batch;
+    if (!b) return LICOM_ENOMEM;
+    b->n = n;
+    b->kmax = kmax;
+    for (int st = 0; st < NSTAGE; ++st) b->totals[st] = starts[(size_t)st * (n + 1) + n];
+    int rc = hip_status(hipMalloc(&b->d_members, sizeof(MemberDev) * n), "hipMalloc batch");
+    if (!rc) rc = hip_status(hipMalloc(&b->d_starts, sizeof(int) * starts.size()), "hipMalloc batch");
+    if (!rc) rc = hip_status(hipMemcpy(b->d_members, hm.data(), sizeof(MemberDev) * n, hipMemcpyHostToDevice),
+                             "hipMemcpy batch");
+    if (!rc) rc = hip_status(hipMemcpy(b->d_starts, starts.data(), sizeof(int) * starts.size(),
+                                       hipMemcpyHostToDevice), "hipMemcpy batch");
+    if (!rc) {
+        rc = hip_status(hipStreamCreate(&b->stream), "hipStreamCreate");
+        b->own_stream = rc == LICOM_OK;
+    }
+    if (rc) {
+        licom_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return LICOM_OK;
+}
+
+extern "C" int licom_batch_set_stream(licom_batch *b, void *hip_stream) {
+    if (!b) return LICOM_EINVAL;
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
+    b->own_stream = false;
+    b->stream = static_cast<hipStream_t>(hip_stream);
+    return LICOM_OK;
+}
+
+extern "C" int licom_batch_set_march_grid(licom_batch *b, int workgroups) {
+    if (!b || workgroups < 0) return LICOM_EINVAL;
+    b->march_grid = workgroups;
+    return LICOM_OK;
+}
+
+extern "C" int licom_batch_step(licom_batch *b, int nsteps, int flags) {
+    if (!b || nsteps < 0) return LICOM_EINVAL;
+    if (int rc = check_batch_flags(flags)) return rc;
+    for (int s = 0; s < nsteps; ++s)
+        if (int rc = batch_launch_step(b, flags)) return rc;
+    return LICOM_OK;
+}
+
+extern "C" int licom_batch_capture(licom_batch *b, int nsteps, int flags) {
+    using namespace licom;
+    if (!b || nsteps <= 0) return LICOM_EINVAL;
+    if (int rc = check_batch_flags(flags)) return rc;
+    if (!b->stream) {
+        set_error("licom_batch_capture: graph capture needs a non-default stream");
+        return LICOM_EINVAL;
+    }
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    int rc = hip_status(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    if (rc) return rc;
+    int lrc = LICOM_OK;
+    for (int s = 0; s < nsteps && !lrc; ++s) lrc = batch_launch_step(b, flags);
+    hipGraph_t g = nullptr;
+    rc = hip_status(hipStreamEndCapture(b->stream, &g), "hipStreamEndCapture");
+    if (lrc || rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return lrc ? lrc : rc;
+    }
+    b->graph = g;
+    return hip_status(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+}
+
+extern "C" int licom_batch_replay(licom_batch *b) {
+    using namespace licom;
+    if (!b || !b->exec) {
+        set_error("licom_batch_replay: nothing captured");
+        return LICOM_EINVAL;
+    }
+    return hip_status(hipGraphLaunch(b->exec, b->stream), "hipGraphLaunch");
+}
+
+extern "C" int licom_batch_sync(licom_batch *b) {
+    if (!b) return LICOM_EINVAL;
+    return licom::hip_status(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+}
+
+extern "C" void licom_batch_destroy(licom_batch *b) {
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_members) (void)hipFree(b->d_members);
+    if (b->d_starts) (void)hipFree(b->d_starts);
+    if (b->own_stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" int licom_autotune_member(licom_ctx *ctx, const licom_fields_mixed *f,
+                                     const licom_diag_mixed *d, int reps, licom_member_tuning *best,
+                                     double *best_ms) {
+    using namespace licom;
+    if (!ctx || !f || !best || reps <= 0) {
+        set_error("licom_autotune_member: NULL argument or reps <= 0");
+        return LICOM_EINVAL;
+    }
+    const int flags = LICOM_STEP_CONVADJ;
+    double best_t = 1e300;
+    licom_member_tuning best_tu{0, 0};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = hip_status(hipEventCreate(&e0), "hipEventCreate");
+    if (!rc) rc = hip_status(hipEventCreate(&e1), "hipEventCreate");
+    for (int fs = 0; fs < 3 && !rc; ++fs)
+        for (int ss = 0; ss < 3 && !rc; ++ss) {
+            const licom_member_tuning tu{fs, ss};
+            licom_batch *b = nullptr;
+            rc = licom_batch_create(&ctx, f, d, &tu, 1, &b);
+            if (rc) break;
+            rc = licom_batch_step(b, 1, flags);                         // warm-up
+            if (!rc) rc = hip_status(hipEventRecord(e0, b->stream), "hipEventRecord");
+            if (!rc) rc = licom_batch_step(b, reps, flags);
+            if (!rc) rc = hip_status(hipEventRecord(e1, b->stream), "hipEventRecord");
+            if (!rc) rc = hip_status(hipEventSynchronize(e1), "hipEventSynchronize");
+            float ms = 0.0f;
+            if (!rc) rc = hip_status(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+            licom_batch_destroy(b);
+            if (!rc && ms / reps < best_t) {
+                best_t = ms / reps;
+                best_tu = tu;
+            }
+        }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    *best = best_tu;
+    if (best_ms) *best_ms = best_t;
+    return LICOM_OK;
+}

@@ -1,0 +1,313 @@
+// tracer_solve.h -- column solve of the tracer step (included by the FP64
+// and mixed-precision kernel files; each gets its own instantiations).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "tracer_device.h"
+#include "tracer_internal.h"
+
+namespace licom {
+namespace {
+
+// ---------------------------------------------------------------- solve
+// Two lanes per column: lane parity = tracer; 32 columns along i per wave.
+//
+// invtrit (tracer.cpp:923-978) eliminates from the bottom and substitutes
+// from the top.  Holding e(k), f(k) for the whole column would take 2*km
+// doubles per lane; instead the elimination keeps them only at every SEG-th
+// level, and the substitution recomputes each SEG-level segment from its
+// checkpoint with the same operations (identical bits).  Both sweeps load
+// the next segment's inputs while computing the current one.
+constexpr int SEG = 8;
+
+// convadj_cu (convadj.cpp:52-256) for one column, driven by the mask of
+// statically unstable interfaces found with the unmixed densities (bit k:
+// rholo(k-1) > rhoup(k)).  Every density the reference reads after mixing
+// starts is either recomputed by it or belongs to a level that mixing has not
+// touched, so the mask plus on-demand recomputation reproduces its decisions.
+// The two lanes of a column run this together: each holds its own tracer's
+// column X (the state just written, stride n2) and they swap values with
+// __shfl_xor.  Returns the mixed level range in lo/hi.
+__device__ __noinline__ void convect_sparse(const double *C, const double *to, const double *so,
+                                            const double *dzp, int km, int kcon, uint64_t mask,
+                                            int n, double *X, size_t n2, int &lo, int &hi) {
+    lo = km;
+    hi = -1;
+    int lcon = __ffsll((unsigned long long)mask) - 1;
+    for (;;) {   // CONV_1
+        int lcona = lcon, lconb = lcon + 1;
+        double dztsum = dzp[lcona - 1] + dzp[lconb - 1];
+        double trasum = X[(size_t)(lcona - 1) * n2] * dzp[lcona - 1] + X[(size_t)(lconb - 1) * n2] * dzp[lconb - 1];
+        double tramix = trasum / dztsum;
+        for (;;) {   // CONV_2
+            if (lconb != kcon) {
+                const double v = X[(size_t)lconb * n2];
+                const double vo = __shfl_xor(v, 1), mo = __shfl_xor(tramix, 1);
+                const double Tm = n == 0 ? tramix : mo, Sm = n == 0 ? mo : tramix;
+                const double Tv = n == 0 ? v : vo, Sv = n == 0 ? vo : v;
+                const double rl = dev::dens(C, km, lconb, Tm - to[lconb], Sm - so[lconb]);
+                const double ru = dev::dens(C, km, lconb, Tv - to[lconb], Sv - so[lconb]);
+                if (rl > ru) {
+                    lconb = lconb + 1;
+                    dztsum = dztsum + dzp[lconb - 1];
+                    trasum = trasum + v * dzp[lconb - 1];
+                    tramix = trasum / dztsum;
+                    continue;
+                }
+            }
+            if (lcona > 1) {
+                const double v = X[(size_t)(lcona - 2) * n2];
+                const double vo = __shfl_xor(v, 1), mo = __shfl_xor(tramix, 1);
+                const double Tm = n == 0 ? tramix : mo, Sm = n == 0 ? mo : tramix;
+                const double Tv = n == 0 ? v : vo, Sv = n == 0 ? vo : v;
+                const double rl = dev::dens(C, km, lcona - 1, Tv - to[lcona - 1], Sv - so[lcona - 1]);
+                const double ru = dev::dens(C, km, lcona - 1, Tm - to[lcona - 1], Sm - so[lcona - 1]);
+                if (rl > ru) {
+                    lcona = lcona - 1;
+                    dztsum = dztsum + dzp[lcona - 1];
+                    trasum = trasum + v * dzp[lcona - 1];
+                    tramix = trasum / dztsum;
+                    continue;
+                }
+            }
+            break;
+        }
+        for (int l = lcona - 1; l < lconb; ++l) X[(size_t)l * n2] = tramix;
+        lo = min(lo, lcona - 1);
+        hi = max(hi, lconb - 1);
+        if (lconb == kcon) return;
+        // CONV_3: next unstable interface below the mixed region
+        const uint64_t below = lconb >= 63 ? 0ull : (mask & ~((2ull << lconb) - 1ull));
+        if (below == 0) return;
+        lcon = __ffsll((unsigned long long)below) - 1;
+    }
+}
+
+// AK: element type of akt; DG: element type of the diagnostics (double for
+// the FP64 drop-in, float for the mixed form; the solve itself is FP64).
+// Launch shapes of the solve: a 256-lane workgroup covers R rows of C
+// columns (R*C = 128 columns, two lanes each); shape 0: 4 x 32, 1: 2 x 64,
+// 2: 1 x 128.  Only the mapping changes, never the arithmetic.
+__host__ __device__ constexpr int solve_rows(int shape) { return shape == 0 ? 4 : (shape == 1 ? 2 : 1); }
+__host__ __device__ constexpr int solve_cols(int shape) { return 128 / solve_rows(shape); }
+inline dim3 solve_grid(int imt, int jmt, int shape) {
+    // up to 15 extra leading columns per row from the line alignment
+    return dim3((imt - 4 + 15 + solve_cols(shape) - 1) / solve_cols(shape),
+                (jmt - 4 + solve_rows(shape) - 1) / solve_rows(shape));
+}
+
+// AK: element type of akt; DG: element type of the diagnostics (double for
+// the FP64 drop-in, float for the mixed form; the solve itself is FP64).
+template <int KM, typename AK, typename DG>
+__device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int flags,
+                                           double *__restrict__ at, double *__restrict__ atb,
+                                           const AK *__restrict__ akt,
+                                           const double *__restrict__ ori,
+                                           const double *__restrict__ stf_n,
+                                           DG *__restrict__ tend, DG *__restrict__ dt_diff,
+                                           DG *__restrict__ dt_conv, int *mix_count, int bx,
+                                           int by, int shape) {
+    constexpr int NSEG = (KM + SEG - 1) / SEG;
+    const int t = threadIdx.x, n = t & 1, col = t >> 1;
+    const int imt = g.imt, jmt = g.jmt, km = g.km;
+    const int R = solve_rows(shape), NC = solve_cols(shape);
+    const int gj = 2 + by * R + col / NC;
+    // Each wave takes 32 consecutive columns of one row, starting on a
+    // 128-byte line of the T arrays (element index a multiple of 16), so its
+    // 256-byte segments of every level are whole cache lines.
+    const long rowbase = (long)gj * imt;
+    const int gi = (int)(((rowbase + 2) & ~15L) + (long)NC * bx + col % NC - rowbase);
+    const bool act = gi >= 2 && gi < imt - 2 && gj < jmt - 2;
+    const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
+    const size_t q = act ? (size_t)gj * imt + gi : 0;
+    const int kz = act ? g.kmt[q] : 0;
+    const double *O = ori + n * n3 + q;
+    const AK *A = akt + n * n3 + q;
+    double *TT = at + n * n3 + q, *BB = atb + n * (n3 + n2) + n2 + q;
+    const double *odzp = g.odzp, *odzt = g.odzt;
+    const double dts = p.dts;
+    const double stf = act ? stf_n[n * n2 + q] : 0.0;
+    const int hmix = p.hmix;
+    const double dwm = p.dwndmix;
+
+    // ---- elimination from the bottom; checkpoints e,f at levels SEG*m (m >= 1)
+    double cke[NSEG + 1], ckf[NSEG + 1];
+#pragma unroll
+    for (int m = 0; m <= NSEG; ++m) cke[m] = ckf[m] = 0.0;
+    {
+        double pd[SEG], pa[SEG];
+#pragma unroll
+        for (int r = 0; r < SEG; ++r) {
+            const int k = SEG * (NSEG - 1) + r;
+            pd[r] = pa[r] = 0.0;
+            if (k < KM && k < kz) {
+                pd[r] = O[(size_t)k * n2];
+                pa[r] = k >= 1 ? (double)A[(size_t)(k - 1) * n2] : 0.0;
+            }
+        }
+        double en = 0.0, fn = 0.0, wk_k = 0.0;
+#pragma unroll
+        for (int m = NSEG - 1; m >= 0; --m) {
+            double cd[SEG], ca[SEG];
+#pragma unroll
+            for (int r = 0; r < SEG; ++r) {
+                cd[r] = pd[r];
+                ca[r] = pa[r];
+            }
+            if (m > 0) {
+#pragma unroll
+                for (int r = 0; r < SEG; ++r) {
+                    const int k = SEG * (m - 1) + r;
+                    pd[r] = pa[r] = 0.0;
+                    if (k < kz) {
+                        pd[r] = O[(size_t)k * n2];
+                        pa[r] = k >= 1 ? (double)A[(size_t)(k - 1) * n2] : 0.0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = SEG - 1; r >= 0; --r) {
+                const int k = SEG * m + r;
+                if (k < KM && k < kz) {
+                    const double wkm1 = k >= 1 ? dev::floor_akt(ca[r], k - 1, n, hmix, dwm) : 0.0;
+                    double aa, bb, cc;
+                    dev::tri_row(k, kz, wkm1, wk_k, odzp, odzt, dts, aa, bb, cc);
+                    const double g0 = 1.0 / (bb - cc * en);
+                    en = aa * g0;
+                    fn = (cd[r] + cc * fn) * g0;
+                    wk_k = wkm1;
+                }
+            }
+            if (m > 0) {
+                cke[m] = en;
+                ckf[m] = fn;
+            }
+        }
+    }
+
+    // ---- substitution from the top, tracer_9, convadj stability scan
+    const bool conv = flags & LICOM_STEP_CONVADJ;
+    const double *C = g.eosc, *to = g.to, *so = g.so;
+    double xprev = 0.0, Tp = 0.0, Sp = 0.0;
+    uint64_t mask = 0;
+    double qd[SEG], qa[SEG], qt[SEG], qw = 0.0;
+    auto load_seg = [&](int m) {
+#pragma unroll
+        for (int r = 0; r < SEG; ++r) {
+            const int k = SEG * m + r;
+            qd[r] = qa[r] = qt[r] = 0.0;
+            if (k < KM && k < km && act) {
+                qd[r] = O[(size_t)k * n2];
+                qt[r] = TT[(size_t)k * n2];
+                if (k >= 1 && k < kz) qa[r] = (double)A[(size_t)(k - 1) * n2];
+            }
+        }
+        const int kb = SEG * m + SEG - 1;
+        qw = (kb < KM && kb < kz - 1) ? (double)A[(size_t)kb * n2] : 0.0;
+    };
+    load_seg(0);
+#pragma unroll
+    for (int m = 0; m < NSEG; ++m) {
+        double cd[SEG], ca[SEG], ct[SEG];
+#pragma unroll
+        for (int r = 0; r < SEG; ++r) {
+            cd[r] = qd[r];
+            ca[r] = qa[r];
+            ct[r] = qt[r];
+        }
+        const double cw = qw;
+        if (m + 1 < NSEG) load_seg(m + 1);
+        const int top = SEG * m;
+        double se[SEG], sf[SEG];
+        {
+            double ee = cke[m + 1], ff = ckf[m + 1];
+            double wk_k = dev::floor_akt(cw, top + SEG - 1, n, hmix, dwm);
+#pragma unroll
+            for (int r = SEG - 1; r >= 0; --r) {
+                const int k = top + r;
+                se[r] = sf[r] = 0.0;
+                if (k < KM && k < kz) {
+                    const double wkm1 = k >= 1 ? dev::floor_akt(ca[r], k - 1, n, hmix, dwm) : 0.0;
+                    double aa, bb, cc;
+                    dev::tri_row(k, kz, wkm1, wk_k, odzp, odzt, dts, aa, bb, cc);
+                    const double g0 = 1.0 / (bb - cc * ee);
+                    ee = aa * g0;
+                    ff = (cd[r] + cc * ff) * g0;
+                    se[r] = ee;
+                    sf[r] = ff;
+                    wk_k = wkm1;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < SEG; ++r) {
+            const int k = top + r;
+            if (k < KM && k < km) {
+                const size_t c = (size_t)k * n2;
+                const bool wet = k < kz;
+                const double o = cd[r];
+                double x = o;
+                if (wet) x = (k == 0) ? (se[0] * stf + sf[0]) * 1.0 : (se[r] * xprev + sf[r]) * 1.0;
+                xprev = x;
+                const double vk = wet ? 1.0 : 0.0;
+                if (act) {
+                    if (dt_diff) {
+                        double v = (x - o) / dts * vk;
+                        if (k == 0) v = v - stf * 0.5 * odzp[0];
+                        dt_diff[n * n3 + q + c] = (DG)v;
+                    }
+                    if (tend) tend[n * n3 + q + c] = (DG)((x - ct[r]) / dts * vk);
+                    if (conv && dt_conv) dt_conv[n * n3 + q + c] = (DG)0.0;
+                    TT[c] = x;
+                    BB[c] = x;
+                }
+                if (conv) {
+                    const double xo = __shfl_xor(x, 1);
+                    const double Tk = n == 0 ? x : xo, Sk = n == 0 ? xo : x;
+                    if (k >= 1 && k <= kz - 1) {
+                        const double up = dev::dens(C, km, k, Tk - to[k], Sk - so[k]);
+                        const double lo = dev::dens(C, km, k, Tp - to[k], Sp - so[k]);
+                        if (lo > up) mask |= 1ull << k;
+                    }
+                    Tp = Tk;
+                    Sp = Sk;
+                }
+            }
+        }
+    }
+
+    // ---- convadj for statically unstable columns (both lanes of the pair)
+    if (conv && act && mask != 0) {
+        if (n == 0) atomicAdd(mix_count, 1);
+        int lo, hi;
+        convect_sparse(C, to, so, g.dzp, km, kz, mask, n, TT, n2, lo, hi);
+        for (int l = lo; l <= hi; ++l) {
+            const size_t c = (size_t)l * n2;
+            const double y = TT[c], b = BB[c];
+            if (y != b) {
+                const double dc = (y - b) / dts * 1.0;
+                if (dt_conv) dt_conv[n * n3 + q + c] = (DG)dc;
+                if (tend) tend[n * n3 + q + c] = (DG)((double)tend[n * n3 + q + c] + dc);
+                BB[c] = y;
+            }
+        }
+    }
+}
+
+// In-place side effects of the reference on its inputs
+// (tracer.cpp:1088, 1107-1108, 1187, 1315).
+template <int KM, typename AK, typename DG>
+__global__ __launch_bounds__(256) void k_solve(KGrid g, KPar p, int flags, double *__restrict__ at,
+                                               double *__restrict__ atb,
+                                               const AK *__restrict__ akt,
+                                               const double *__restrict__ ori,
+                                               const double *__restrict__ stf_n,
+                                               DG *__restrict__ tend,
+                                               DG *__restrict__ dt_diff,
+                                               DG *__restrict__ dt_conv, int *mix_count, int shape) {
+    solve_body<KM, AK, DG>(g, p, flags, at, atb, akt, ori, stf_n, tend, dt_diff, dt_conv, mix_count,
+                           blockIdx.x, blockIdx.y, shape);
+}
+
+}  // namespace
+}  // namespace licom

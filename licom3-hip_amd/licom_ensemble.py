@@ -1,0 +1,303 @@
+"""Batched ensemble engine, launch-shape autotuner and request loop over the
+C ABI (mixed-precision form).
+
+* :class:`Ensemble` binds ``licom_batch_*``: many members, each a
+  :class:`licom_tracer.TracerModel` of its own grid, advance together, one
+  launch per stage and step (the k-march as persistent workgroups over
+  (member, tile, tracer) items); N steps can be captured in a HIP graph and
+  replayed.
+* :class:`Autotuner` times every launch-shape candidate
+  (``licom_autotune_member``) once per (imt, jmt, km) and caches the winner.
+* :func:`ensemble_cases` draws the members of the ensemble workload: domains
+  from {96x64, 128x96, 256x192, 360x218} x km in {30, 55}, member m seeded
+  4000+m, domain draw seeded 4999, initial T perturbed by N(0, 0.1).
+* :class:`RequestLoop` serves tracer-advance requests with continuous
+  batching: requests that arrive while others run are admitted at the next
+  chunk boundary when their HBM estimate fits under the cap; member state
+  stays resident on the device until its request completes.
+
+A member's results are bit-identical alone and in any batch (same kernels,
+per-member reductions with per-member block counts).  There is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import time
+
+import numpy as np
+
+import licom_synth as ls
+import licom_tracer as lt
+
+MIXED_FLAGS = lt.LICOM_STEP_SIDE_EFFECTS | lt.LICOM_STEP_CONVADJ
+DOMAINS = ((96, 64), (128, 96), (256, 192), (360, 218))
+LEVELS = (30, 55)
+
+
+class MemberTuning(ctypes.Structure):
+    _fields_ = [("facediv_shape", ctypes.c_int), ("solve_shape", ctypes.c_int)]
+
+
+def _lib():
+    L = lt.lib()
+    if not getattr(L, "_batch_bound", False):
+        V = ctypes.c_void_p
+        L.licom_batch_create.argtypes = [ctypes.POINTER(V), ctypes.POINTER(lt.Fields),
+                                         ctypes.POINTER(lt.Diag), ctypes.POINTER(MemberTuning),
+                                         ctypes.c_int, ctypes.POINTER(V)]
+        L.licom_batch_set_stream.argtypes = [V, V]
+        L.licom_batch_set_march_grid.argtypes = [V, ctypes.c_int]
+        L.licom_batch_step.argtypes = [V, ctypes.c_int, ctypes.c_int]
+        L.licom_batch_capture.argtypes = [V, ctypes.c_int, ctypes.c_int]
+        L.licom_batch_replay.argtypes = [V]
+        L.licom_batch_sync.argtypes = [V]
+        L.licom_batch_destroy.argtypes = [V]
+        L.licom_autotune_member.argtypes = [V, ctypes.POINTER(lt.Fields), ctypes.POINTER(lt.Diag),
+                                            ctypes.c_int, ctypes.POINTER(MemberTuning),
+                                            ctypes.POINTER(ctypes.c_double)]
+        L._batch_bound = True
+    return L
+
+
+class Ensemble:
+    """Members (mixed-precision TracerModels) advanced as one batch."""
+
+    def __init__(self, members, tuning=None, march_grid=0, stream=None):
+        if not members:
+            raise lt.LicomError("Ensemble needs at least one member")
+        for m in members:
+            if m.precision != "mixed":
+                raise lt.LicomError("ensemble members use the mixed-precision form")
+        L = _lib()
+        n = len(members)
+        self.members = list(members)
+        ctxs = (ctypes.c_void_p * n)(*[m.ctx.value for m in members])
+        F = (lt.Fields * n)(*[m._fields() for m in members])
+        D = (lt.Diag * n)(*[m._diag() for m in members])
+        T = None
+        if tuning is not None:
+            T = (MemberTuning * n)(*[MemberTuning(int(a), int(b)) for a, b in tuning])
+        self._keep = (ctxs, F, D, T)
+        h = ctypes.c_void_p()
+        lt.check(L.licom_batch_create(ctxs, F, D, T, n, ctypes.byref(h)), "licom_batch_create")
+        self.handle = h
+        # a stream of the ensemble's own (default priority): graph capture
+        # needs one; it waits for the members' stream before every launch
+        import torch
+        self.torch = torch
+        s = stream if stream is not None else torch.cuda.Stream(device=members[0].device)
+        lt.check(L.licom_batch_set_stream(h, ctypes.c_void_p(s.cuda_stream)), "licom_batch_set_stream")
+        self.stream = s
+        self.member_stream = members[0].stream
+        if march_grid:
+            lt.check(L.licom_batch_set_march_grid(h, int(march_grid)), "licom_batch_set_march_grid")
+        self.captured = 0
+
+    def _order(self):
+        self.stream.wait_stream(self.member_stream)
+
+    def step(self, nsteps=1, flags=MIXED_FLAGS):
+        self._order()
+        lt.check(_lib().licom_batch_step(self.handle, int(nsteps), int(flags)), "licom_batch_step")
+        for m in self.members:
+            m.ist += nsteps
+
+    def capture(self, nsteps=24, flags=MIXED_FLAGS):
+        self._order()
+        self.sync()
+        lt.check(_lib().licom_batch_capture(self.handle, int(nsteps), int(flags)), "licom_batch_capture")
+        self.captured = int(nsteps)
+
+    def replay(self):
+        self._order()
+        lt.check(_lib().licom_batch_replay(self.handle), "licom_batch_replay")
+        for m in self.members:
+            m.ist += self.captured
+
+    def sync(self):
+        """Waits for the batch's work; afterwards the members' host_state()
+        reads the advanced state."""
+        lt.check(_lib().licom_batch_sync(self.handle), "licom_batch_sync")
+
+    def cell_updates_per_step(self):
+        return sum(m.case.nx * m.case.ny * m.case.km * 2 for m in self.members)
+
+    def close(self):
+        if getattr(self, "handle", None):
+            _lib().licom_batch_destroy(self.handle)
+            self.handle = None
+
+
+def member_case(seed, nx, ny, km, t_pert=0.1):
+    """One ensemble member from the workload generator: the 1 deg global-like
+    grid for 360x218, the 0.5 deg regional grid otherwise."""
+    base = "C2" if (nx, ny) == (360, 218) else "C1"
+    return ls.make_case(base, nx=nx, ny=ny, km=km, seed=seed, t_pert=t_pert)
+
+
+def draw_domains(n, seed=4999):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nx, ny = DOMAINS[int(rng.integers(len(DOMAINS)))]
+        out.append((nx, ny, LEVELS[int(rng.integers(len(LEVELS)))]))
+    return out
+
+
+def ensemble_cases(n=64, seed0=4000, shape_seed=4999):
+    return [member_case(seed0 + m, *shape) for m, shape in enumerate(draw_domains(n, shape_seed))]
+
+
+def member_bytes(nx, ny, km):
+    """HBM held by one resident mixed-precision member: FP64 at/atb, FP32
+    akt/velocities/ws and diagnostics, the context's scratch (ori and the face
+    velocities in FP64-sized buffers) and the 2-D fields."""
+    n2 = (nx + 4) * (ny + 4)
+    n3 = n2 * km
+    state = 8 * (2 * n3 + 2 * (n3 + n2)) + 4 * (2 * n3 + 4 * n3 + (n3 + n2))
+    diag = 4 * (14 * n3 + n3) + 8 * 2 * n2
+    scratch = 8 * (2 * n3 + 2 * n3) + 8 * 12 * n2
+    return state + diag + scratch + 8 * 16 * n2
+
+
+class Autotuner:
+    """Chooses launch shapes per (imt, jmt, km) by timing every candidate
+    on a throw-away member of that shape; results are cached in a JSON file."""
+
+    def __init__(self, cache_path=None, reps=3):
+        self.cache_path = cache_path
+        self.reps = int(reps)
+        self.table = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                self.table = json.load(f)
+
+    @staticmethod
+    def key(case):
+        return f"{case.imt}x{case.jmt}x{case.km}"
+
+    def tune(self, case):
+        k = self.key(case)
+        if k not in self.table:
+            m = lt.TracerModel(case, diagnostics=True, precision="mixed")
+            m.torch.cuda.synchronize()
+            best = MemberTuning()
+            ms = ctypes.c_double()
+            F, D = m._fields(), m._diag()
+            lt.check(_lib().licom_autotune_member(m.ctx, ctypes.byref(F), ctypes.byref(D),
+                                                  self.reps, ctypes.byref(best), ctypes.byref(ms)),
+                     "licom_autotune_member")
+            m.close()
+            self.table[k] = {"facediv_shape": best.facediv_shape, "solve_shape": best.solve_shape,
+                             "ms_per_step": ms.value}
+            if self.cache_path:
+                with open(self.cache_path, "w") as f:
+                    json.dump(self.table, f, indent=1)
+        e = self.table[k]
+        return e["facediv_shape"], e["solve_shape"]
+
+
+class Request:
+    __slots__ = ("rid", "arrival", "case", "n_steps", "done_steps", "model", "admitted",
+                 "finished", "bytes")
+
+    def __init__(self, rid, arrival, case, n_steps):
+        self.rid, self.arrival, self.case, self.n_steps = rid, arrival, case, int(n_steps)
+        self.done_steps = 0
+        self.model = None
+        self.admitted = None
+        self.finished = None
+        self.bytes = member_bytes(case.nx, case.ny, case.km)
+
+
+class RequestLoop:
+    """Continuous batching of tracer-advance requests on one GPU.
+
+    Each tick: admit every arrived request whose HBM estimate fits under the
+    cap (arrival order), upload its state, rebuild the batch over the active
+    members, advance all of them by the chunk min(chunk_steps, fewest
+    remaining steps), then retire finished members (their results are copied
+    out and their memory freed).  Latency = completion - arrival."""
+
+    def __init__(self, hbm_cap_bytes=200e9, chunk_steps=24, tuner=None, keep_results=False):
+        self.cap = float(hbm_cap_bytes)
+        self.chunk = int(chunk_steps)
+        self.tuner = tuner
+        self.keep_results = keep_results
+        self.results = {}
+
+    def run(self, requests, clock=time.perf_counter):
+        import torch
+        pending = sorted(requests, key=lambda r: r.arrival)
+        active, done = [], []
+        used = 0.0
+        t0 = clock()
+        updates = 0
+        i = 0
+        peak_active = 0
+        while i < len(pending) or active:
+            now = clock() - t0
+            while i < len(pending) and pending[i].arrival <= now and used + pending[i].bytes <= self.cap:
+                r = pending[i]
+                i += 1
+                r.model = lt.TracerModel(r.case, diagnostics=True, precision="mixed")
+                r.admitted = clock() - t0
+                used += r.bytes
+                active.append(r)
+            if not active:
+                # idle: wait for the next arrival
+                wait = pending[i].arrival - (clock() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+                continue
+            peak_active = max(peak_active, len(active))
+            tuning = [self.tuner.tune(r.case) for r in active] if self.tuner else None
+            ens = Ensemble([r.model for r in active], tuning=tuning)
+            c = min(self.chunk, min(r.n_steps - r.done_steps for r in active))
+            ens.step(c)
+            ens.sync()
+            ens.close()
+            torch.cuda.synchronize()
+            updates += c * sum(r.case.nx * r.case.ny * r.case.km * 2 for r in active)
+            t_now = clock() - t0
+            still = []
+            for r in active:
+                r.done_steps += c
+                if r.done_steps >= r.n_steps:
+                    r.finished = t_now
+                    if self.keep_results:
+                        self.results[r.rid] = r.model.host_state()["at"]
+                    r.model.close()
+                    r.model = None
+                    used -= r.bytes
+                    done.append(r)
+                else:
+                    still.append(r)
+            active = still
+        makespan = clock() - t0
+        lat = np.array([r.finished - r.arrival for r in done])
+        return {"requests": len(done), "makespan_s": makespan, "cell_updates": updates,
+                "cell_updates_per_s": updates / makespan if makespan > 0 else 0.0,
+                "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
+                "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
+                "peak_active": peak_active}
+
+
+def poisson_requests(n, rate_per_s, seed_arrivals=5001, seed_requests=5002, t_pert=0.1):
+    """n requests with exponential inter-arrival times (rate per second); each
+    draws a member domain from the ensemble distribution and n_steps uniform
+    in 24..240."""
+    ra = np.random.default_rng(seed_arrivals)
+    rr = np.random.default_rng(seed_requests)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += float(ra.exponential(1.0 / rate_per_s))
+        nx, ny = DOMAINS[int(rr.integers(len(DOMAINS)))]
+        km = LEVELS[int(rr.integers(len(LEVELS)))]
+        steps = int(rr.integers(24, 241))
+        out.append(Request(k, t, member_case(6000 + k, nx, ny, km, t_pert), steps))
+    return out

@@ -73,7 +73,10 @@ class GridDerived(ctypes.Structure):
 
 
 EXPORTED = ["licom_tracer_create", "licom_tracer_set_stream", "licom_tracer_step",
-            "licom_convadj", "licom_tracer_sync", "licom_grid_export", "licom_set_profiling",
+            "licom_tracer_step_mixed", "licom_batch_create", "licom_batch_set_stream",
+            "licom_batch_set_march_grid", "licom_batch_step", "licom_batch_capture",
+            "licom_batch_replay", "licom_batch_sync", "licom_batch_destroy",
+            "licom_autotune_member", "licom_convadj", "licom_tracer_sync", "licom_grid_export", "licom_set_profiling",
             "licom_kernel_times", "licom_scratch_bytes", "licom_last_mix_count",
             "licom_stream_copy", "licom_tracer_destroy", "licom_strerror", "licom_last_error"]
 
@@ -99,6 +102,9 @@ def lib():
         L.licom_tracer_set_stream.argtypes = [_VP, _VP]
         L.licom_tracer_step.argtypes = [_VP, ctypes.POINTER(Fields), ctypes.POINTER(Diag),
                                         ctypes.c_int]
+        # licom_fields_mixed / licom_diag_mixed have the same pointer layout
+        L.licom_tracer_step_mixed.argtypes = [_VP, ctypes.POINTER(Fields), ctypes.POINTER(Diag),
+                                              ctypes.c_int]
         L.licom_convadj.argtypes = [_VP, ctypes.POINTER(Fields), ctypes.POINTER(Diag)]
         L.licom_tracer_sync.argtypes = [_VP]
         L.licom_grid_export.argtypes = [_VP, ctypes.POINTER(GridDerived)]
@@ -171,8 +177,14 @@ class TracerModel:
     shapes; ``diag`` holds the diagnostics the reference produces.
     """
 
-    def __init__(self, case, device="cuda:0", diagnostics=True, stream=None):
+    # arrays held in FP32 by the mixed-precision form
+    MIXED_F32 = ("akt", "utf", "vtf", "utl", "vtl", "ws")
+
+    def __init__(self, case, device="cuda:0", diagnostics=True, stream=None, precision="fp64"):
         import torch
+        if precision not in ("fp64", "mixed"):
+            raise LicomError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
+        self.precision = precision
         if not torch.cuda.is_available():
             raise LicomError("TracerModel needs a GPU (torch.cuda.is_available() is False)")
         self.torch = torch
@@ -189,8 +201,11 @@ class TracerModel:
         self.ctx = h
         self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
         check(L.licom_tracer_set_stream(self.ctx, _VP(self.stream.cuda_stream)), "set_stream")
-        self.state = {k: torch.from_numpy(np.ascontiguousarray(v)).to(self.device)
-                      for k, v in case.state.items()}
+        mixed = precision == "mixed"
+        self.state = {}
+        for k, v in case.state.items():
+            t = torch.from_numpy(np.ascontiguousarray(v)).to(self.device)
+            self.state[k] = t.to(torch.float32) if (mixed and k in self.MIXED_F32) else t
         self.forcing = {k: (None if case.forcing.get(k) is None else
                             torch.from_numpy(np.ascontiguousarray(case.forcing[k])).to(self.device))
                         for k in FORCING_KEYS}
@@ -199,11 +214,13 @@ class TracerModel:
         km, jmt, imt = case.km, case.jmt, case.imt
         self.diag = None
         if diagnostics:
-            z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.device)  # noqa: E731
+            dt3 = torch.float32 if mixed else torch.float64
+            z = lambda *s: torch.zeros(s, dtype=dt3, device=self.device)  # noqa: E731
+            z64 = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.device)  # noqa: E731
             self.diag = dict(ax=z(2, km, jmt, imt), ay=z(2, km, jmt, imt), az=z(2, km, jmt, imt),
                              dz=z(2, km, jmt, imt), tend=z(2, km, jmt, imt),
                              dt_diff=z(2, km, jmt, imt), dt_conv=z(2, km, jmt, imt),
-                             penetrate=z(km, jmt, imt), net=z(2, jmt, imt), fw_norm2=z(1))
+                             penetrate=z(km, jmt, imt), net=z64(2, jmt, imt), fw_norm2=z64(1))
         self.ist = 0
 
     # -- C structs -------------------------------------------------------
@@ -227,12 +244,18 @@ class TracerModel:
     def tracer(self, ii=0, flags=LICOM_STEP_SIDE_EFFECTS):
         """tracer_(ii): one tracer step of T and S (tracer.cpp:1701-1903)."""
         F, D = self._fields(), self._diag()
-        check(lib().licom_tracer_step(self.ctx, ctypes.byref(F), ctypes.byref(D), int(flags)),
-              "licom_tracer_step")
+        if self.precision == "mixed":
+            check(lib().licom_tracer_step_mixed(self.ctx, ctypes.byref(F), ctypes.byref(D),
+                                                int(flags)), "licom_tracer_step_mixed")
+        else:
+            check(lib().licom_tracer_step(self.ctx, ctypes.byref(F), ctypes.byref(D), int(flags)),
+                  "licom_tracer_step")
         self.ist += 1
 
     def convadj(self):
         """convadj_() over every column (convadj.cpp:290-298)."""
+        if self.precision == "mixed":
+            raise LicomError("convadj() alone is FP64-only; use step() with LICOM_STEP_CONVADJ")
         F, D = self._fields(), self._diag()
         check(lib().licom_convadj(self.ctx, ctypes.byref(F), ctypes.byref(D)), "licom_convadj")
 
@@ -293,11 +316,11 @@ class TracerModel:
 
     def host_state(self):
         self.sync()
-        return {k: v.cpu().numpy() for k, v in self.state.items()}
+        return {k: v.double().cpu().numpy() for k, v in self.state.items()}
 
     def host_diag(self):
         self.sync()
-        return {k: v.cpu().numpy() for k, v in self.diag.items()}
+        return {k: v.double().cpu().numpy() for k, v in self.diag.items()}
 
     def close(self):
         if getattr(self, "ctx", None):

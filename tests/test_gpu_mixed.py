@@ -1,0 +1,147 @@
+"""GPU tests of the mixed-precision form (FP64 state, FP32 fluxes, stencils,
+velocities, akt and diagnostics; FP64 implicit solve, convadj and sums).
+
+Errors are stated against the FP64 result on the same inputs: the oracle
+itself for small cases, and the FP64 GPU path (bit-identical to the oracle,
+see test_gpu_parity.py) for the 100-step run.  Bounds:
+  * after 100 steps (1 deg grid): max |dT|, max |dS| within 1e-4 (K, psu) or
+    within 3x of the FP64 path's own divergence from a 1e-8 K perturbation
+    of the initial state (limiter decisions flip at near-ties),
+  * heat-content drift relative to the FP64 run <= 1e-9,
+  * after one step, max |dT| <= 1e-6 K and the FP32 diagnostics within
+    1e-4 of the FP64 ones (relative to each field's maximum)."""
+import sys
+
+import numpy as np
+import pytest
+
+import licom_synth as ls
+import licom_tracer as lt
+import orc
+from helpers import inner
+
+pytestmark = pytest.mark.gpu
+
+FLAGS = lt.LICOM_STEP_SIDE_EFFECTS | lt.LICOM_STEP_CONVADJ
+PSU = 1000.0          # S is stored as (S - 35) * 1e-3
+
+
+def announce(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def wet(case):
+    kmt = case.grid["kmt"]
+    return (np.arange(case.km)[:, None, None] < kmt[None])[:, 2:-2, 2:-2]
+
+
+def errors(case, a_mixed, a_ref):
+    m = wet(case)
+    dT = np.abs(inner(a_mixed[0]) - inner(a_ref[0]))[m]
+    dS = np.abs(inner(a_mixed[1]) - inner(a_ref[1]))[m] * PSU
+    return float(dT.max()), float(dS.max())
+
+
+def heat(case, o, at):
+    dz = o.d["dzp"][:, None, None]
+    ta = case.grid["tarea"][None, 2:-2, 2:-2]
+    return float(np.sum(dz * ta * o.d["vit"][:, 2:-2, 2:-2] * inner(at[0])))
+
+
+def test_one_step_against_oracle(need_gpu):
+    case = ls.make_case("C1", depth_min=-1000.0, sigma=2.0, seed=11)
+    announce("mixed C1 1 step vs oracle")
+    o = orc.Oracle(case)
+    st = case.copy_state()
+    od = orc.Oracle.new_diag(case)
+    o.step(st, od, convadj=1)
+    m = lt.TracerModel(case, precision="mixed")
+    m.tracer(0, FLAGS)
+    gs, gd = m.host_state(), m.host_diag()
+    m.close()
+    eT, eS = errors(case, gs["at"], st["at"])
+    print(f"[mixed C1 x1] max|dT| {eT:.3e} K, max|dS| {eS:.3e} psu", flush=True)
+    assert eT <= 1e-6 and eS <= 1e-6
+    for k in ("ax", "ay", "az", "dz", "tend", "dt_diff"):
+        a, b = inner(gd[k]), inner(od[k])
+        scale = np.abs(b).max()
+        err = np.abs(a - b).max() / scale if scale > 0 else np.abs(a).max()
+        print(f"  {k}: rel {err:.3e}", flush=True)
+        assert err <= 1e-4, k
+    # ws in FP32 against the FP64 continuity w
+    a, b = gs["ws"][1:-1, 1:-1, 1:-1], st["ws"][1:-1, 1:-1, 1:-1]
+    assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
+
+
+def test_ten_steps_c1_with_forcing(need_gpu):
+    case = ls.make_case("C1", seed=12)
+    rng = np.random.default_rng(3)
+    f = case.forcing
+    f["swv"] = rng.uniform(0, 200, (case.jmt, case.imt))
+    f["nswv"] = rng.uniform(-150, 50, (case.jmt, case.imt))
+    f["fresh"] = rng.uniform(-1e-5, 1e-5, (case.jmt, case.imt))
+    announce("mixed C1 10 steps with forcing vs oracle")
+    o = orc.Oracle(case)
+    st = case.copy_state()
+    od = orc.Oracle.new_diag(case)
+    m = lt.TracerModel(case, precision="mixed")
+    for s in range(10):
+        o.step(st, od, convadj=1)
+        m.tracer(s, FLAGS)
+    gs = m.host_state()
+    m.close()
+    eT, eS = errors(case, gs["at"], st["at"])
+    print(f"[mixed C1 x10 forcing] max|dT| {eT:.3e} K, max|dS| {eS:.3e} psu", flush=True)
+    assert eT <= 1e-5 and eS <= 1e-5
+
+
+def test_hundred_steps_c2_error_bound(need_gpu):
+    """100 steps on the 1 deg grid.  The tspas limiter switches a face between
+    the Lax-Wendroff and the upwind flux when the predictor crosses the local
+    bounds; a state perturbation of the size of one mixed step's error
+    (~1e-8 K) flips some of these decisions over 100 steps, and each flip
+    moves T by ~CFL * dT across the face.  The control run is the FP64 path
+    itself started from T perturbed by 1e-8 K: its divergence from the
+    unperturbed FP64 run is the scheme's own sensitivity, and the mixed form
+    must stay within 3x of it (and within 1e-4 K wherever the control does)."""
+    case = ls.make_case("C2")
+    announce("mixed vs fp64 C2 100 steps, with a perturbed fp64 control")
+    pcase = ls.make_case("C2")
+    rng = np.random.default_rng(99)
+    kmt = pcase.grid["kmt"]
+    wet3 = np.arange(pcase.km)[:, None, None] < kmt[None]
+    noise = 1e-8 * rng.standard_normal(pcase.state["at"][0].shape) * wet3
+    pcase.state["at"][0] += noise
+    pcase.state["atb"][0, 1:] += noise
+    ref = lt.TracerModel(case)
+    mix = lt.TracerModel(case, precision="mixed")
+    ctl = lt.TracerModel(pcase)
+    for s in range(100):
+        ref.tracer(s, FLAGS)
+        mix.tracer(s, FLAGS)
+        ctl.tracer(s, FLAGS)
+    a_ref, a_mix, a_ctl = ref.host_state()["at"], mix.host_state()["at"], ctl.host_state()["at"]
+    for m in (ref, mix, ctl):
+        m.close()
+    eT, eS = errors(case, a_mix, a_ref)
+    cT, cS = errors(case, a_ctl, a_ref)
+    o = orc.Oracle(case)
+    h_ref, h_mix = heat(case, o, a_ref), heat(case, o, a_mix)
+    drift = abs(h_mix - h_ref) / abs(h_ref)
+    print(f"[mixed C2 x100] max|dT| {eT:.3e} K, max|dS| {eS:.3e} psu, heat drift {drift:.3e}; "
+          f"fp64 control from T+1e-8 K: max|dT| {cT:.3e} K, max|dS| {cS:.3e} psu", flush=True)
+    assert eT <= max(1e-4, 3.0 * cT)
+    assert eS <= max(1e-4, 3.0 * cS)
+    assert drift <= 1e-9
+
+
+def test_mixed_rejects_fp64_only_flags(need_gpu):
+    case = ls.make_case("C1", nx=16, ny=8, km=6)
+    announce("mixed flag rejection")
+    m = lt.TracerModel(case, precision="mixed")
+    with pytest.raises(lt.LicomError):
+        m.tracer(0, lt.LICOM_STEP_VTL_SCRATCH)
+    with pytest.raises(lt.LicomError):
+        m.tracer(0, lt.LICOM_STEP_STAGED)
+    m.close()
