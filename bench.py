@@ -19,6 +19,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "licom3-hip_amd"))
 
@@ -209,6 +211,77 @@ def run_variant(model, args, torch, lt, case, updates):
             "step_roofline": step_roof, "kernel_ms": {k: round(t[0] / t[1], 4) for k, t in kt.items()}}
 
 
+def run_ensemble(args, torch, lt, le):
+    """The 64-member ensemble workload (mixed form): members drawn from
+    {96x64, 128x96, 256x192, 360x218} x km {30, 55}, launch shapes from the
+    autotuner, 24 steps captured in a HIP graph and replayed."""
+    t0 = time.time()
+    cases = le.ensemble_cases(args.members)
+    tuner = le.Autotuner(os.path.join("gpurun_out", "autotune_cache.json")
+                         if os.path.isdir("gpurun_out") else None, reps=3)
+    tuning = [tuner.tune(c) for c in cases]
+    print(f"[bench] ensemble: {len(cases)} members generated and tuned in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    models = [lt.TracerModel(c, diagnostics=True, precision="mixed") for c in cases]
+    ens = le.Ensemble(models, tuning=tuning)
+    ups = ens.cell_updates_per_step()
+    ens.capture(24)
+    ens.replay()               # warm-up
+    ens.sync()
+    reps = max(1, args.ensemble_replays)
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        ens.replay()
+    ens.sync()
+    dt = time.perf_counter() - t1
+    ens.close()
+    # isolated single-member step time per shape (for the serving slowdown)
+    iso = {}
+    for c in cases:
+        key = (c.nx, c.ny, c.km)
+        if key in iso:
+            continue
+        m = lt.TracerModel(c, diagnostics=True, precision="mixed")
+        e1 = le.Ensemble([m], tuning=[tuner.tune(c)])
+        e1.step(2)
+        e1.sync()
+        t2 = time.perf_counter()
+        e1.step(10)
+        e1.sync()
+        iso[key] = (time.perf_counter() - t2) / 10
+        e1.close()
+        m.close()
+    for m in models:
+        m.close()
+    torch.cuda.empty_cache()
+    value = ups * 24 * reps / dt
+    return {"members": len(cases), "value": value, "unit": "tracer cell-updates/s",
+            "ms_per_step": 1000.0 * dt / (24 * reps), "graph_steps": 24, "replays": reps,
+            "cell_updates_per_step": ups, "dtype": "f64/f32",
+            "tuning": {k: v for k, v in tuner.table.items()},
+            "isolated_step_ms": {f"{a}x{b}x{c}": round(1000 * t, 4) for (a, b, c), t in iso.items()}}, iso
+
+
+def run_serving(args, torch, lt, le, offline_value, iso):
+    """Request loop: Poisson arrivals at a fixed fraction of the offline
+    batched capacity, HBM admission under the cap, continuous batching."""
+    mean_updates = np.mean([st * nx * ny * km * 2 for nx, ny, km, st in le.request_specs(args.requests)])
+    rate = args.load * offline_value / mean_updates
+    reqs = le.poisson_requests(args.requests, rate)
+    print(f"[bench] serving: {len(reqs)} requests at {rate:.2f}/s", file=sys.stderr, flush=True)
+    loop = le.RequestLoop(hbm_cap_bytes=args.hbm_cap_gb * 1e9, chunk_steps=24)
+    stats = loop.run(reqs)
+    slow = np.array([(r.finished - r.arrival) / (r.n_steps * iso.get((r.case.nx, r.case.ny, r.case.km),
+                                                                      np.nan)) for r in reqs])
+    slow = slow[np.isfinite(slow)]
+    stats.update({"arrival_rate_per_s": rate, "load_fraction": args.load,
+                  "hbm_cap_gb": args.hbm_cap_gb, "chunk_steps": 24,
+                  "vs_offline": stats["cell_updates_per_s"] / offline_value,
+                  "p50_slowdown_vs_isolated": float(np.percentile(slow, 50)) if slow.size else None,
+                  "p99_slowdown_vs_isolated": float(np.percentile(slow, 99)) if slow.size else None})
+    return stats
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,6 +295,13 @@ def main():
     ap.add_argument("--cpu-band", type=int, default=128)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--ensemble-replays", type=int, default=3)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--load", type=float, default=0.8)
+    ap.add_argument("--hbm-cap-gb", type=float, default=200.0)
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the HBM traffic summary from two rocprofv3 --pmc runs")
     ap.add_argument("--summarize-trace", metavar="TRACE_DIR",
@@ -275,6 +355,13 @@ def main():
     del models
     torch.cuda.empty_cache()
 
+    ens_out, serve_out = None, None
+    if not args.no_ensemble:
+        import licom_ensemble as le
+        ens_out, iso = run_ensemble(args, torch, lt, le)
+        if not args.no_serving:
+            serve_out = run_serving(args, torch, lt, le, ens_out["value"], iso)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
@@ -307,6 +394,10 @@ def main():
                         "speedup_vs_fp64": rm["value"] / r["value"],
                         "roofline": rm["roofline"], "step_roofline": rm["step_roofline"],
                         "kernel_ms": rm["kernel_ms"], "error": mixed_err}
+    if ens_out:
+        out["ensemble"] = ens_out
+    if serve_out:
+        out["serving"] = serve_out
     print(json.dumps(out), flush=True)
 
 

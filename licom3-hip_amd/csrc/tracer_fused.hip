@@ -7,10 +7,10 @@
 //                  barotropic-corrected velocity (upwell_2, operators_div).
 //   k_wcol<KM>     one lane per column: continuity w (upwell_3/4), the
 //                  divergence profile held in registers.
-//   k_march        one five-wave workgroup per 16x16 tile of columns and
-//                  tracer, marching k top to bottom.  Waves 0-3 own the tile,
-//                  wave 4 the 64-cell ring the limiter and del4 read around
-//                  it; wave 4 also streams the planes of at, atb and the
+//   k_march        one four-wave workgroup per 16x12 tile of columns and
+//                  tracer, marching k top to bottom.  Waves 0-2 own the tile,
+//                  wave 3 the 56-cell ring the limiter and del4 read around
+//                  it; wave 3 also streams the planes of at, atb and the
 //                  velocities (tile plus a 2-cell halo) straight into LDS a
 //                  level ahead.  The tspas predictor of level k+1 feeds the
 //                  limiter of level k through LDS flag planes; the del4 first
@@ -135,13 +135,14 @@ __global__ __launch_bounds__(256) void k_wcol(KGrid g, const double *__restrict_
 }
 
 // --------------------------------------------------------------- march
-constexpr int BX = 16, BY = 16;                 // interior tile (one lane per column)
+constexpr int BX = 16, BY = 12;                 // interior tile (one lane per column)
 constexpr int PX = BX + 2, PY = BY + 2;         // tile + 1-cell ring (predictor, d2tk)
 constexpr int RX = BX + 4, RY = BY + 4;         // tile + 2-cell halo (staged planes)
-constexpr int NOWN = BX * BY;                   // 256 tile lanes: waves 0-3
-constexpr int NRING = 2 * (BX + BY);            // 64 ring cells: wave 4 (the ring's
+constexpr int NOWN = BX * BY;                   // 192 tile lanes: waves 0-2
+constexpr int NRING = 2 * (BX + BY);            // 56 ring cells: wave 3 (the ring's
                                                 // corners are never read)
-constexpr int NT = NOWN + NRING;                // 320 lanes, five waves
+constexpr int NT = 256;                         // four waves
+static_assert(NOWN == 192 && NRING <= 64, "three tile waves and one ring wave");
 constexpr int NP = PX * PY, NR = RX * RY;
 constexpr int NAT = 4, NUV = 3, NW = 4, NB = 3; // LDS slots per staged array
 
@@ -153,11 +154,11 @@ struct MarchArgs {
 
 // Ring cell r (0..NRING-1) -> P-plane coordinates (pi, pj): north row, south
 // row, west column, east column, corners excluded.
-static_assert(BX == 16 && BY == 16, "ring_cell assumes a 16x16 tile");
 __device__ __forceinline__ int2 ring_cell(int r) {
-    const int side = r >> 4, o = 1 + (r & 15);
-    const int pj = side == 0 ? 0 : (side == 1 ? PY - 1 : o);
-    const int pi = side <= 1 ? o : (side == 2 ? 0 : PX - 1);
+    const bool ns = r < 2 * BX;                       // north / south rows
+    const int rr = ns ? r : r - 2 * BX;               // else west / east columns
+    const int pi = ns ? 1 + (rr % BX) : (rr < BY ? 0 : PX - 1);
+    const int pj = ns ? (rr < BX ? 0 : PY - 1) : 1 + (rr % BY);
     return make_int2(pi, pj);
 }
 
@@ -254,6 +255,23 @@ __device__ __forceinline__ double d2tk_at(int r, const double *B, const int *sK,
 
 #define LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
 
+// Direct global->LDS loads issued through inline assembly: the compiler
+// then neither reorders them nor inserts vector-memory waits before this
+// wave's later LDS reads (which touch other slots); the staging wave waits
+// for them itself (wait_vmem) before the barrier that publishes the planes.
+// M0 holds the wave's LDS destination; nothing else in these kernels uses M0.
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+    return __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)LDS_PTR(p));
+}
+__device__ __forceinline__ void dma16(const void *g, uint32_t lds) {
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(g)
+                 : "memory");
+}
+__device__ __forceinline__ void dma4(const void *g, uint32_t lds) {
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off" ::"s"(lds), "v"(g)
+                 : "memory");
+}
+
 // One wave copies the R-window (tile plus 2-cell halo) of one plane straight
 // from global memory into LDS (global_load_lds: lane l of an instruction
 // lands at LDS base + 16 l), in 16-byte chunks, which needs an even imt and
@@ -269,7 +287,7 @@ __device__ __forceinline__ void stage_plane(const double *src, double *dst, cons
 #pragma unroll
     for (int u = 0; u < NCI; ++u)
         if (u * 64 + lane < NCH)
-            __builtin_amdgcn_global_load_lds(src + off[u], LDS_PTR(dst + 128 * u), 16, 0, 0);
+            dma16(src + off[u], lds_addr(dst + 128 * u));
 }
 
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
@@ -280,15 +298,17 @@ __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" 
 // vector-memory counters at a barrier.
 __device__ __forceinline__ void bar_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// k-march over one 16x16 tile of columns and one tracer, five waves.
-// Waves 0-3 own the tile's cells; wave 4 owns the 64 ring cells the limiter
-// and the del4 second pass read around the tile, and stages every plane.
+// k-march over one 16x12 tile of columns and one tracer, four waves.
+// Waves 0-2 own the tile's cells; wave 3 owns the 56 ring cells the limiter
+// and the del4 second pass read around the tile, and stages every plane
+// (a four-wave workgroup keeps three workgroups resident per CU).
 // Per level k two phases, each closed by a barrier:
-//   P(k): tspas predictor of level k+1 and del4 first pass of level k at
-//         every cell (tile and ring, one cell per lane); wave 4 then issues
-//         the direct-to-LDS loads of the planes P(k+1) and C(k+1) need.
+//   P(k): wave 3 first issues the direct-to-LDS loads of the planes P(k+1)
+//         and C(k+1) need; then the tspas predictor of level k+1 and the
+//         del4 first pass of level k at every cell (tile and ring, one cell
+//         per lane).
 //   C(k): limiter, fluxes and explicit update of level k on the tile (waves
-//         0-3); wave 4 waits for its loads.
+//         0-2); wave 3 waits for its loads.
 // LDS slots rotate: at x4, u_wface/v_sface x3, ws x4, atb x3.
 __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
     __shared__ __align__(16) double sAT[NAT][NR];
@@ -313,11 +333,13 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
 
     const int t = threadIdx.x, lane = t & 63;
     const bool ringw = t >= NOWN;
-    const int2 rc = ring_cell((t - NOWN) & 63);
+    const int rl = t - NOWN;                     // ring lane (wave 3)
+    const bool has_cell = !ringw || rl < NRING;
+    const int2 rc = ring_cell(has_cell && ringw ? rl : 0);
     const int pi = ringw ? rc.x : 1 + t % BX, pj = ringw ? rc.y : 1 + t / BX;
     const int gi = gi0 - 1 + pi, gj = gj0 - 1 + pj;
     const int rC = (pj + 1) * RX + (pi + 1), pC = pj * PX + pi;
-    const bool cell_ok = gj >= 1 && gj < jmt - 1 && gi >= 1 && gi < imt - 1;
+    const bool cell_ok = has_cell && gj >= 1 && gj < jmt - 1 && gi >= 1 && gi < imt - 1;
     const bool act = !ringw && gi < imt - 2 && gj < jmt - 2;
     const size_t q = cell_ok ? (size_t)gj * imt + gi : 0;
 
@@ -339,7 +361,7 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
     const bool rest = p.boundary_restore == 1 && a.restore != nullptr;
 
     if (ringw) {
-        // ---- wave 4: ring cells and plane staging
+        // ---- wave 3: ring cells and plane staging
         int off[NCI];
 #pragma unroll
         for (int u = 0; u < NCI; ++u) {
@@ -361,23 +383,16 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
         {
             LW tmp;
             const Lv L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
-            sFL[0][pC] = cell_ok ? pred_flag(g, rC, L0, sK, 0, dts, M.tr, M.D1, M.D2, M.D3, M.D4, tmp)
-                                 : false;
+            const bool f0 = cell_ok ? pred_flag(g, rC, L0, sK, 0, dts, M.tr, M.D1, M.D2, M.D3, M.D4, tmp)
+                                    : false;
+            if (has_cell) sFL[0][pC] = f0;
         }
         bar_lds();
         for (int k = 0; k < km; ++k) {
             const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3, s3 = (k + 3) & 3;
             const int u0 = k % 3, u1 = (k + 1) % 3, u2 = (k + 2) % 3;
-            if (k + 1 < km) {
-                LW tmp;
-                const Lv L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
-                sFL[(k + 1) & 1][pC] =
-                    cell_ok ? pred_flag(g, rC, L1, sK, k + 1, dts, M.tr, M.D1, M.D2, M.D3, M.D4, tmp)
-                            : false;
-            }
-            if (p.hmix == LICOM_HMIX_DEL4)
-                sD2[pC] = cell_ok ? d2tk_at(rC, sB[u0], sK, k, M.dtn, M.dts, M.dte, M.dtw, M.ahf) : 0.0;
-            // planes for P(k+1) and C(k+1); their slots were last read in C(k-1)
+            // planes for P(k+1) and C(k+1), issued first so that they have the
+            // whole level to land; their slots were last read in C(k-1)
             if (k + 3 < km) {
                 stage_plane(T + (size_t)(k + 3) * n2, sAT[s3], off, lane);
                 stage_plane(a.ws + (size_t)(k + 3) * n2, sW[s3], off, lane);
@@ -387,6 +402,16 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
                 stage_plane(a.vs + (size_t)(k + 2) * n2, sV[u2], off, lane);
                 stage_plane(TB + (size_t)(k + 2) * n2, sB[u2], off, lane);
             }
+            if (k + 1 < km) {
+                LW tmp;
+                const Lv L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
+                const bool f =
+                    cell_ok ? pred_flag(g, rC, L1, sK, k + 1, dts, M.tr, M.D1, M.D2, M.D3, M.D4, tmp)
+                            : false;
+                if (has_cell) sFL[(k + 1) & 1][pC] = f;
+            }
+            if (p.hmix == LICOM_HMIX_DEL4 && has_cell)
+                sD2[pC] = cell_ok ? d2tk_at(rC, sB[u0], sK, k, M.dtn, M.dts, M.dte, M.dtw, M.ahf) : 0.0;
             bar_lds();     // end of P(k)
             wait_vmem();   // C(k): the staged planes land
             bar_lds();     // end of C(k)
@@ -394,7 +419,7 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
         return;
     }
 
-    // ---- waves 0-3: the tile
+    // ---- waves 0-2: the tile
     double fA = act ? A[q] : 0.0;
     double fR = (rest && act) ? a.restore[n * n3 + q] : 0.0;
     bar_lds();

@@ -13,7 +13,7 @@
 // increments.
 //
 //   k_facediv_mx / k_wcol_mx<KM>  face velocities, divergence, continuity w
-//   k_march_mx                    five-wave k-march (as the FP64 k_march)
+//   k_march_mx                    four-wave k-march (as the FP64 k_march)
 //   k_solve<KM, float, float>     FP64 invtrit + convadj (tracer_solve.h)
 //   k_side3<float4> / k_side2_mx  in-place side effects
 #include <hip/hip_runtime.h>
@@ -152,10 +152,11 @@ __global__ __launch_bounds__(256) void k_wcol_mx(KGrid g, const float *__restric
 }
 
 // --------------------------------------------------------------- march
-constexpr int BX = 16, BY = 16;
+constexpr int BX = 16, BY = 12;                 // tile (waves 0-2), ring (wave 3)
 constexpr int PX = BX + 2, PY = BY + 2;
 constexpr int RX = BX + 4, RY = BY + 4;
-constexpr int NOWN = BX * BY, NRING = 2 * (BX + BY), NT = NOWN + NRING;
+constexpr int NOWN = BX * BY, NRING = 2 * (BX + BY), NT = 256;
+static_assert(NOWN == 192 && NRING <= 64, "three tile waves and one ring wave");
 constexpr int NP = PX * PY, NR = RX * RY;
 constexpr int NCH = NR / 2, NCI = (NCH + 63) / 64;   // 16-byte chunks of an FP64 window
 constexpr int NCF = (NR + 63) / 64;                  // dword loads of an FP32 window
@@ -169,11 +170,13 @@ struct MarchArgsMx {
     int ntx, ntiles, tiles_per_xcd;
 };
 
-static_assert(BX == 16 && BY == 16, "ring_cell assumes a 16x16 tile");
+// Ring cell r (0..NRING-1) -> P-plane coordinates (pi, pj): north row, south
+// row, west column, east column, corners excluded.
 __device__ __forceinline__ int2 ring_cell(int r) {
-    const int side = r >> 4, o = 1 + (r & 15);
-    const int pj = side == 0 ? 0 : (side == 1 ? PY - 1 : o);
-    const int pi = side <= 1 ? o : (side == 2 ? 0 : PX - 1);
+    const bool ns = r < 2 * BX;
+    const int rr = ns ? r : r - 2 * BX;
+    const int pi = ns ? 1 + (rr % BX) : (rr < BY ? 0 : PX - 1);
+    const int pj = ns ? (rr < BX ? 0 : PY - 1) : 1 + (rr % BY);
     return make_int2(pi, pj);
 }
 
@@ -290,12 +293,29 @@ __device__ __forceinline__ float d2tk_mx(int r, const double *B, const int *sK, 
 
 #define LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
 
+// Direct global->LDS loads issued through inline assembly: the compiler
+// then neither reorders them nor inserts vector-memory waits before this
+// wave's later LDS reads (which touch other slots); the staging wave waits
+// for them itself (wait_vmem) before the barrier that publishes the planes.
+// M0 holds the wave's LDS destination; nothing else in these kernels uses M0.
+__device__ __forceinline__ uint32_t lds_addr(const void *p) {
+    return __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)LDS_PTR(p));
+}
+__device__ __forceinline__ void dma16(const void *g, uint32_t lds) {
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(g)
+                 : "memory");
+}
+__device__ __forceinline__ void dma4(const void *g, uint32_t lds) {
+    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off" ::"s"(lds), "v"(g)
+                 : "memory");
+}
+
 __device__ __forceinline__ void stage_d(const double *src, double *dst, const int (&off)[NCI],
                                         int lane) {
 #pragma unroll
     for (int u = 0; u < NCI; ++u)
         if (u * 64 + lane < NCH)
-            __builtin_amdgcn_global_load_lds(src + off[u], LDS_PTR(dst + 128 * u), 16, 0, 0);
+            dma16(src + off[u], lds_addr(dst + 128 * u));
 }
 
 __device__ __forceinline__ void stage_f(const float *src, float *dst, const int (&off)[NCF],
@@ -303,7 +323,7 @@ __device__ __forceinline__ void stage_f(const float *src, float *dst, const int 
 #pragma unroll
     for (int u = 0; u < NCF; ++u)
         if (u * 64 + lane < NR)
-            __builtin_amdgcn_global_load_lds(src + off[u], LDS_PTR(dst + 64 * u), 4, 0, 0);
+            dma4(src + off[u], lds_addr(dst + 64 * u));
 }
 
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
@@ -327,11 +347,13 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
 
     const int t = threadIdx.x, lane = t & 63;
     const bool ringw = t >= NOWN;
-    const int2 rc = ring_cell((t - NOWN) & 63);
+    const int rl = t - NOWN;
+    const bool has_cell = !ringw || rl < NRING;
+    const int2 rc = ring_cell(has_cell && ringw ? rl : 0);
     const int pi = ringw ? rc.x : 1 + t % BX, pj = ringw ? rc.y : 1 + t / BX;
     const int gi = gi0 - 1 + pi, gj = gj0 - 1 + pj;
     const int rC = (pj + 1) * RX + (pi + 1), pC = pj * PX + pi;
-    const bool cell_ok = gj >= 1 && gj < jmt - 1 && gi >= 1 && gi < imt - 1;
+    const bool cell_ok = has_cell && gj >= 1 && gj < jmt - 1 && gi >= 1 && gi < imt - 1;
     const bool act = !ringw && gi < imt - 2 && gj < jmt - 2;
     const size_t q = cell_ok ? (size_t)gj * imt + gi : 0;
 
@@ -374,18 +396,13 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
         {
             LWf tmp;
             const LvM L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
-            sFL[0][pC] = cell_ok ? pred_flag_mx(g, rC, L0, sK, 0, dtsf, M, tmp) : false;
+            const bool f0 = cell_ok ? pred_flag_mx(g, rC, L0, sK, 0, dtsf, M, tmp) : false;
+            if (has_cell) sFL[0][pC] = f0;
         }
         bar_lds();
         for (int k = 0; k < km; ++k) {
             const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3, s3 = (k + 3) & 3;
             const int u0 = k % 3, u1 = (k + 1) % 3, u2 = (k + 2) % 3;
-            if (k + 1 < km) {
-                LWf tmp;
-                const LvM L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
-                sFL[(k + 1) & 1][pC] = cell_ok ? pred_flag_mx(g, rC, L1, sK, k + 1, dtsf, M, tmp) : false;
-            }
-            if (p.hmix == LICOM_HMIX_DEL4) sD2[pC] = cell_ok ? d2tk_mx(rC, sB[u0], sK, k, M) : 0.0f;
             if (k + 3 < km) {
                 stage_d(T + (size_t)(k + 3) * n2, sAT[s3], offd, lane);
                 stage_f(a.ws + (size_t)(k + 3) * n2, sW[s3], offf, lane);
@@ -395,6 +412,13 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
                 stage_f(a.vs + (size_t)(k + 2) * n2, sV[u2], offf, lane);
                 stage_d(TB + (size_t)(k + 2) * n2, sB[u2], offd, lane);
             }
+            if (k + 1 < km) {
+                LWf tmp;
+                const LvM L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
+                const bool f = cell_ok ? pred_flag_mx(g, rC, L1, sK, k + 1, dtsf, M, tmp) : false;
+                if (has_cell) sFL[(k + 1) & 1][pC] = f;
+            }
+            if (p.hmix == LICOM_HMIX_DEL4 && has_cell) sD2[pC] = cell_ok ? d2tk_mx(rC, sB[u0], sK, k, M) : 0.0f;
             bar_lds();
             wait_vmem();
             bar_lds();

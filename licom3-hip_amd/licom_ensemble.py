@@ -286,18 +286,25 @@ class RequestLoop:
                 "peak_active": peak_active}
 
 
-def poisson_requests(n, rate_per_s, seed_arrivals=5001, seed_requests=5002, t_pert=0.1):
-    """n requests with exponential inter-arrival times (rate per second); each
-    draws a member domain from the ensemble distribution and n_steps uniform
-    in 24..240."""
-    ra = np.random.default_rng(seed_arrivals)
+def request_specs(n, seed_requests=5002):
+    """(nx, ny, km, n_steps) of n requests: a member domain from the ensemble
+    distribution and n_steps uniform in 24..240."""
     rr = np.random.default_rng(seed_requests)
-    t = 0.0
     out = []
-    for k in range(n):
-        t += float(ra.exponential(1.0 / rate_per_s))
+    for _ in range(n):
         nx, ny = DOMAINS[int(rr.integers(len(DOMAINS)))]
         km = LEVELS[int(rr.integers(len(LEVELS)))]
-        steps = int(rr.integers(24, 241))
+        out.append((nx, ny, km, int(rr.integers(24, 241))))
+    return out
+
+
+def poisson_requests(n, rate_per_s, seed_arrivals=5001, seed_requests=5002, t_pert=0.1):
+    """n requests with exponential inter-arrival times (rate per second) and
+    the shapes of request_specs; request k's member is seeded 6000+k."""
+    ra = np.random.default_rng(seed_arrivals)
+    t = 0.0
+    out = []
+    for k, (nx, ny, km, steps) in enumerate(request_specs(n, seed_requests)):
+        t += float(ra.exponential(1.0 / rate_per_s))
         out.append(Request(k, t, member_case(6000 + k, nx, ny, km, t_pert), steps))
     return out
