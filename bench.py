@@ -46,6 +46,26 @@ def kernel_algorithmic_bytes(name, case, variant="fp64"):
             "k_side": 4 * 4,
         }
         return per_cell.get(name, 0) * cells
+    per_cell = {
+        # fused path
+        # utf vtf utl vtl -> u_wface v_sface wka; wka -> ws
+        "k_prelude": 8 * 4 + 8 * 3 + 8 + 8,
+        # at atb akt (x2) u_wface v_sface ws -> ori ax ay az dz (x2), penetrate
+        "k_march": 8 * (6 + 3) + 8 * (10 + 1),
+        # ori akt at (x2) -> at atb tend dt_diff dt_conv (x2)
+        "k_solve": 8 * 6 + 8 * 10,
+        # utf vtf read and written
+        "k_side": 8 * 4,
+        # per-stage path
+        # at(2) atb(2) akt(2) uw vs ws flags(2x1B) [d2tk(2)] -> ori(2) ax ay az dz (x2) pen
+        "k_correct": 8 * (2 + 2 + 2 + 2 + 1) + 2 + (16 if del4 else 0) + 8 * (2 + 8 + 1),
+        # ori(2) akt(2) at(2) utf vtf -> at(2) atb(2) tend dt_diff dt_conv (x2) utf vtf
+        "k_column": 8 * (2 + 2 + 2 + 2) + 8 * (2 + 2 + 6 + 2),
+        "k_predict": 8 * (2 + 2 + 1) + 2,
+        "k_faces": 8 * 4 + 8 * 2,
+        "k_ws": 8 * 4 + 8,
+        "k_d2tk": 8 * 2 + 8 * 2,
+    }
     return per_cell.get(name, 0) * cells
 
 

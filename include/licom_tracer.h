@@ -192,7 +192,7 @@ typedef struct licom_batch licom_batch;
 
 /* Launch-shape choices of one member (mapping only, never the arithmetic). */
 typedef struct licom_member_tuning {
-    int facediv_shape;        /* faces/divergence workgroup: 0: 64x4 cells,
+    int prelude_shape;        /* faces/continuity workgroup: 0: 64x4 columns,
                                  1: 32x8, 2: 128x2                          */
     int solve_shape;          /* column solve workgroup: 0: 4 rows x 32
                                  columns, 1: 2 x 64, 2: 1 x 128             */

@@ -37,97 +37,118 @@ namespace {
 constexpr int KMAXF = LICOM_MAX_KM;
 
 // ------------------------------------------------------------- prelude
-// Two passes.  k_facediv, one lane per cell of every level: the face
-// velocities u_wface / v_sface of advection_tracer_1_0 and the divergence
-// wka of operators_div on upwell_2's barotropic-corrected uk, vk
-// (tracer.cpp:83-119, 980-1027).  k_wcol, one lane per column: the column
-// sum and the continuity recursion of upwell_3 / upwell_4
-// (tracer.cpp:1029-1080), the divergence profile held in registers.
-__global__ __launch_bounds__(256) void k_facediv(KGrid g, KPar p, const double *__restrict__ utf,
-                                                 const double *__restrict__ utl,
-                                                 const double *__restrict__ vtf,
-                                                 const double *__restrict__ vtl,
-                                                 const double *__restrict__ work_u,
-                                                 double *__restrict__ uw, double *__restrict__ vs,
-                                                 double *__restrict__ wka) {
-    // wkd / wkb of a 65 x 5 window (the 64 x 4 block plus its east column
-    // and north row), each element read from HBM once
-    __shared__ double sD[5][65], sBv[5][65];
-    const int tx = threadIdx.x, ty = threadIdx.y;
+// One-pass prelude: a 64x4 block of columns marches k.  Each level's wkd /
+// wkb window (the block plus its east column and north row, 65 x 5) is
+// staged in LDS from registers loaded a level ahead, so every velocity value
+// is read from HBM once; the metrics and the barotropic factors
+// (1 + work_u*ohbu, upwell_2) are loaded once per column; the divergence
+// profile stays in registers for the column sum and the continuity recursion
+// of upwell_3/4 (tracer.cpp:1029-1080).
+template <int KM>
+__global__ __launch_bounds__(256) void k_prelude_col(KGrid g, KPar p, const double *__restrict__ utf,
+                                                     const double *__restrict__ utl,
+                                                     const double *__restrict__ vtf,
+                                                     const double *__restrict__ vtl,
+                                                     const double *__restrict__ work_u,
+                                                     const double *__restrict__ stf_pre,
+                                                     double *__restrict__ ws, double *__restrict__ uw,
+                                                     double *__restrict__ vs) {
+    constexpr int WX = 65, WN = 5 * WX;
+    __shared__ double sD[2][WN], sBv[2][WN];
+    const int t = threadIdx.x, tx = t % 64, ty = t / 64;
     const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
     const int i = i0 + tx, j = j0 + ty;
-    const int k = blockIdx.z;
-    const int imt = g.imt, jmt = g.jmt;
-    const size_t n2 = (size_t)imt * jmt, o = (size_t)k * n2;
-    const double oncc = p.oncc;
-    for (int e = ty * 64 + tx; e < 5 * 65; e += 256) {
-        const int wr = e / 65, wc = e - wr * 65;
-        const int jj = j0 - 1 + wr, ii = i0 + wc;
-        double d = 0.0, b = 0.0;
-        if (jj >= 0 && jj < jmt && ii < imt) {
-            const size_t c = o + (size_t)jj * imt + ii;
-            d = dev::wk(utf[c], utl[c], oncc);
-            b = dev::wk(vtf[c], vtl[c], oncc);
-        }
-        sD[wr][wc] = d;
-        sBv[wr][wc] = b;
-    }
-    __syncthreads();
-    if (i >= imt || j >= jmt) return;
-    const size_t q = (size_t)j * imt + i, c = o + q;
-    const int wr = ty + 1, wc = tx;          // window position of (j, i)
-    const double dc = sD[wr][wc], bc = sBv[wr][wc];
-    double u = 0.0, v = 0.0, d = 0.0;
-    if (j >= 1 && j < jmt - 1) u = (sD[wr - 1][wc] + dc) * g.htw[q] * 0.25;
-    if (i >= 1 && i < imt - 1) {
-        const double be = sBv[wr][wc + 1];
-        v = (bc + be) * g.hts[q] * 0.25;
-        if (j >= 1 && j < jmt - 1 && k <= g.kmt[q] - 1) {
-            const size_t qe = q + 1, qn = q - imt, qne = q - imt + 1;
-            const double fc = 1.0 + work_u[q] * g.ohbu[q];
-            const double fe = 1.0 + work_u[qe] * g.ohbu[qe];
-            const double fn = 1.0 + work_u[qn] * g.ohbu[qn];
-            const double fne = 1.0 + work_u[qne] * g.ohbu[qne];
-            const double de = sD[wr][wc + 1], dn = sD[wr - 1][wc], dne = sD[wr - 1][wc + 1];
-            const double bn = sBv[wr - 1][wc], bne = sBv[wr - 1][wc + 1];
-            d = 0.5 *
-                ((fe * de + fne * dne) * g.htw[qe] - (fc * dc + fn * dn) * g.htw[q] +
-                 (fe * be + fc * bc) * g.hts[q] - (fne * bne + fn * bn) * g.hts[qn]) *
-                g.tarea_r[q];
-        }
-    }
-    uw[c] = u;
-    vs[c] = v;
-    wka[c] = d;
-}
-
-template <int KM>
-__global__ __launch_bounds__(256) void k_wcol(KGrid g, const double *__restrict__ wka,
-                                              const double *__restrict__ stf_pre,
-                                              double *__restrict__ ws) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
-    if (i < 1 || i >= imt - 1 || j < 1 || j >= jmt - 1) return;
-    const size_t n2 = (size_t)imt * jmt, q = (size_t)j * imt + i;
-    const int kmt = g.kmt[q];
-    double d[KM];
+    const size_t n2 = (size_t)imt * jmt;
+    const double oncc = p.oncc;
+    // this lane's window elements: e = t and e = t + 256
+    size_t wq[2];
+    bool wv[2], wok[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int e = t + 256 * u, wr = e / WX, wc = e - wr * WX;
+        const int jj = j0 - 1 + wr, ii = i0 + wc;
+        wv[u] = e < WN;
+        wok[u] = wv[u] && jj >= 0 && jj < jmt && ii < imt;
+        wq[u] = wok[u] ? (size_t)jj * imt + ii : 0;
+    }
+    double f_ut[2], f_ul[2], f_vt[2], f_vl[2];
+    auto fetch = [&](int k) {
+        const size_t o = (size_t)k * n2;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            f_ut[u] = f_ul[u] = f_vt[u] = f_vl[u] = 0.0;
+            if (wok[u]) {
+                f_ut[u] = utf[o + wq[u]];
+                f_ul[u] = utl[o + wq[u]];
+                f_vt[u] = vtf[o + wq[u]];
+                f_vl[u] = vtl[o + wq[u]];
+            }
+        }
+    };
+    fetch(0);
+    const bool in = i < imt && j < jmt;
+    const size_t q = in ? (size_t)j * imt + i : 0;
+    const bool do_u = in && j >= 1 && j < jmt - 1;
+    const bool do_v = in && i >= 1 && i < imt - 1;
+    const bool do_w = do_u && do_v;
+    const double htw_c = in ? g.htw[q] : 0.0, hts_c = in ? g.hts[q] : 0.0;
+    double fc = 0, fe = 0, fn = 0, fne = 0, htw_e = 0, hts_n = 0, tr = 0;
+    int kmt = 0;
+    if (do_w) {
+        const size_t qe = q + 1, qn = q - imt, qne = q - imt + 1;
+        fc = 1.0 + work_u[q] * g.ohbu[q];
+        fe = 1.0 + work_u[qe] * g.ohbu[qe];
+        fn = 1.0 + work_u[qn] * g.ohbu[qn];
+        fne = 1.0 + work_u[qne] * g.ohbu[qne];
+        htw_e = g.htw[qe];
+        hts_n = g.hts[qn];
+        tr = g.tarea_r[q];
+        kmt = g.kmt[q];
+    }
+    const int w0 = (ty + 1) * WX + tx;
+    double wka[KM];
     double w = 0.0;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
-        d[k] = 0.0;
-        if (k < km && k <= kmt - 1) {
-            d[k] = wka[(size_t)k * n2 + q];
-            w = w - g.dzp[k] * d[k] * 1.0;
+        wka[k] = 0.0;
+        if (k < km) {
+            const int b = k & 1;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (wv[u]) {
+                    sD[b][t + 256 * u] = dev::wk(f_ut[u], f_ul[u], oncc);
+                    sBv[b][t + 256 * u] = dev::wk(f_vt[u], f_vl[u], oncc);
+                }
+            }
+            if (k + 1 < km) fetch(k + 1);
+            __syncthreads();
+            const double dc = sD[b][w0], bc = sBv[b][w0];
+            const size_t c = (size_t)k * n2 + q;
+            if (do_u) uw[c] = (sD[b][w0 - WX] + dc) * htw_c * 0.25;
+            if (do_v) vs[c] = (bc + sBv[b][w0 + 1]) * hts_c * 0.25;
+            if (do_w && k <= kmt - 1) {
+                const double de = sD[b][w0 + 1], dn = sD[b][w0 - WX], dne = sD[b][w0 - WX + 1];
+                const double be = sBv[b][w0 + 1], bn = sBv[b][w0 - WX], bne = sBv[b][w0 - WX + 1];
+                // operators_div (tracer.cpp:995-1003) on uk,vk of upwell_2
+                const double d = 0.5 *
+                                 ((fe * de + fne * dne) * htw_e - (fc * dc + fn * dn) * htw_c +
+                                  (fe * be + fc * bc) * hts_c - (fne * bne + fn * bn) * hts_n) *
+                                 tr;
+                wka[k] = d;
+                w = w - g.dzp[k] * d * 1.0;
+            }
         }
     }
+    if (!do_w) return;
+    // upwell_3 / upwell_4 (tracer.cpp:1045-1079)
     const double ohbt = g.ohbt[q];
     const double sc = 1.0 / (1.0 + stf_pre[q] * ohbt);
     double prev = ws[q];
 #pragma unroll
     for (int k = 1; k < KM; ++k) {
         if (k < km) {
-            const double cur = (k < kmt ? 1.0 : 0.0) * (prev + g.dzp[k - 1] * (w * ohbt + d[k - 1]));
+            const double cur = (k < kmt ? 1.0 : 0.0) * (prev + g.dzp[k - 1] * (w * ohbt + wka[k - 1]));
             ws[(size_t)k * n2 + q] = cur * sc;
             prev = cur;
         }
@@ -265,10 +286,6 @@ __device__ __forceinline__ uint32_t lds_addr(const void *p) {
 }
 __device__ __forceinline__ void dma16(const void *g, uint32_t lds) {
     asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %1, off" ::"s"(lds), "v"(g)
-                 : "memory");
-}
-__device__ __forceinline__ void dma4(const void *g, uint32_t lds) {
-    asm volatile("s_mov_b32 m0, %0\n\ts_nop 0\n\tglobal_load_lds_dword %1, off" ::"s"(lds), "v"(g)
                  : "memory");
 }
 
@@ -640,12 +657,9 @@ void launch_prelude_solve(licom_ctx *ctx, const licom_fields *f, const licom_dia
     hipStream_t st = ctx->stream;
     if (prelude) {
         Timer t(ctx, "k_prelude");
-        // the S half of ori holds wka until k_march writes it
-        double *wka = s.ori + ctx->n3;
-        const dim3 b(64, 4), gr((g.imt + 63) / 64, (g.jmt + 3) / 4);
-        k_facediv<<<dim3(gr.x, gr.y, g.km), b, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl,
-                                                        s.work_u, s.uw, s.vs, wka);
-        k_wcol<KM><<<gr, b, 0, st>>>(g, wka, s.stf_pre, f->ws);
+        const dim3 gr((g.imt + 63) / 64, (g.jmt + 3) / 4);
+        k_prelude_col<KM><<<gr, 256, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl, s.work_u,
+                                              s.stf_pre, f->ws, s.uw, s.vs);
     } else {
         Timer t(ctx, "k_solve");
         const dim3 gr = solve_grid(g.imt, g.jmt, 0);

@@ -12,7 +12,7 @@
 // O(10 K) tracer values, so FP32 keeps ~1e-7 relative accuracy on the
 // increments.
 //
-//   k_facediv_mx / k_wcol_mx<KM>  face velocities, divergence, continuity w
+//   k_prelude_mx<KM>              face velocities, divergence, continuity w
 //   k_march_mx                    four-wave k-march (as the FP64 k_march)
 //   k_solve<KM, float, float>     FP64 invtrit + convadj (tracer_solve.h)
 //   k_side3<float4> / k_side2_mx  in-place side effects
@@ -36,108 +36,118 @@ __device__ __forceinline__ float wkf(float uf, float ul, float oncc) {
 }
 
 // ------------------------------------------------------------- prelude
-// Launch shapes of k_facediv_mx: a 256-lane workgroup covers FX x (256/FX)
-// cells of one level; shape 0: 64 x 4, 1: 32 x 8, 2: 128 x 2.
+// Launch shapes of the prelude: a 256-lane workgroup covers FX x (256/FX)
+// columns; shape 0: 64 x 4, 1: 32 x 8, 2: 128 x 2.
 __host__ __device__ constexpr int fd_cols(int shape) { return shape == 0 ? 64 : (shape == 1 ? 32 : 128); }
 __host__ __device__ constexpr int fd_rows(int shape) { return 256 / fd_cols(shape); }
 
-__device__ __forceinline__ void facediv_body(const KGrid &g, float oncc, const float *__restrict__ utf,
-                                             const float *__restrict__ utl,
-                                             const float *__restrict__ vtf,
-                                             const float *__restrict__ vtl,
-                                             const double *__restrict__ work_u,
-                                             float *__restrict__ uw, float *__restrict__ vs,
-                                             float *__restrict__ wka, int bx, int by, int k,
-                                             int shape) {
-    // wkd / wkb of the (FX+1) x (FY+1) window (the block plus its east column
-    // and north row), each element read from HBM once
-    __shared__ float sD[400], sBv[400];
-    const int FX = fd_cols(shape), FY = fd_rows(shape), WX = FX + 1;
-    const int t = threadIdx.x, tx = t % FX, ty = t / FX;
-    const int i0 = bx * FX, j0 = by * FY;
-    const int i = i0 + tx, j = j0 + ty;
-    const int imt = g.imt, jmt = g.jmt;
-    const size_t n2 = (size_t)imt * jmt, o = (size_t)k * n2;
-    for (int e = t; e < WX * (FY + 1); e += 256) {
-        const int wr = e / WX, wc = e - wr * WX;
-        const int jj = j0 - 1 + wr, ii = i0 + wc;
-        float d = 0.0f, b = 0.0f;
-        if (jj >= 0 && jj < jmt && ii < imt) {
-            const size_t c = o + (size_t)jj * imt + ii;
-            d = wkf(utf[c], utl[c], oncc);
-            b = wkf(vtf[c], vtl[c], oncc);
-        }
-        sD[e] = d;
-        sBv[e] = b;
-    }
-    __syncthreads();
-    if (i >= imt || j >= jmt) return;
-    const size_t q = (size_t)j * imt + i, c = o + q;
-    const int w0 = (ty + 1) * WX + tx;       // window position of (j, i)
-    const float dc = sD[w0], bc = sBv[w0];
-    float u = 0.0f, v = 0.0f, d = 0.0f;
-    if (j >= 1 && j < jmt - 1) u = (sD[w0 - WX] + dc) * (float)g.htw[q] * 0.25f;
-    if (i >= 1 && i < imt - 1) {
-        const float be = sBv[w0 + 1];
-        v = (bc + be) * (float)g.hts[q] * 0.25f;
-        if (j >= 1 && j < jmt - 1 && k <= g.kmt[q] - 1) {
-            const size_t qe = q + 1, qn = q - imt, qne = q - imt + 1;
-            const float fc = (float)(1.0 + work_u[q] * g.ohbu[q]);
-            const float fe = (float)(1.0 + work_u[qe] * g.ohbu[qe]);
-            const float fn = (float)(1.0 + work_u[qn] * g.ohbu[qn]);
-            const float fne = (float)(1.0 + work_u[qne] * g.ohbu[qne]);
-            const float de = sD[w0 + 1], dn = sD[w0 - WX], dne = sD[w0 - WX + 1];
-            const float bn = sBv[w0 - WX], bne = sBv[w0 - WX + 1];
-            d = 0.5f *
-                ((fe * de + fne * dne) * (float)g.htw[qe] - (fc * dc + fn * dn) * (float)g.htw[q] +
-                 (fe * be + fc * bc) * (float)g.hts[q] - (fne * bne + fn * bn) * (float)g.hts[qn]) *
-                (float)g.tarea_r[q];
-        }
-    }
-    uw[c] = u;
-    vs[c] = v;
-    wka[c] = d;
-}
-
-__global__ __launch_bounds__(256) void k_facediv_mx(KGrid g, float oncc, const float *__restrict__ utf,
+// One-pass prelude of the mixed form (see k_prelude_col of the FP64 form):
+// a block of FX x FY columns (fd_cols / fd_rows) marches k with the
+// (FX+1) x (FY+1) window of wkd / wkb staged in LDS a level ahead, metrics
+// loaded once per column, the divergence profile in registers, the column
+// sum and the w recursion in FP64.
+template <int KM>
+__device__ __forceinline__ void prelude_col_mx_body(const KGrid &g, float oncc,
+                                                    const float *__restrict__ utf,
                                                     const float *__restrict__ utl,
                                                     const float *__restrict__ vtf,
                                                     const float *__restrict__ vtl,
                                                     const double *__restrict__ work_u,
-                                                    float *__restrict__ uw, float *__restrict__ vs,
-                                                    float *__restrict__ wka, int shape) {
-    facediv_body(g, oncc, utf, utl, vtf, vtl, work_u, uw, vs, wka, blockIdx.x, blockIdx.y, blockIdx.z,
-                 shape);
-}
-
-// Continuity w (upwell_3/4) with the column sum and the recursion in FP64.
-template <int KM>
-__device__ __forceinline__ void wcol_body(const KGrid &g, const float *__restrict__ wka,
-                                          const double *__restrict__ stf_pre, float *__restrict__ ws,
-                                          int bx, int by) {
-    const int i = bx * 64 + (int)threadIdx.x % 64;
-    const int j = by * 4 + (int)threadIdx.x / 64;
+                                                    const double *__restrict__ stf_pre,
+                                                    float *__restrict__ ws, float *__restrict__ uw,
+                                                    float *__restrict__ vs, int bx, int by, int shape) {
+    __shared__ float sD[2][400], sBv[2][400];
+    const int FX = fd_cols(shape), FY = fd_rows(shape), WX = FX + 1, WN = WX * (FY + 1);
+    const int t = threadIdx.x, tx = t % FX, ty = t / FX;
+    const int i0 = bx * FX, j0 = by * FY;
+    const int i = i0 + tx, j = j0 + ty;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
-    if (i < 1 || i >= imt - 1 || j < 1 || j >= jmt - 1) return;
-    const size_t n2 = (size_t)imt * jmt, q = (size_t)j * imt + i;
-    const int kmt = g.kmt[q];
-    float d[KM];
+    const size_t n2 = (size_t)imt * jmt;
+    size_t wq[2];
+    bool wv[2], wok[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int e = t + 256 * u, wr = e / WX, wc = e - wr * WX;
+        const int jj = j0 - 1 + wr, ii = i0 + wc;
+        wv[u] = e < WN;
+        wok[u] = wv[u] && jj >= 0 && jj < jmt && ii < imt;
+        wq[u] = wok[u] ? (size_t)jj * imt + ii : 0;
+    }
+    float f_ut[2], f_ul[2], f_vt[2], f_vl[2];
+    auto fetch = [&](int k) {
+        const size_t o = (size_t)k * n2;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            f_ut[u] = f_ul[u] = f_vt[u] = f_vl[u] = 0.0f;
+            if (wok[u]) {
+                f_ut[u] = utf[o + wq[u]];
+                f_ul[u] = utl[o + wq[u]];
+                f_vt[u] = vtf[o + wq[u]];
+                f_vl[u] = vtl[o + wq[u]];
+            }
+        }
+    };
+    fetch(0);
+    const bool in = i < imt && j < jmt;
+    const size_t q = in ? (size_t)j * imt + i : 0;
+    const bool do_u = in && j >= 1 && j < jmt - 1;
+    const bool do_v = in && i >= 1 && i < imt - 1;
+    const bool do_w = do_u && do_v;
+    const float htw_c = in ? (float)g.htw[q] : 0.0f, hts_c = in ? (float)g.hts[q] : 0.0f;
+    float fc = 0, fe = 0, fn = 0, fne = 0, htw_e = 0, hts_n = 0, tr = 0;
+    int kmt = 0;
+    if (do_w) {
+        const size_t qe = q + 1, qn = q - imt, qne = q - imt + 1;
+        fc = (float)(1.0 + work_u[q] * g.ohbu[q]);
+        fe = (float)(1.0 + work_u[qe] * g.ohbu[qe]);
+        fn = (float)(1.0 + work_u[qn] * g.ohbu[qn]);
+        fne = (float)(1.0 + work_u[qne] * g.ohbu[qne]);
+        htw_e = (float)g.htw[qe];
+        hts_n = (float)g.hts[qn];
+        tr = (float)g.tarea_r[q];
+        kmt = g.kmt[q];
+    }
+    const int w0 = (ty + 1) * WX + tx;
+    float wka[KM];
     double w = 0.0;
 #pragma unroll
     for (int k = 0; k < KM; ++k) {
-        d[k] = 0.0f;
-        if (k < km && k <= kmt - 1) {
-            d[k] = wka[(size_t)k * n2 + q];
-            w = w - g.dzp[k] * (double)d[k];
+        wka[k] = 0.0f;
+        if (k < km) {
+            const int b = k & 1;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                if (wv[u]) {
+                    sD[b][t + 256 * u] = wkf(f_ut[u], f_ul[u], oncc);
+                    sBv[b][t + 256 * u] = wkf(f_vt[u], f_vl[u], oncc);
+                }
+            }
+            if (k + 1 < km) fetch(k + 1);
+            __syncthreads();
+            const float dc = sD[b][w0], bc = sBv[b][w0];
+            const size_t c = (size_t)k * n2 + q;
+            if (do_u) uw[c] = (sD[b][w0 - WX] + dc) * htw_c * 0.25f;
+            if (do_v) vs[c] = (bc + sBv[b][w0 + 1]) * hts_c * 0.25f;
+            if (do_w && k <= kmt - 1) {
+                const float de = sD[b][w0 + 1], dn = sD[b][w0 - WX], dne = sD[b][w0 - WX + 1];
+                const float be = sBv[b][w0 + 1], bn = sBv[b][w0 - WX], bne = sBv[b][w0 - WX + 1];
+                const float d = 0.5f *
+                                ((fe * de + fne * dne) * htw_e - (fc * dc + fn * dn) * htw_c +
+                                 (fe * be + fc * bc) * hts_c - (fne * bne + fn * bn) * hts_n) *
+                                tr;
+                wka[k] = d;
+                w = w - g.dzp[k] * (double)d;
+            }
         }
     }
+    if (!do_w) return;
     const double ohbt = g.ohbt[q];
     const double sc = 1.0 / (1.0 + stf_pre[q] * ohbt);
     double prev = (double)ws[q];
 #pragma unroll
     for (int k = 1; k < KM; ++k) {
         if (k < km) {
-            const double cur = (k < kmt ? 1.0 : 0.0) * (prev + g.dzp[k - 1] * (w * ohbt + (double)d[k - 1]));
+            const double cur = (k < kmt ? 1.0 : 0.0) * (prev + g.dzp[k - 1] * (w * ohbt + (double)wka[k - 1]));
             ws[(size_t)k * n2 + q] = (float)(cur * sc);
             prev = cur;
         }
@@ -145,10 +155,16 @@ __device__ __forceinline__ void wcol_body(const KGrid &g, const float *__restric
 }
 
 template <int KM>
-__global__ __launch_bounds__(256) void k_wcol_mx(KGrid g, const float *__restrict__ wka,
-                                                 const double *__restrict__ stf_pre,
-                                                 float *__restrict__ ws) {
-    wcol_body<KM>(g, wka, stf_pre, ws, blockIdx.x, blockIdx.y);
+__global__ __launch_bounds__(256) void k_prelude_mx(KGrid g, float oncc, const float *__restrict__ utf,
+                                                    const float *__restrict__ utl,
+                                                    const float *__restrict__ vtf,
+                                                    const float *__restrict__ vtl,
+                                                    const double *__restrict__ work_u,
+                                                    const double *__restrict__ stf_pre,
+                                                    float *__restrict__ ws, float *__restrict__ uw,
+                                                    float *__restrict__ vs, int shape) {
+    prelude_col_mx_body<KM>(g, oncc, utf, utl, vtf, vtl, work_u, stf_pre, ws, uw, vs, blockIdx.x,
+                            blockIdx.y, shape);
 }
 
 // --------------------------------------------------------------- march
@@ -628,11 +644,9 @@ void launch_prelude_solve_mx(licom_ctx *ctx, const licom_fields_mixed *f, const 
     float *uw = reinterpret_cast<float *>(s.uw), *vs = reinterpret_cast<float *>(s.vs);
     if (prelude) {
         Timer t(ctx, "k_prelude");
-        float *wka = reinterpret_cast<float *>(s.ori + ctx->n3);
         const dim3 gr((g.imt + 63) / 64, (g.jmt + 3) / 4);
-        k_facediv_mx<<<dim3(gr.x, gr.y, g.km), 256, 0, st>>>(g, (float)p.oncc, f->utf, f->utl, f->vtf,
-                                                             f->vtl, s.work_u, uw, vs, wka, 0);
-        k_wcol_mx<KM><<<gr, 256, 0, st>>>(g, wka, s.stf_pre, f->ws);
+        k_prelude_mx<KM><<<gr, 256, 0, st>>>(g, (float)p.oncc, f->utf, f->utl, f->vtf, f->vtl, s.work_u,
+                                             s.stf_pre, f->ws, uw, vs, 0);
     } else {
         Timer t(ctx, "k_solve");
         const dim3 gr = solve_grid(g.imt, g.jmt, 0);
@@ -737,19 +751,19 @@ struct MemberDev {
     float *ax, *ay, *az, *dz, *tend, *dt_diff, *dt_conv, *pen;
     double *net, *fw_norm2;
     double *stf_pre, *work_u, *stf_n, *temp11, *partials, *err_norm2, *ori;
-    float *uw, *vs, *wka;
+    float *uw, *vs;
     int *mix_count;
     double area_t;
     int nblk;
     int sf_bx, sf_by;                 // 64 x 4 blocks of the 2-D kernels
-    int fd_shape, fd_bx, fd_by;       // faces/divergence blocks per level
+    int fd_shape, fd_bx, fd_by;       // prelude column blocks
     int sv_shape, sv_bx, sv_by;       // column-solve blocks
     int ntx, ntiles;                  // 16 x 16 k-march tiles
     int side_vec, side_blocks, side2_blocks;
     int halo_bx;                      // halo blocks per plane
 };
 
-enum { ST_SURF, ST_SSSP, ST_SSSF, ST_FD, ST_WC, ST_MARCH, ST_SOLVE, ST_SIDE3, ST_SIDE2, ST_HALO,
+enum { ST_SURF, ST_SSSP, ST_SSSF, ST_FD, ST_WC /* unused */, ST_MARCH, ST_SOLVE, ST_SIDE3, ST_SIDE2, ST_HALO,
        NSTAGE };
 
 // Member owning global block b of a stage (starts: nm+1 prefix sums).
@@ -787,17 +801,11 @@ __global__ __launch_bounds__(kRedThreads) void kb_sss_final(const MemberDev *M, 
     sss_final_block(d.partials, d.nblk, d.area_t, d.p.sssnorm, d.err_norm2, d.fw_norm2);
 }
 
-__global__ __launch_bounds__(256) void kb_facediv(const MemberDev *M, const int *starts, int nm) {
-    LICOM_MEMBER(ST_FD)
-    const int per = d.fd_bx * d.fd_by, k = l / per, r = l - k * per;
-    facediv_body(d.g, (float)d.p.oncc, d.utf, d.utl, d.vtf, d.vtl, d.work_u, d.uw, d.vs, d.wka,
-                 r % d.fd_bx, r / d.fd_bx, k, d.fd_shape);
-}
-
 template <int KM>
-__global__ __launch_bounds__(256) void kb_wcol(const MemberDev *M, const int *starts, int nm) {
-    LICOM_MEMBER(ST_WC)
-    wcol_body<KM>(d.g, d.wka, d.stf_pre, d.ws, l % d.sf_bx, l / d.sf_bx);
+__global__ __launch_bounds__(256) void kb_prelude(const MemberDev *M, const int *starts, int nm) {
+    LICOM_MEMBER(ST_FD)
+    prelude_col_mx_body<KM>(d.g, (float)d.p.oncc, d.utf, d.utl, d.vtf, d.vtl, d.work_u, d.stf_pre, d.ws,
+                            d.uw, d.vs, l % d.fd_bx, l / d.fd_bx, d.fd_shape);
 }
 
 // Persistent k-march: each workgroup walks the (member, tile, tracer) list.
@@ -911,10 +919,9 @@ int batch_launch_step(licom_batch *b, int flags) {
     kb_surface<<<grid(ST_SURF), 256, 0, st>>>(M, S, nm);
     kb_sss_partial<<<grid(ST_SSSP), kRedThreads, 0, st>>>(M, S, nm);
     kb_sss_final<<<grid(ST_SSSF), kRedThreads, 0, st>>>(M, S, nm);
-    kb_facediv<<<grid(ST_FD), 256, 0, st>>>(M, S, nm);
-    if (b->kmax <= 30) kb_wcol<30><<<grid(ST_WC), 256, 0, st>>>(M, S, nm);
-    else if (b->kmax <= 55) kb_wcol<55><<<grid(ST_WC), 256, 0, st>>>(M, S, nm);
-    else kb_wcol<LICOM_MAX_KM><<<grid(ST_WC), 256, 0, st>>>(M, S, nm);
+    if (b->kmax <= 30) kb_prelude<30><<<grid(ST_FD), 256, 0, st>>>(M, S, nm);
+    else if (b->kmax <= 55) kb_prelude<55><<<grid(ST_FD), 256, 0, st>>>(M, S, nm);
+    else kb_prelude<LICOM_MAX_KM><<<grid(ST_FD), 256, 0, st>>>(M, S, nm);
     const int items = b->totals[ST_MARCH];
     const int mg = b->march_grid > 0 ? std::min(b->march_grid, items) : items;
     kb_march<<<(unsigned)mg, NT, 0, st>>>(M, S, nm, items);
@@ -969,7 +976,7 @@ extern "C" int licom_batch_create(licom_ctx *const *members, const licom_fields_
             return LICOM_EINVAL;
         }
         const licom_member_tuning tu = tuning ? tuning[m] : licom_member_tuning{0, 0};
-        if (tu.facediv_shape < 0 || tu.facediv_shape > 2 || tu.solve_shape < 0 || tu.solve_shape > 2) {
+        if (tu.prelude_shape < 0 || tu.prelude_shape > 2 || tu.solve_shape < 0 || tu.solve_shape > 2) {
             set_error("licom_batch_create: member %d has an unknown launch shape", m);
             return LICOM_EINVAL;
         }
@@ -1017,7 +1024,6 @@ extern "C" int licom_batch_create(licom_ctx *const *members, const licom_fields_
         d.ori = s.ori;
         d.uw = reinterpret_cast<float *>(s.uw);
         d.vs = reinterpret_cast<float *>(s.vs);
-        d.wka = reinterpret_cast<float *>(s.ori + c->n3);
         d.mix_count = s.mix_count;
         d.area_t = c->area_t;
         d.nblk = s.nblk;
@@ -1025,7 +1031,7 @@ extern "C" int licom_batch_create(licom_ctx *const *members, const licom_fields_
         kmax = std::max(kmax, km);
         d.sf_bx = (imt + 63) / 64;
         d.sf_by = (jmt + 3) / 4;
-        d.fd_shape = tu.facediv_shape;
+        d.fd_shape = tu.prelude_shape;
         d.fd_bx = (imt + fd_cols(d.fd_shape) - 1) / fd_cols(d.fd_shape);
         d.fd_by = (jmt + fd_rows(d.fd_shape) - 1) / fd_rows(d.fd_shape);
         d.sv_shape = tu.solve_shape;
@@ -1043,8 +1049,8 @@ extern "C" int licom_batch_create(licom_ctx *const *members, const licom_fields_
         const long cnt[NSTAGE] = {(long)d.sf_bx * d.sf_by,
                                   d.nblk,
                                   1,
-                                  (long)d.fd_bx * d.fd_by * km,
-                                  (long)d.sf_bx * d.sf_by,
+                                  (long)d.fd_bx * d.fd_by,
+                                  0,
                                   2L * d.ntiles,
                                   (long)d.sv_bx * d.sv_by,
                                   d.side_blocks,

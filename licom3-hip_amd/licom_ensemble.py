@@ -38,7 +38,7 @@ LEVELS = (30, 55)
 
 
 class MemberTuning(ctypes.Structure):
-    _fields_ = [("facediv_shape", ctypes.c_int), ("solve_shape", ctypes.c_int)]
+    _fields_ = [("prelude_shape", ctypes.c_int), ("solve_shape", ctypes.c_int)]
 
 
 def _lib():
@@ -191,13 +191,13 @@ class Autotuner:
                                                   self.reps, ctypes.byref(best), ctypes.byref(ms)),
                      "licom_autotune_member")
             m.close()
-            self.table[k] = {"facediv_shape": best.facediv_shape, "solve_shape": best.solve_shape,
+            self.table[k] = {"prelude_shape": best.prelude_shape, "solve_shape": best.solve_shape,
                              "ms_per_step": ms.value}
             if self.cache_path:
                 with open(self.cache_path, "w") as f:
                     json.dump(self.table, f, indent=1)
         e = self.table[k]
-        return e["facediv_shape"], e["solve_shape"]
+        return e["prelude_shape"], e["solve_shape"]
 
 
 class Request:
