@@ -18,7 +18,7 @@ namespace {
 // level, and the substitution recomputes each SEG-level segment from its
 // checkpoint with the same operations (identical bits).  Both sweeps load
 // the next segment's inputs while computing the current one.
-constexpr int SEG = 8;
+constexpr int SEG = 4;
 
 // convadj_cu (convadj.cpp:52-256) for one column, driven by the mask of
 // statically unstable interfaces found with the unmixed densities (bit k:
