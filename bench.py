@@ -143,18 +143,23 @@ def summarize_pmc(fetch_dir, write_dir, workload, commands, variant="fp64"):
     print(json.dumps(out, indent=1))
 
 
-def summarize_trace(trace_dir, workload, command):
+def trace_file(workload, variant="fp64"):
+    tag = "tracer_step" if variant == "fp64" else "tracer_step_mixed"
+    return os.path.join(ROOT, "profiles", f"kernel_trace_{tag}_{workload}.md")
+
+
+def summarize_trace(trace_dir, workload, command, variant="fp64"):
     """Markdown table of a rocprofv3 --kernel-trace --stats run."""
     rows = read_rocprof_csv(trace_dir, "kernel_stats.csv")
-    lines = [f"# Kernel times, tracer step, {workload}", "", "Command (repo root):", "",
+    form = "FP64 form" if variant == "fp64" else "mixed-precision form"
+    lines = [f"# Kernel times, tracer step ({form}), {workload}", "", "Command (repo root):", "",
              "```", command, "```", "",
              "| kernel | calls | avg ms | min ms | max ms | share % |", "|---|---|---|---|---|---|"]
     for r in sorted(rows, key=lambda r: -float(r["TotalDurationNs"])):
         lines.append(f"| {short_kernel_name(r['Name'])} | {r['Calls']} | "
                      f"{float(r['AverageNs']) / 1e6:.3f} | {float(r['MinNs']) / 1e6:.3f} | "
                      f"{float(r['MaxNs']) / 1e6:.3f} | {float(r['Percentage']):.1f} |")
-    p = os.path.join(ROOT, "profiles", f"kernel_trace_tracer_step_{workload}.md")
-    with open(p, "w") as f:
+    with open(trace_file(workload, variant), "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
@@ -334,11 +339,12 @@ def main():
                       "mixed" if args.variant == "mixed" else "fp64")
         return
     if args.summarize_trace:
-        summarize_trace(args.summarize_trace, args.workload, " && ".join(args.command))
+        summarize_trace(args.summarize_trace, args.workload, " && ".join(args.command),
+                        "mixed" if args.variant == "mixed" else "fp64")
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the tracer step would shard by 2-D horizontal tiles; see DESIGN.md)",
+              "(the tracer step would shard by 2-D horizontal tiles with 2-cell halos)",
               file=sys.stderr)
         sys.exit(2)
 
