@@ -36,8 +36,9 @@ def kernel_algorithmic_bytes(name, case, variant="fp64"):
     del4 = case.params["hmix"] == 4
     if variant == "mixed":
         per_cell = {
-            # utf vtf utl vtl (f32) -> u_wface v_sface wka (f32); wka -> ws (f32)
-            "k_prelude": 4 * 4 + 4 * 3 + 4 + 4,
+            # utf vtf utl vtl (f32) -> u_wface v_sface ws (f32); the divergence
+            # profile stays in registers
+            "k_prelude": 4 * 4 + 4 * 3,
             # at atb (f64 x2) akt (f32 x2) faces ws (f32) -> ori (f64 x2), ax ay az dz (f32 x2), pen
             "k_march": 8 * 4 + 4 * 2 + 4 * 3 + 8 * 2 + 4 * 8 + 4,
             # ori at (f64 x2) akt (f32 x2) -> at atb (f64 x2) tend dt_diff dt_conv (f32 x2)
@@ -48,8 +49,9 @@ def kernel_algorithmic_bytes(name, case, variant="fp64"):
         return per_cell.get(name, 0) * cells
     per_cell = {
         # fused path
-        # utf vtf utl vtl -> u_wface v_sface wka; wka -> ws
-        "k_prelude": 8 * 4 + 8 * 3 + 8 + 8,
+        # utf vtf utl vtl -> u_wface v_sface ws; the divergence profile stays
+        # in registers
+        "k_prelude": 8 * 4 + 8 * 3,
         # at atb akt (x2) u_wface v_sface ws -> ori ax ay az dz (x2), penetrate
         "k_march": 8 * (6 + 3) + 8 * (10 + 1),
         # ori akt at (x2) -> at atb tend dt_diff dt_conv (x2)
@@ -232,8 +234,15 @@ def run_variant(model, args, torch, lt, case, updates):
                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(sb * updates / sec / 1e9 / HBM_PEAK_GBS, 4)}
     print(f"[bench] {v}: {1000 * sec:.2f} ms/step", file=sys.stderr, flush=True)
+    # algorithmic GB/s of every kernel with a per-cell byte count
+    kg = {}
+    for k, t in kt.items():
+        alg = kernel_algorithmic_bytes(k, case, v)
+        if alg:
+            kg[k] = round(alg / (t[0] / t[1] * 1e-3) / 1e9, 1)
     return {"value": updates / sec, "ms_per_step": 1000.0 * sec, "roofline": roof,
-            "step_roofline": step_roof, "kernel_ms": {k: round(t[0] / t[1], 4) for k, t in kt.items()}}
+            "step_roofline": step_roof, "kernel_ms": {k: round(t[0] / t[1], 4) for k, t in kt.items()},
+            "kernel_alg_gbs": kg}
 
 
 def run_ensemble(args, torch, lt, le):
@@ -370,11 +379,19 @@ def main():
         ref, mix = models["fp64"].state["at"], models["mixed"].state["at"]
         kmt = torch.from_numpy(case.grid["kmt"]).to(ref.device)
         wet = (torch.arange(case.km, device=ref.device)[:, None, None] < kmt[None])[:, 2:-2, 2:-2]
-        err = []
+        err, rms, frac = [], [], []
+        nwet = float(wet.sum())
         for n in range(2):
             d = (mix[n, :, 2:-2, 2:-2] - ref[n, :, 2:-2, 2:-2]).abs()
-            err.append(float(torch.where(wet, d, torch.zeros_like(d)).max()))
-        mixed_err = {"max_dT_K": err[0], "max_dS_psu": err[1] * 1000.0,
+            d = torch.where(wet, d, torch.zeros_like(d))
+            scale = 1.0 if n == 0 else 1000.0          # S is stored as (S - 35) / 1000
+            err.append(float(d.max()) * scale)
+            rms.append(float((d * d).sum() / nwet) ** 0.5 * scale)
+            frac.append(float((d * scale > 1e-4).sum()) / nwet)
+            del d
+        mixed_err = {"max_dT_K": err[0], "max_dS_psu": err[1], "rms_dT_K": rms[0],
+                     "rms_dS_psu": rms[1], "wet_fraction_dT_above_1e-4": frac[0],
+                     "wet_fraction_dS_above_1e-4": frac[1],
                      "steps": args.warmup + args.steps, "against": "FP64 drop-in"}
     for m in models.values():
         m.close()
@@ -411,6 +428,7 @@ def main():
         "roofline": r["roofline"],
         "step_roofline": r["step_roofline"],
         "kernel_ms": r["kernel_ms"],
+        "kernel_alg_gbs": r["kernel_alg_gbs"],
         "stream_copy_gbs": round(copy_gbs, 1),
         "cpu_baseline": cpu,
     }
@@ -419,7 +437,8 @@ def main():
         out["mixed"] = {"value": rm["value"], "ms_per_step": rm["ms_per_step"], "dtype": "f64/f32",
                         "speedup_vs_fp64": rm["value"] / r["value"],
                         "roofline": rm["roofline"], "step_roofline": rm["step_roofline"],
-                        "kernel_ms": rm["kernel_ms"], "error": mixed_err}
+                        "kernel_ms": rm["kernel_ms"], "kernel_alg_gbs": rm["kernel_alg_gbs"],
+                        "error": mixed_err}
     if ens_out:
         out["ensemble"] = ens_out
     if serve_out:

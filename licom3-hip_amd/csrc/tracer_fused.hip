@@ -1,11 +1,13 @@
 // tracer_fused.hip -- production path of the FP64 tracer step on gfx950.
 //
-// Five kernels instead of the reference's ~21 launches per tracer:
+// Four 3-D passes per step instead of the reference's ~21 launches per
+// tracer (plus the 2-D surface, SSSNORM and halo kernels that the staged
+// path in tracer_kernels.hip shares):
 //
-//   k_facediv      one lane per cell: face velocities u_wface/v_sface
-//                  (advection_tracer_1_0) and the divergence of the
-//                  barotropic-corrected velocity (upwell_2, operators_div).
-//   k_wcol<KM>     one lane per column: continuity w (upwell_3/4), the
+//   k_prelude_col<KM>  a 64x4 block of columns marches k: face velocities
+//                  u_wface/v_sface (advection_tracer_1_0), the divergence of
+//                  the barotropic-corrected velocity (upwell_2,
+//                  operators_div) and the continuity w (upwell_3/4), the
 //                  divergence profile held in registers.
 //   k_march        one four-wave workgroup per 16x12 tile of columns and
 //                  tracer, marching k top to bottom.  Waves 0-2 own the tile,
