@@ -354,10 +354,10 @@ int licom_tracer_step(licom_ctx *c, const licom_fields *f, const licom_diag *d, 
         return LICOM_EINVAL;
     }
     // The fused k-march stages planes in 16-byte chunks: it needs an even imt
-    // and 16-byte aligned state arrays; other layouts take the per-stage
-    // kernels (same results).
+    // and 16-byte aligned at, atb, ws and akt; other layouts take the
+    // per-stage kernels (same results).
     const uintptr_t al = reinterpret_cast<uintptr_t>(f->at) | reinterpret_cast<uintptr_t>(f->atb) |
-                         reinterpret_cast<uintptr_t>(f->ws);
+                         reinterpret_cast<uintptr_t>(f->ws) | reinterpret_cast<uintptr_t>(f->akt);
     if (c->imt % 2 != 0 || (al & 15) != 0) flags |= LICOM_STEP_STAGED;
     if (flags & LICOM_STEP_STAGED) {
         // scratch of the per-stage kernels, allocated on first use

@@ -54,11 +54,13 @@ __global__ __launch_bounds__(256) void k_prelude_col(KGrid g, KPar p, const doub
                                                      const double *__restrict__ work_u,
                                                      const double *__restrict__ stf_pre,
                                                      double *__restrict__ ws, double *__restrict__ uw,
-                                                     double *__restrict__ vs) {
+                                                     double *__restrict__ vs, XcdGrid xg) {
     constexpr int WX = 65, WN = 5 * WX;
     __shared__ double sD[2][WN], sBv[2][WN];
+    int bx, by;
+    if (!xcd_block(xg, bx, by)) return;
     const int t = threadIdx.x, tx = t % 64, ty = t / 64;
-    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
+    const int i0 = bx * 64, j0 = by * 4;
     const int i = i0 + tx, j = j0 + ty;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
     const size_t n2 = (size_t)imt * jmt;
@@ -220,8 +222,9 @@ struct Lv {
 
 // Predictor and limiter flag of level K at R-plane index r.
 __device__ __forceinline__ bool pred_flag(const KGrid &g, int r, const Lv &L, const int *sK, int K,
-                                          double dts, double tr, double D1, double D2, double D3,
-                                          double D4, LW &lw) {
+                                          const double *odzp, const double *odzt, double dts,
+                                          double tr, double D1, double D2, double D3, double D4,
+                                          LW &lw) {
     const double Tc = L.Pk[r], Te = L.Pk[r + 1], Tw = L.Pk[r - 1], Ts = L.Pk[r + RX],
                  Tn = L.Pk[r - RX];
     const double Tu = K > 0 ? L.Pu[r] : 0.0;
@@ -230,7 +233,7 @@ __device__ __forceinline__ bool pred_flag(const KGrid &g, int r, const Lv &L, co
     const double Wk = L.Wk[r], Wb = K < g.km - 1 ? L.Wb[r] : 0.0;
     dev::PTerms t;
     dev::pterms(Tc, Te, Tw, Ts, Tn, Tu, Td, Ue, Uw, Vs, Vn, Wk, Wb, tr, D1, D2, D3, D4, K, g.km,
-                g.odzp, g.odzt, dts, t);
+                odzp, odzt, dts, t);
     lw.xy1 = t.xy1;
     lw.xy2 = t.xy2;
     lw.xy3 = t.xy3;
@@ -309,6 +312,18 @@ __device__ __forceinline__ void stage_plane(const double *src, double *dst, cons
             dma16(src + off[u], lds_addr(dst + 128 * u));
 }
 
+// The 16x12 tile of one plane (akt: read only at the tile's own cells):
+// 12 rows of 16 doubles, 8 chunks per row, 96 chunks in two instructions;
+// chunk c lands at element 2c of the slot, which is the tile lane index.
+constexpr int NCA = BX * BY / 2;
+__device__ __forceinline__ void stage_tile(const double *src, double *dst, const int (&off)[2],
+                                           int lane) {
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+        if (u * 64 + lane < NCA)
+            dma16(src + off[u], lds_addr(dst + 128 * u));
+}
+
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 // Workgroup barrier that waits only for this wave's LDS traffic.  The
@@ -338,6 +353,11 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
     __shared__ double sD2[NP];
     __shared__ unsigned char sFL[2][NP];
     __shared__ int sK[NR];
+    // odzp, odzt and pen: per-level constants read from LDS, so that no
+    // vector-memory load inside the level loop carries a vmcnt wait (on gfx9
+    // vmcnt also covers the staged planes, the akt prefetch and the stores)
+    __shared__ double sZP[KMAXF], sZT[KMAXF], sPN[KMAXF];
+    __shared__ __align__(16) double sAK[NUV][NOWN];   // akt of the tile, staged like u/v
 
     // XCD-aware placement: consecutive blocks of one XCD take the T and S
     // halves of one tile, then the next tile of the same row of tiles.
@@ -351,7 +371,12 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
     const long e0 = (long)(gj0 - 2) * imt + (gi0 - 2);
 
     const int t = threadIdx.x, lane = t & 63;
-    const bool ringw = t >= NOWN;
+    // wave 3 (lanes NOWN..NT-1) is the ring/staging wave; the test goes
+    // through a scalar register so that the two roles compile to two
+    // separate loops behind a uniform branch (with a per-lane test they are
+    // merged into one exec-masked loop, and the tile waves' loads and stores
+    // then carry vmcnt waits into the staging wave's code)
+    const bool ringw = __builtin_amdgcn_readfirstlane(t >> 6) == NOWN / 64;
     const int rl = t - NOWN;                     // ring lane (wave 3)
     const bool has_cell = !ringw || rl < NRING;
     const int2 rc = ring_cell(has_cell && ringw ? rl : 0);
@@ -371,6 +396,11 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
         const int rj = gj0 - 2 + r / RX, ri = gi0 - 2 + r % RX;
         sK[r] = (rj < jmt && ri < imt) ? g.kmt[(size_t)rj * imt + ri] : 0;
     }
+    for (int r = t; r < km; r += NT) {
+        sZP[r] = g.odzp[r];
+        sZT[r] = g.odzt[r];
+        sPN[r] = r < km - 1 ? g.pen[r] : 0.0;
+    }
     Metrics M{};
     if (cell_ok) M = load_metrics(g, gj, gi);
     const double sw = (n == 0 && act && a.swv) ? a.swv[q] : 0.0;
@@ -387,7 +417,13 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
             const int c = min(u * 64 + lane, NCH - 1), row = c / (RX / 2), cp = c - row * (RX / 2);
             off[u] = (int)min(e0 + (long)row * imt + 2 * cp, (long)n2 - 2);
         }
-        // prologue: at 0..2, faces 0..1, ws 0..2, atb 0..1
+        int offa[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int c = min(u * 64 + lane, NCA - 1), row = c / (BX / 2), cp = c - row * (BX / 2);
+            offa[u] = (int)min((long)(gj0 + row) * imt + gi0 + 2 * cp, (long)n2 - 2);
+        }
+        // prologue: at 0..2, faces 0..1, ws 0..2, atb 0..1, akt 0..1
         for (int l = 0; l < 3 && l < km; ++l) {
             stage_plane(T + (size_t)l * n2, sAT[l], off, lane);
             stage_plane(a.ws + (size_t)l * n2, sW[l], off, lane);
@@ -396,13 +432,15 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
             stage_plane(a.uw + (size_t)l * n2, sU[l], off, lane);
             stage_plane(a.vs + (size_t)l * n2, sV[l], off, lane);
             stage_plane(TB + (size_t)l * n2, sB[l], off, lane);
+            stage_tile(A + (size_t)l * n2, sAK[l], offa, lane);
         }
         wait_vmem();
         bar_lds();
         {
             LW tmp;
             const Lv L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
-            const bool f0 = cell_ok ? pred_flag(g, rC, L0, sK, 0, dts, M.tr, M.D1, M.D2, M.D3, M.D4, tmp)
+            const bool f0 = cell_ok ? pred_flag(g, rC, L0, sK, 0, sZP, sZT, dts, M.tr, M.D1, M.D2, M.D3,
+                                                M.D4, tmp)
                                     : false;
             if (has_cell) sFL[0][pC] = f0;
         }
@@ -420,12 +458,14 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
                 stage_plane(a.uw + (size_t)(k + 2) * n2, sU[u2], off, lane);
                 stage_plane(a.vs + (size_t)(k + 2) * n2, sV[u2], off, lane);
                 stage_plane(TB + (size_t)(k + 2) * n2, sB[u2], off, lane);
+                stage_tile(A + (size_t)(k + 2) * n2, sAK[u2], offa, lane);
             }
             if (k + 1 < km) {
                 LW tmp;
                 const Lv L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
                 const bool f =
-                    cell_ok ? pred_flag(g, rC, L1, sK, k + 1, dts, M.tr, M.D1, M.D2, M.D3, M.D4, tmp)
+                    cell_ok ? pred_flag(g, rC, L1, sK, k + 1, sZP, sZT, dts, M.tr, M.D1, M.D2, M.D3,
+                                        M.D4, tmp)
                             : false;
                 if (has_cell) sFL[(k + 1) & 1][pC] = f;
             }
@@ -439,13 +479,16 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
     }
 
     // ---- waves 0-2: the tile
-    double fA = act ? A[q] : 0.0;
-    double fR = (rest && act) ? a.restore[n * n3 + q] : 0.0;
+    // the per-cell loads above complete here, before the level loop, so
+    // that no wait for them (which would also wait for the level's stores)
+    // stays inside it
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
     bar_lds();
     LW lc{}, ln{};
     {
         const Lv L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
-        sFL[0][pC] = cell_ok ? pred_flag(g, rC, L0, sK, 0, dts, M.tr, M.D1, M.D2, M.D3, M.D4, lc)
+        sFL[0][pC] = cell_ok ? pred_flag(g, rC, L0, sK, 0, sZP, sZT, dts, M.tr, M.D1, M.D2, M.D3, M.D4,
+                                         lc)
                              : false;
     }
     bar_lds();
@@ -459,14 +502,12 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
         const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3;
         const int u0 = k % 3, u1 = (k + 1) % 3;
         // ---- phase P(k)
-        const double Acur = act ? dev::floor_akt(fA, k, n, p.hmix, p.dwndmix) : 0.0;
-        const double Rk = fR;
+        const double Acur = act ? dev::floor_akt(sAK[u0][t], k, n, p.hmix, p.dwndmix) : 0.0;
         if (k + 1 < km) {
-            fA = act ? A[(size_t)(k + 1) * n2 + q] : 0.0;
-            fR = (rest && act) ? a.restore[n * n3 + (size_t)(k + 1) * n2 + q] : 0.0;
             const Lv L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
             sFL[(k + 1) & 1][pC] =
-                cell_ok ? pred_flag(g, rC, L1, sK, k + 1, dts, M.tr, M.D1, M.D2, M.D3, M.D4, ln)
+                cell_ok ? pred_flag(g, rC, L1, sK, k + 1, sZP, sZT, dts, M.tr, M.D1, M.D2, M.D3, M.D4,
+                                    ln)
                         : false;
         }
         if (p.hmix == LICOM_HMIX_DEL4)
@@ -500,7 +541,7 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
             }
             const double tr = M.tr;
             dev::PTerms tc;
-            dev::pterms_cheap(Tc, Te, Tw, Ts, Tn, Tu, Td, Ue, Uw, Vs, Vn, Wk, Wb, tr, k, km, g.odzp, tc);
+            dev::pterms_cheap(Tc, Te, Tw, Ts, Tn, Tu, Td, Ue, Uw, Vs, Vn, Wk, Wb, tr, k, km, sZP, tc);
             const double xy1 = up1 ? dev::up_xy1(Te, Tc, Ue, tr) : lc.xy1;
             const double xy2 = up2 ? dev::up_xy2(Tc, Tw, Uw, tr) : lc.xy2;
             const double xy3 = up3 ? dev::up_xy1(Ts, Tc, Vs, tr) : lc.xy3;
@@ -508,13 +549,13 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
             double zb1, zb2;
             if (k == 0) {
                 zb1 = 0.0;
-                zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, Wb, g.odzp[0]) : lc.zb2;
+                zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, Wb, sZP[0]) : lc.zb2;
             } else if (k == km - 1) {
-                zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, Wk, g.odzp[k]) : lc.zb1;
+                zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, Wk, sZP[k]) : lc.zb1;
                 zb2 = 0.0;
             } else {
-                zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, Wk, g.odzp[k]) : lc.zb1;
-                zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, Wb, g.odzp[k]) : lc.zb2;
+                zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, Wk, sZP[k]) : lc.zb1;
+                zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, Wb, sZP[k]) : lc.zb2;
             }
             const double xx = -(tc.x0 + xy1 + xy2 + tc.c1);
             const double yy = -(tc.y0 + xy3 + xy4 + tc.c2);
@@ -542,21 +583,21 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
             }
             // shortwave penetration (T) and explicit vertical diffusion
             // (tracer_5, boundary levels as its k==0 thread applies them)
-            const double *odzp = g.odzp, *odzt = g.odzt;
+            const double *odzp = sZP, *odzt = sZT;
             const double v1 = (k + 1 < km && k + 1 < kmt_own) ? 1.0 : 0.0;
             if (n == 0) {
                 double pv;
                 if (k == 0) {
-                    const double wt1 = sw * g.pen[0] * (1 < kmt_own ? 1.0 : 0.0);
+                    const double wt1 = sw * sPN[0] * (1 < kmt_own ? 1.0 : 0.0);
                     tf = tf - odzp[0] * wt1;
                     pv = -wt1 * odzp[0];
                 } else if (k == km - 1) {
-                    const double wt2 = sw * g.pen[km - 2] * vk;
+                    const double wt2 = sw * sPN[km - 2] * vk;
                     tf = tf + odzp[km - 1] * wt2;
                     pv = wt2 * odzp[km - 1];
                 } else {
-                    const double wt1 = sw * g.pen[k - 1] * vk;
-                    const double wt2 = sw * g.pen[k] * v1;
+                    const double wt1 = sw * sPN[k - 1] * vk;
+                    const double wt2 = sw * sPN[k] * v1;
                     tf = tf + (wt1 - wt2) * odzp[k];
                     pv = (wt1 - wt2) * odzp[k];
                 }
@@ -588,7 +629,10 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
                     tf = tf + stf_n * odzp[0] * 0.5;
                 }
             }
-            if (rest && k >= 1) tf = tf + vk * (Rk - Bc) * lamda;
+            if (rest && k >= 1) {
+                const double Rk = a.restore[n * n3 + c];
+                tf = tf + vk * (Rk - Bc) * lamda;
+            }
             // tracer_8: the explicit update entering the implicit solve
             a.ori[n * n3 + c] = Tc + dts * tf;
             Tprev = Tc;
@@ -659,15 +703,15 @@ void launch_prelude_solve(licom_ctx *ctx, const licom_fields *f, const licom_dia
     hipStream_t st = ctx->stream;
     if (prelude) {
         Timer t(ctx, "k_prelude");
-        const dim3 gr((g.imt + 63) / 64, (g.jmt + 3) / 4);
-        k_prelude_col<KM><<<gr, 256, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl, s.work_u,
-                                              s.stf_pre, f->ws, s.uw, s.vs);
+        const XcdGrid xg = xcd_grid((g.imt + 63) / 64, (g.jmt + 3) / 4);
+        k_prelude_col<KM><<<xcd_blocks(xg), 256, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl, s.work_u,
+                                                          s.stf_pre, f->ws, s.uw, s.vs, xg);
     } else {
         Timer t(ctx, "k_solve");
         const dim3 gr = solve_grid(g.imt, g.jmt, 0);
         k_solve<KM, double, double><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
-                                       d ? d->tend : nullptr, d ? d->dt_diff : nullptr,
-                                       d ? d->dt_conv : nullptr, s.mix_count, 0);
+                                                  d ? d->tend : nullptr, d ? d->dt_diff : nullptr,
+                                                  d ? d->dt_conv : nullptr, s.mix_count, 0);
     }
 }
 

@@ -94,6 +94,29 @@ int launch_step_mixed(licom_ctx *ctx, const licom_fields_mixed *f, const licom_d
 
 int launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t st);
 
+// XCD-aware 1-D launch of a gx x gy grid of column blocks.  The hardware
+// deals the workgroups of a launch to the eight XCDs round-robin, so
+// workgroup b runs on XCD b % 8; xcd_block gives it block L = (b % 8) * per +
+// b / 8 (row-major), so each XCD walks a contiguous run of blocks and the
+// cache lines neighbouring blocks share (halo columns, lines that straddle a
+// block edge) meet in that XCD's L2.  Launch xcd_blocks() workgroups; those
+// with L >= gx * gy return at once.
+struct XcdGrid {
+    int gx, n, per;
+};
+inline XcdGrid xcd_grid(unsigned gx, unsigned gy) {
+    const int n = (int)(gx * gy);
+    return XcdGrid{(int)gx, n, (n + 7) / 8};
+}
+inline unsigned xcd_blocks(const XcdGrid &x) { return 8u * (unsigned)x.per; }
+__device__ __forceinline__ bool xcd_block(const XcdGrid &x, int &bx, int &by) {
+    const int b = (int)blockIdx.x, L = (b & 7) * x.per + (b >> 3);
+    if (L >= x.n) return false;
+    by = L / x.gx;
+    bx = L - by * x.gx;
+    return true;
+}
+
 // RAII HIP-event bracket used when ctx->profiling is set.
 struct Timer {
     licom_ctx *ctx;

@@ -162,9 +162,10 @@ __global__ __launch_bounds__(256) void k_prelude_mx(KGrid g, float oncc, const f
                                                     const double *__restrict__ work_u,
                                                     const double *__restrict__ stf_pre,
                                                     float *__restrict__ ws, float *__restrict__ uw,
-                                                    float *__restrict__ vs, int shape) {
-    prelude_col_mx_body<KM>(g, oncc, utf, utl, vtf, vtl, work_u, stf_pre, ws, uw, vs, blockIdx.x,
-                            blockIdx.y, shape);
+                                                    float *__restrict__ vs, int shape, XcdGrid xg) {
+    int bx, by;
+    if (!xcd_block(xg, bx, by)) return;
+    prelude_col_mx_body<KM>(g, oncc, utf, utl, vtf, vtl, work_u, stf_pre, ws, uw, vs, bx, by, shape);
 }
 
 // --------------------------------------------------------------- march
@@ -245,7 +246,8 @@ __device__ __forceinline__ void cheap_mx(const Anom &a, float Ue, float Uw, floa
 
 // Predictor and limiter flag of level K at R-plane index r (mixed form).
 __device__ __forceinline__ bool pred_flag_mx(const KGrid &g, int r, const LvM &L, const int *sK,
-                                             int K, float dts, const MetricsF &M, LWf &lw) {
+                                             int K, const float *odzp, const float *odzt, float dts,
+                                             const MetricsF &M, LWf &lw) {
     const int km = g.km;
     const double Tc = L.Pk[r];
     Anom a;
@@ -257,15 +259,15 @@ __device__ __forceinline__ bool pred_flag_mx(const KGrid &g, int r, const LvM &L
     a.d = K < km - 1 ? (float)(L.Pd[r] - Tc) : 0.0f;
     const float Ue = L.U[r + 1], Uw = L.U[r], Vs = L.V[r], Vn = L.V[r - RX];
     const float Wk = K > 0 ? L.Wk[r] : 0.0f, Wb = K < km - 1 ? L.Wb[r] : 0.0f;
-    const float odzpk = (float)g.odzp[K];
+    const float odzpk = odzp[K];
     float xc, yc, zc;
     cheap_mx(a, Ue, Uw, Vs, Vn, Wk, Wb, M.tr, odzpk, xc, yc, zc);
     lw.xy1 = -(a.e * Ue * Ue) * M.cx1;
     lw.xy2 = -(a.w * Uw * Uw) * M.cx2;
     lw.xy3 = -(a.s * Vs * Vs) * M.cx3;
     lw.xy4 = -(a.n * Vn * Vn) * M.cx4;
-    lw.zb1 = K > 0 ? -0.5f * odzpk * Wk * Wk * (float)g.odzt[K] * a.u * dts : 0.0f;
-    lw.zb2 = K < km - 1 ? -0.5f * odzpk * Wb * Wb * (float)g.odzt[K + 1] * a.d * dts : 0.0f;
+    lw.zb1 = K > 0 ? -0.5f * odzpk * Wk * Wk * odzt[K] * a.u * dts : 0.0f;
+    lw.zb2 = K < km - 1 ? -0.5f * odzpk * Wb * Wb * odzt[K + 1] * a.d * dts : 0.0f;
     const float xx = -(xc + lw.xy1 + lw.xy2);
     const float yy = -(yc + lw.xy3 + lw.xy4);
     const float zz = -(lw.zb1 + lw.zb2 + zc);
@@ -342,6 +344,14 @@ __device__ __forceinline__ void stage_f(const float *src, float *dst, const int 
             dma4(src + off[u], lds_addr(dst + 64 * u));
 }
 
+// akt of the 16x12 tile (read only at the tile's own cells): 192 floats in
+// three dword instructions; element c of the slot is tile lane c.
+__device__ __forceinline__ void stage_tile_f(const float *src, float *dst, const int (&off)[3],
+                                             int lane) {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) dma4(src + off[u], lds_addr(dst + 64 * u));
+}
+
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 __device__ __forceinline__ void bar_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
@@ -355,6 +365,11 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
     __shared__ float sD2[NP];
     __shared__ unsigned char sFL[2][NP];
     __shared__ int sK[NR];
+    // per-level constants and the tile's akt come from LDS, so that no
+    // vector-memory load in the level loop carries a vmcnt wait (on gfx9
+    // vmcnt also covers the staged planes and the level's stores)
+    __shared__ float sZP[KMAXM], sZT[KMAXM], sPN[KMAXM];
+    __shared__ __align__(16) float sAK[3][NOWN];
 
     const int imt = g.imt, jmt = g.jmt, km = g.km;
     const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
@@ -362,7 +377,12 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
     const long e0 = (long)(gj0 - 2) * imt + (gi0 - 2);
 
     const int t = threadIdx.x, lane = t & 63;
-    const bool ringw = t >= NOWN;
+    // wave 3 (lanes NOWN..NT-1) is the ring/staging wave; the test goes
+    // through a scalar register so that the two roles compile to two
+    // separate loops behind a uniform branch (with a per-lane test they are
+    // merged into one exec-masked loop, and the tile waves' loads and stores
+    // then carry vmcnt waits into the staging wave's code)
+    const bool ringw = __builtin_amdgcn_readfirstlane(t >> 6) == NOWN / 64;
     const int rl = t - NOWN;
     const bool has_cell = !ringw || rl < NRING;
     const int2 rc = ring_cell(has_cell && ringw ? rl : 0);
@@ -383,6 +403,11 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
         const int rj = gj0 - 2 + r / RX, ri = gi0 - 2 + r % RX;
         sK[r] = (rj < jmt && ri < imt) ? g.kmt[(size_t)rj * imt + ri] : 0;
     }
+    for (int r = t; r < km; r += NT) {
+        sZP[r] = (float)g.odzp[r];
+        sZT[r] = (float)g.odzt[r];
+        sPN[r] = r < km - 1 ? (float)g.pen[r] : 0.0f;
+    }
     MetricsF M{};
     if (cell_ok) M = load_metrics_f(g, gj, gi, dts);
 
@@ -398,6 +423,12 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
             const int c = min(u * 64 + lane, NR - 1), row = c / RX, col = c - row * RX;
             offf[u] = (int)min(e0 + (long)row * imt + col, (long)n2 - 1);
         }
+        int offa[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const int c = u * 64 + lane, row = c / BX, col = c - row * BX;
+            offa[u] = (int)min((long)(gj0 + row) * imt + gi0 + col, (long)n2 - 1);
+        }
         for (int l = 0; l < 3 && l < km; ++l) {
             stage_d(T + (size_t)l * n2, sAT[l], offd, lane);
             stage_f(a.ws + (size_t)l * n2, sW[l], offf, lane);
@@ -406,13 +437,14 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
             stage_f(a.uw + (size_t)l * n2, sU[l], offf, lane);
             stage_f(a.vs + (size_t)l * n2, sV[l], offf, lane);
             stage_d(TB + (size_t)l * n2, sB[l], offd, lane);
+            stage_tile_f(A + (size_t)l * n2, sAK[l], offa, lane);
         }
         wait_vmem();
         bar_lds();
         {
             LWf tmp;
             const LvM L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
-            const bool f0 = cell_ok ? pred_flag_mx(g, rC, L0, sK, 0, dtsf, M, tmp) : false;
+            const bool f0 = cell_ok ? pred_flag_mx(g, rC, L0, sK, 0, sZP, sZT, dtsf, M, tmp) : false;
             if (has_cell) sFL[0][pC] = f0;
         }
         bar_lds();
@@ -427,11 +459,12 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
                 stage_f(a.uw + (size_t)(k + 2) * n2, sU[u2], offf, lane);
                 stage_f(a.vs + (size_t)(k + 2) * n2, sV[u2], offf, lane);
                 stage_d(TB + (size_t)(k + 2) * n2, sB[u2], offd, lane);
+                stage_tile_f(A + (size_t)(k + 2) * n2, sAK[u2], offa, lane);
             }
             if (k + 1 < km) {
                 LWf tmp;
                 const LvM L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
-                const bool f = cell_ok ? pred_flag_mx(g, rC, L1, sK, k + 1, dtsf, M, tmp) : false;
+                const bool f = cell_ok ? pred_flag_mx(g, rC, L1, sK, k + 1, sZP, sZT, dtsf, M, tmp) : false;
                 if (has_cell) sFL[(k + 1) & 1][pC] = f;
             }
             if (p.hmix == LICOM_HMIX_DEL4 && has_cell) sD2[pC] = cell_ok ? d2tk_mx(rC, sB[u0], sK, k, M) : 0.0f;
@@ -447,13 +480,16 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
     const double stf_n = act ? a.stf_n[n * n2 + q] : 0.0;
     const double enorm = (n == 1 && p.sssnorm && act) ? a.err_norm2[0] : 0.0;
     const bool rest = p.boundary_restore == 1 && a.restore != nullptr;
-    float fA = act ? A[q] : 0.0f;
-    double fR = (rest && act) ? a.restore[n * n3 + q] : 0.0;
+    const double odzp0 = g.odzp[0];
+    // the per-cell loads above complete here, before the level loop, so
+    // that no wait for them (which would also wait for the level's stores)
+    // stays inside it
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
     bar_lds();
     LWf lc{}, ln{};
     {
         const LvM L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
-        sFL[0][pC] = cell_ok ? pred_flag_mx(g, rC, L0, sK, 0, dtsf, M, lc) : false;
+        sFL[0][pC] = cell_ok ? pred_flag_mx(g, rC, L0, sK, 0, sZP, sZT, dtsf, M, lc) : false;
     }
     bar_lds();
 
@@ -466,13 +502,10 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
     for (int k = 0; k < km; ++k) {
         const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3;
         const int u0 = k % 3, u1 = (k + 1) % 3;
-        const float Acur = act ? (float)dev::floor_akt((double)fA, k, n, p.hmix, p.dwndmix) : 0.0f;
-        const double Rk = fR;
+        const float Acur = act ? (float)dev::floor_akt((double)sAK[u0][t], k, n, p.hmix, p.dwndmix) : 0.0f;
         if (k + 1 < km) {
-            fA = act ? A[(size_t)(k + 1) * n2 + q] : 0.0f;
-            fR = (rest && act) ? a.restore[n * n3 + (size_t)(k + 1) * n2 + q] : 0.0;
             const LvM L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
-            sFL[(k + 1) & 1][pC] = cell_ok ? pred_flag_mx(g, rC, L1, sK, k + 1, dtsf, M, ln) : false;
+            sFL[(k + 1) & 1][pC] = cell_ok ? pred_flag_mx(g, rC, L1, sK, k + 1, sZP, sZT, dtsf, M, ln) : false;
         }
         if (p.hmix == LICOM_HMIX_DEL4) sD2[pC] = cell_ok ? d2tk_mx(rC, sB[u0], sK, k, M) : 0.0f;
         bar_lds();
@@ -508,7 +541,7 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
                 up4 = (fc || fn) && gj >= 2;
             }
             const float tr = M.tr;
-            const float odzpk = (float)g.odzp[k];
+            const float odzpk = sZP[k];
             float xc, yc, zc;
             cheap_mx(an, Ue, Uw, Vs, Vn, Wk, Wb, tr, odzpk, xc, yc, zc);
             const float xy1 = up1 ? -an.e * fabsf(Ue) * tr : lc.xy1;
@@ -554,11 +587,11 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
                 float pv;
                 const float swf = (float)sw;
                 if (k == 0) {
-                    pv = -swf * (float)g.pen[0] * (1 < kmt_own ? 1.0f : 0.0f) * odzpk;
+                    pv = -swf * sPN[0] * (1 < kmt_own ? 1.0f : 0.0f) * odzpk;
                 } else if (k == km - 1) {
-                    pv = swf * (float)g.pen[km - 2] * vk * odzpk;
+                    pv = swf * sPN[km - 2] * vk * odzpk;
                 } else {
-                    pv = (swf * (float)g.pen[k - 1] * vk - swf * (float)g.pen[k] * v1) * odzpk;
+                    pv = (swf * sPN[k - 1] * vk - swf * sPN[k] * v1) * odzpk;
                 }
                 tf += pv;
                 if (a.pen) a.pen[c] = pv;
@@ -566,23 +599,26 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
             const double Bc = Bk[r];
             float dzv;
             if (k == 0) {
-                dzv = -odzpk * Acur * (float)(Bc - Bk1[r]) * (float)g.odzt[1] * v1 * 0.5f;
+                dzv = -odzpk * Acur * (float)(Bc - Bk1[r]) * sZT[1] * v1 * 0.5f;
             } else if (k == km - 1) {
-                dzv = odzpk * Aprev * (float)(Bprev - Bc) * (float)g.odzt[km - 1] * vk * 0.5f;
+                dzv = odzpk * Aprev * (float)(Bprev - Bc) * sZT[km - 1] * vk * 0.5f;
             } else {
-                const float wt1 = Aprev * (float)(Bprev - Bc) * (float)g.odzt[k] * vk;
-                const float wt2 = Acur * (float)(Bc - Bk1[r]) * (float)g.odzt[k + 1] * v1;
+                const float wt1 = Aprev * (float)(Bprev - Bc) * sZT[k] * vk;
+                const float wt2 = Acur * (float)(Bc - Bk1[r]) * sZT[k + 1] * v1;
                 dzv = odzpk * (wt1 - wt2) * 0.5f;
             }
             tf += dzv;
             if (a.dz) a.dz[n * n3 + c] = dzv;
             if (k == 0 && kmt_own > 0) {
-                tf += (float)(stf_n * 0.5 * g.odzp[0]);
+                tf += (float)(stf_n * 0.5 * odzp0);
                 if (n == 1 && p.sssnorm) tf += (float)enorm * vk;
             } else if (k == 0 && n == 1 && p.sssnorm) {
                 tf += (float)enorm * vk;
             }
-            if (rest && k >= 1) tf += vk * (float)((Rk - Bc) * lamda);
+            if (rest && k >= 1) {
+                const double Rk = a.restore[n * n3 + c];
+                tf += vk * (float)((Rk - Bc) * lamda);
+            }
             a.ori[n * n3 + c] = Tc + dts * (double)tf;
             Tprev = Tc;
             Bprev = Bc;
@@ -644,15 +680,15 @@ void launch_prelude_solve_mx(licom_ctx *ctx, const licom_fields_mixed *f, const 
     float *uw = reinterpret_cast<float *>(s.uw), *vs = reinterpret_cast<float *>(s.vs);
     if (prelude) {
         Timer t(ctx, "k_prelude");
-        const dim3 gr((g.imt + 63) / 64, (g.jmt + 3) / 4);
-        k_prelude_mx<KM><<<gr, 256, 0, st>>>(g, (float)p.oncc, f->utf, f->utl, f->vtf, f->vtl, s.work_u,
-                                             s.stf_pre, f->ws, uw, vs, 0);
+        const XcdGrid xg = xcd_grid((g.imt + 63) / 64, (g.jmt + 3) / 4);
+        k_prelude_mx<KM><<<xcd_blocks(xg), 256, 0, st>>>(g, (float)p.oncc, f->utf, f->utl, f->vtf, f->vtl,
+                                                         s.work_u, s.stf_pre, f->ws, uw, vs, 0, xg);
     } else {
         Timer t(ctx, "k_solve");
         const dim3 gr = solve_grid(g.imt, g.jmt, 0);
         k_solve<KM, float, float><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
-                                                      d ? d->tend : nullptr, d ? d->dt_diff : nullptr,
-                                                      d ? d->dt_conv : nullptr, s.mix_count, 0);
+                                                  d ? d->tend : nullptr, d ? d->dt_diff : nullptr,
+                                                  d ? d->dt_conv : nullptr, s.mix_count, 0);
     }
 }
 
