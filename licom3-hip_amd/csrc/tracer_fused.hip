@@ -169,7 +169,14 @@ constexpr int NRING = 2 * (BX + BY);            // 56 ring cells: wave 3 (the ri
 constexpr int NT = 256;                         // four waves
 static_assert(NOWN == 192 && NRING <= 64, "three tile waves and one ring wave");
 constexpr int NP = PX * PY, NR = RX * RY;
-constexpr int NAT = 4, NUV = 3, NW = 4, NB = 3; // LDS slots per staged array
+// Staged planes are loaded LA levels ahead of the phase that first reads
+// them: at and ws are issued for level k+LA+2 at the start of P(k), the face
+// velocities, atb and akt for level k+LA+1.  LDS slots per array follow.
+constexpr int LA = 2;
+constexpr int NAT = LA + 3, NW = LA + 3, NUV = LA + 2, NB = LA + 2;
+// direct-to-LDS instructions the staging wave issues per level
+constexpr int DMA_ATW = 2 * ((NR / 2 + 63) / 64);          // at, ws
+constexpr int DMA_UVBA = 3 * ((NR / 2 + 63) / 64) + 2;     // u, v, atb + akt
 
 struct MarchArgs {
     const double *at, *atb, *akt, *uw, *vs, *ws, *stf_n, *err_norm2, *swv, *restore;
@@ -325,6 +332,14 @@ __device__ __forceinline__ void stage_tile(const double *src, double *dst, const
 }
 
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Waits for every staged load except the `issued` issued at the start of
+// the current level (direct-to-LDS loads complete in order).
+__device__ __forceinline__ void wait_vmem_keep(int issued) {
+    static_assert(DMA_ATW + DMA_UVBA == 17 && DMA_UVBA == 11, "wait counts below");
+    if (LA < 2 || issued == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    else if (issued >= 17) asm volatile("s_waitcnt vmcnt(17)" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(11)" ::: "memory");
+}
 
 // Workgroup barrier that waits only for this wave's LDS traffic.  The
 // staging wave's direct-to-LDS loads are waited for explicitly (wait_vmem)
@@ -344,7 +359,7 @@ __device__ __forceinline__ void bar_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n
 //   C(k): limiter, fluxes and explicit update of level k on the tile (waves
 //         0-2); wave 3 waits for its loads.
 // LDS slots rotate: at x4, u_wface/v_sface x3, ws x4, atb x3.
-__global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
+__global__ __launch_bounds__(NT, (LA >= 2 ? 2 : 3)) void k_march(KGrid g, KPar p, MarchArgs a) {
     __shared__ __align__(16) double sAT[NAT][NR];
     __shared__ __align__(16) double sU[NUV][NR];
     __shared__ __align__(16) double sV[NUV][NR];
@@ -357,7 +372,7 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
     // vector-memory load inside the level loop carries a vmcnt wait (on gfx9
     // vmcnt also covers the staged planes, the akt prefetch and the stores)
     __shared__ double sZP[KMAXF], sZT[KMAXF], sPN[KMAXF];
-    __shared__ __align__(16) double sAK[NUV][NOWN];   // akt of the tile, staged like u/v
+    __shared__ __align__(16) double sAK[NB][NOWN];    // akt of the tile, staged like atb
 
     // XCD-aware placement: consecutive blocks of one XCD take the T and S
     // halves of one tile, then the next tile of the same row of tiles.
@@ -423,12 +438,12 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
             const int c = min(u * 64 + lane, NCA - 1), row = c / (BX / 2), cp = c - row * (BX / 2);
             offa[u] = (int)min((long)(gj0 + row) * imt + gi0 + 2 * cp, (long)n2 - 2);
         }
-        // prologue: at 0..2, faces 0..1, ws 0..2, atb 0..1, akt 0..1
-        for (int l = 0; l < 3 && l < km; ++l) {
+        // prologue: at, ws 0..LA+1; faces, atb, akt 0..LA
+        for (int l = 0; l < NAT - 1 && l < km; ++l) {
             stage_plane(T + (size_t)l * n2, sAT[l], off, lane);
             stage_plane(a.ws + (size_t)l * n2, sW[l], off, lane);
         }
-        for (int l = 0; l < 2 && l < km; ++l) {
+        for (int l = 0; l < NUV - 1 && l < km; ++l) {
             stage_plane(a.uw + (size_t)l * n2, sU[l], off, lane);
             stage_plane(a.vs + (size_t)l * n2, sV[l], off, lane);
             stage_plane(TB + (size_t)l * n2, sB[l], off, lane);
@@ -446,19 +461,22 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
         }
         bar_lds();
         for (int k = 0; k < km; ++k) {
-            const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3, s3 = (k + 3) & 3;
-            const int u0 = k % 3, u1 = (k + 1) % 3, u2 = (k + 2) % 3;
-            // planes for P(k+1) and C(k+1), issued first so that they have the
-            // whole level to land; their slots were last read in C(k-1)
-            if (k + 3 < km) {
-                stage_plane(T + (size_t)(k + 3) * n2, sAT[s3], off, lane);
-                stage_plane(a.ws + (size_t)(k + 3) * n2, sW[s3], off, lane);
+            const int s0 = k % NAT, s1 = (k + 1) % NAT, s2 = (k + 2) % NAT, s3 = (k + LA + 2) % NAT;
+            const int u0 = k % NUV, u1 = (k + 1) % NUV, u2 = (k + LA + 1) % NUV;
+            // planes for P(k+LA) and C(k+LA), issued first so that they have
+            // LA levels to land; their slots were last read in C(k-1)
+            int issued = 0;
+            if (k + LA + 2 < km) {
+                stage_plane(T + (size_t)(k + LA + 2) * n2, sAT[s3], off, lane);
+                stage_plane(a.ws + (size_t)(k + LA + 2) * n2, sW[s3], off, lane);
+                issued += DMA_ATW;
             }
-            if (k + 2 < km) {
-                stage_plane(a.uw + (size_t)(k + 2) * n2, sU[u2], off, lane);
-                stage_plane(a.vs + (size_t)(k + 2) * n2, sV[u2], off, lane);
-                stage_plane(TB + (size_t)(k + 2) * n2, sB[u2], off, lane);
-                stage_tile(A + (size_t)(k + 2) * n2, sAK[u2], offa, lane);
+            if (k + LA + 1 < km) {
+                stage_plane(a.uw + (size_t)(k + LA + 1) * n2, sU[u2], off, lane);
+                stage_plane(a.vs + (size_t)(k + LA + 1) * n2, sV[u2], off, lane);
+                stage_plane(TB + (size_t)(k + LA + 1) * n2, sB[u2], off, lane);
+                stage_tile(A + (size_t)(k + LA + 1) * n2, sAK[u2], offa, lane);
+                issued += DMA_UVBA;
             }
             if (k + 1 < km) {
                 LW tmp;
@@ -471,9 +489,9 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
             }
             if (p.hmix == LICOM_HMIX_DEL4 && has_cell)
                 sD2[pC] = cell_ok ? d2tk_at(rC, sB[u0], sK, k, M.dtn, M.dts, M.dte, M.dtw, M.ahf) : 0.0;
-            bar_lds();     // end of P(k)
-            wait_vmem();   // C(k): the staged planes land
-            bar_lds();     // end of C(k)
+            bar_lds();                // end of P(k)
+            wait_vmem_keep(issued);   // C(k): the planes P(k+1) and C(k+1) read land
+            bar_lds();                // end of C(k)
         }
         return;
     }
@@ -499,8 +517,8 @@ __global__ __launch_bounds__(NT, 3) void k_march(KGrid g, KPar p, MarchArgs a) {
     const double aidif = 0.5;
 
     for (int k = 0; k < km; ++k) {
-        const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3;
-        const int u0 = k % 3, u1 = (k + 1) % 3;
+        const int s0 = k % NAT, s1 = (k + 1) % NAT, s2 = (k + 2) % NAT;
+        const int u0 = k % NUV, u1 = (k + 1) % NUV;
         // ---- phase P(k)
         const double Acur = act ? dev::floor_akt(sAK[u0][t], k, n, p.hmix, p.dwndmix) : 0.0;
         if (k + 1 < km) {

@@ -178,6 +178,12 @@ constexpr int NP = PX * PY, NR = RX * RY;
 constexpr int NCH = NR / 2, NCI = (NCH + 63) / 64;   // 16-byte chunks of an FP64 window
 constexpr int NCF = (NR + 63) / 64;                  // dword loads of an FP32 window
 constexpr float BIG = 1.0e10f;
+// Look-ahead of the staged planes (as in the FP64 k_march): at and ws are
+// issued for level k+LA+2 at the start of P(k); faces, atb, akt for k+LA+1.
+constexpr int LA = 2;
+constexpr int NAT = LA + 3, NW = LA + 3, NUV = LA + 2, NB = LA + 2;
+constexpr int DMA_ATW = NCI + NCF;                  // at (16-byte), ws (dword)
+constexpr int DMA_UVBA = 2 * NCF + NCI + 3;         // u, v (dword), atb (16-byte), akt
 
 struct MarchArgsMx {
     const double *at, *atb, *stf_n, *err_norm2, *swv, *restore;
@@ -353,15 +359,23 @@ __device__ __forceinline__ void stage_tile_f(const float *src, float *dst, const
 }
 
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// Waits for every staged load except the `issued` issued at the start of
+// the current level (direct-to-LDS loads complete in order).
+__device__ __forceinline__ void wait_vmem_keep(int issued) {
+    static_assert(DMA_ATW + DMA_UVBA == 24 && DMA_UVBA == 16, "wait counts below");
+    if (LA < 2 || issued == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    else if (issued >= 24) asm volatile("s_waitcnt vmcnt(24)" ::: "memory");
+    else asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+}
 __device__ __forceinline__ void bar_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const MarchArgsMx &a, int tile,
                                            int n) {
-    __shared__ __align__(16) double sAT[4][NR];
-    __shared__ __align__(16) double sB[3][NR];
-    __shared__ __align__(16) float sU[3][NR];
-    __shared__ __align__(16) float sV[3][NR];
-    __shared__ __align__(16) float sW[4][NR];
+    __shared__ __align__(16) double sAT[NAT][NR];
+    __shared__ __align__(16) double sB[NB][NR];
+    __shared__ __align__(16) float sU[NUV][NR];
+    __shared__ __align__(16) float sV[NUV][NR];
+    __shared__ __align__(16) float sW[NW][NR];
     __shared__ float sD2[NP];
     __shared__ unsigned char sFL[2][NP];
     __shared__ int sK[NR];
@@ -369,7 +383,7 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
     // vector-memory load in the level loop carries a vmcnt wait (on gfx9
     // vmcnt also covers the staged planes and the level's stores)
     __shared__ float sZP[KMAXM], sZT[KMAXM], sPN[KMAXM];
-    __shared__ __align__(16) float sAK[3][NOWN];
+    __shared__ __align__(16) float sAK[NB][NOWN];
 
     const int imt = g.imt, jmt = g.jmt, km = g.km;
     const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
@@ -429,11 +443,11 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
             const int c = u * 64 + lane, row = c / BX, col = c - row * BX;
             offa[u] = (int)min((long)(gj0 + row) * imt + gi0 + col, (long)n2 - 1);
         }
-        for (int l = 0; l < 3 && l < km; ++l) {
+        for (int l = 0; l < NAT - 1 && l < km; ++l) {
             stage_d(T + (size_t)l * n2, sAT[l], offd, lane);
             stage_f(a.ws + (size_t)l * n2, sW[l], offf, lane);
         }
-        for (int l = 0; l < 2 && l < km; ++l) {
+        for (int l = 0; l < NUV - 1 && l < km; ++l) {
             stage_f(a.uw + (size_t)l * n2, sU[l], offf, lane);
             stage_f(a.vs + (size_t)l * n2, sV[l], offf, lane);
             stage_d(TB + (size_t)l * n2, sB[l], offd, lane);
@@ -449,17 +463,20 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
         }
         bar_lds();
         for (int k = 0; k < km; ++k) {
-            const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3, s3 = (k + 3) & 3;
-            const int u0 = k % 3, u1 = (k + 1) % 3, u2 = (k + 2) % 3;
-            if (k + 3 < km) {
-                stage_d(T + (size_t)(k + 3) * n2, sAT[s3], offd, lane);
-                stage_f(a.ws + (size_t)(k + 3) * n2, sW[s3], offf, lane);
+            const int s0 = k % NAT, s1 = (k + 1) % NAT, s2 = (k + 2) % NAT, s3 = (k + LA + 2) % NAT;
+            const int u0 = k % NUV, u1 = (k + 1) % NUV, u2 = (k + LA + 1) % NUV;
+            int issued = 0;
+            if (k + LA + 2 < km) {
+                stage_d(T + (size_t)(k + LA + 2) * n2, sAT[s3], offd, lane);
+                stage_f(a.ws + (size_t)(k + LA + 2) * n2, sW[s3], offf, lane);
+                issued += DMA_ATW;
             }
-            if (k + 2 < km) {
-                stage_f(a.uw + (size_t)(k + 2) * n2, sU[u2], offf, lane);
-                stage_f(a.vs + (size_t)(k + 2) * n2, sV[u2], offf, lane);
-                stage_d(TB + (size_t)(k + 2) * n2, sB[u2], offd, lane);
-                stage_tile_f(A + (size_t)(k + 2) * n2, sAK[u2], offa, lane);
+            if (k + LA + 1 < km) {
+                stage_f(a.uw + (size_t)(k + LA + 1) * n2, sU[u2], offf, lane);
+                stage_f(a.vs + (size_t)(k + LA + 1) * n2, sV[u2], offf, lane);
+                stage_d(TB + (size_t)(k + LA + 1) * n2, sB[u2], offd, lane);
+                stage_tile_f(A + (size_t)(k + LA + 1) * n2, sAK[u2], offa, lane);
+                issued += DMA_UVBA;
             }
             if (k + 1 < km) {
                 LWf tmp;
@@ -469,7 +486,7 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
             }
             if (p.hmix == LICOM_HMIX_DEL4 && has_cell) sD2[pC] = cell_ok ? d2tk_mx(rC, sB[u0], sK, k, M) : 0.0f;
             bar_lds();
-            wait_vmem();
+            wait_vmem_keep(issued);
             bar_lds();
         }
         return;
@@ -500,8 +517,8 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
     const float ah = (float)p.ah;
 
     for (int k = 0; k < km; ++k) {
-        const int s0 = k & 3, s1 = (k + 1) & 3, s2 = (k + 2) & 3;
-        const int u0 = k % 3, u1 = (k + 1) % 3;
+        const int s0 = k % NAT, s1 = (k + 1) % NAT, s2 = (k + 2) % NAT;
+        const int u0 = k % NUV, u1 = (k + 1) % NUV;
         const float Acur = act ? (float)dev::floor_akt((double)sAK[u0][t], k, n, p.hmix, p.dwndmix) : 0.0f;
         if (k + 1 < km) {
             const LvM L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};

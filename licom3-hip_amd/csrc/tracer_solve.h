@@ -110,6 +110,20 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
     constexpr int NSEG = (KM + SEG - 1) / SEG;
     const int t = threadIdx.x, n = t & 1, col = t >> 1;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
+    // The per-level constants (odzp, odzt, the density coefficients, to, so)
+    // are read from LDS: as global loads after the first stores they cannot
+    // be scalar loads, and every vector load's wait would also wait for the
+    // stores issued before it (one vmcnt covers both on gfx9).
+    __shared__ double sZP[LICOM_MAX_KM], sZT[LICOM_MAX_KM], sTO[LICOM_MAX_KM], sSO[LICOM_MAX_KM];
+    __shared__ double sC[9 * LICOM_MAX_KM];
+    for (int r = t; r < km; r += 256) {
+        sZP[r] = g.odzp[r];
+        sZT[r] = g.odzt[r];
+        sTO[r] = g.to[r];
+        sSO[r] = g.so[r];
+    }
+    for (int r = t; r < 9 * km; r += 256) sC[r] = g.eosc[r];
+    __syncthreads();
     const int R = solve_rows(shape), NC = solve_cols(shape);
     const int gj = 2 + by * R + col / NC;
     // Each wave takes 32 consecutive columns of one row, starting on a
@@ -124,7 +138,7 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
     const double *O = ori + n * n3 + q;
     const AK *A = akt + n * n3 + q;
     double *TT = at + n * n3 + q, *BB = atb + n * (n3 + n2) + n2 + q;
-    const double *odzp = g.odzp, *odzt = g.odzt;
+    const double *odzp = sZP, *odzt = sZT;
     const double dts = p.dts;
     const double stf = act ? stf_n[n * n2 + q] : 0.0;
     const int hmix = p.hmix;
@@ -187,7 +201,7 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
 
     // ---- substitution from the top, tracer_9, convadj stability scan
     const bool conv = flags & LICOM_STEP_CONVADJ;
-    const double *C = g.eosc, *to = g.to, *so = g.so;
+    const double *C = sC, *to = sTO, *so = sSO;
     double xprev = 0.0, Tp = 0.0, Sp = 0.0;
     uint64_t mask = 0;
     double qd[SEG], qa[SEG], qt[SEG], qw = 0.0;
@@ -280,7 +294,7 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
     if (conv && act && mask != 0) {
         if (n == 0) atomicAdd(mix_count, 1);
         int lo, hi;
-        convect_sparse(C, to, so, g.dzp, km, kz, mask, n, TT, n2, lo, hi);
+        convect_sparse(g.eosc, g.to, g.so, g.dzp, km, kz, mask, n, TT, n2, lo, hi);
         for (int l = lo; l <= hi; ++l) {
             const size_t c = (size_t)l * n2;
             const double y = TT[c], b = BB[c];
