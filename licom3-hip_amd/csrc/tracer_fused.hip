@@ -57,12 +57,16 @@ __global__ __launch_bounds__(256) void k_prelude_col(KGrid g, KPar p, const doub
                                                      double *__restrict__ vs, XcdGrid xg) {
     constexpr int WX = 65, WN = 5 * WX;
     __shared__ double sD[2][WN], sBv[2][WN];
+    // dzp from LDS: a global load inside the level loop would carry a vmcnt
+    // wait that also waits for the level's stores and the next level's loads
+    __shared__ double sDZ[KMAXF];
     int bx, by;
     if (!xcd_block(xg, bx, by)) return;
     const int t = threadIdx.x, tx = t % 64, ty = t / 64;
     const int i0 = bx * 64, j0 = by * 4;
     const int i = i0 + tx, j = j0 + ty;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
+    for (int r = t; r < km; r += 256) sDZ[r] = g.dzp[r];
     const size_t n2 = (size_t)imt * jmt;
     const double oncc = p.oncc;
     // this lane's window elements: e = t and e = t + 256
@@ -76,18 +80,23 @@ __global__ __launch_bounds__(256) void k_prelude_col(KGrid g, KPar p, const doub
         wok[u] = wv[u] && jj >= 0 && jj < jmt && ii < imt;
         wq[u] = wok[u] ? (size_t)jj * imt + ii : 0;
     }
+    // window loads and the face-velocity stores go through plane buffers
+    // (masked lanes read 0 / drop their store without a branch)
+    uint32_t wo[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) wo[u] = buf_off(wok[u], wq[u], 8);
+    const size_t pb = n2 * sizeof(double);
     double f_ut[2], f_ul[2], f_vt[2], f_vl[2];
     auto fetch = [&](int k) {
         const size_t o = (size_t)k * n2;
+        const auto r_ut = plane_rsrc(utf + o, pb), r_ul = plane_rsrc(utl + o, pb);
+        const auto r_vt = plane_rsrc(vtf + o, pb), r_vl = plane_rsrc(vtl + o, pb);
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            f_ut[u] = f_ul[u] = f_vt[u] = f_vl[u] = 0.0;
-            if (wok[u]) {
-                f_ut[u] = utf[o + wq[u]];
-                f_ul[u] = utl[o + wq[u]];
-                f_vt[u] = vtf[o + wq[u]];
-                f_vl[u] = vtl[o + wq[u]];
-            }
+            f_ut[u] = buf_load(r_ut, wo[u], 0.0);
+            f_ul[u] = buf_load(r_ul, wo[u], 0.0);
+            f_vt[u] = buf_load(r_vt, wo[u], 0.0);
+            f_vl[u] = buf_load(r_vl, wo[u], 0.0);
         }
     };
     fetch(0);
@@ -98,8 +107,12 @@ __global__ __launch_bounds__(256) void k_prelude_col(KGrid g, KPar p, const doub
     const bool do_w = do_u && do_v;
     const double htw_c = in ? g.htw[q] : 0.0, hts_c = in ? g.hts[q] : 0.0;
     double fc = 0, fe = 0, fn = 0, fne = 0, htw_e = 0, hts_n = 0, tr = 0;
+    double ohbt = 0, stfp = 0, prev = 0;
     int kmt = 0;
     if (do_w) {
+        ohbt = g.ohbt[q];
+        stfp = stf_pre[q];
+        prev = ws[q];
         const size_t qe = q + 1, qn = q - imt, qne = q - imt + 1;
         fc = 1.0 + work_u[q] * g.ohbu[q];
         fe = 1.0 + work_u[qe] * g.ohbu[qe];
@@ -110,6 +123,10 @@ __global__ __launch_bounds__(256) void k_prelude_col(KGrid g, KPar p, const doub
         tr = g.tarea_r[q];
         kmt = g.kmt[q];
     }
+    const uint32_t ou = buf_off(do_u, q, 8), ov = buf_off(do_v, q, 8);
+    // everything loaded so far completes here, so that no wait for it stays
+    // inside the level loop
+    wait_all_vmem();
     const int w0 = (ty + 1) * WX + tx;
     double wka[KM];
     double w = 0.0;
@@ -128,9 +145,8 @@ __global__ __launch_bounds__(256) void k_prelude_col(KGrid g, KPar p, const doub
             if (k + 1 < km) fetch(k + 1);
             __syncthreads();
             const double dc = sD[b][w0], bc = sBv[b][w0];
-            const size_t c = (size_t)k * n2 + q;
-            if (do_u) uw[c] = (sD[b][w0 - WX] + dc) * htw_c * 0.25;
-            if (do_v) vs[c] = (bc + sBv[b][w0 + 1]) * hts_c * 0.25;
+            buf_store(plane_rsrc(uw + (size_t)k * n2, pb), ou, (sD[b][w0 - WX] + dc) * htw_c * 0.25);
+            buf_store(plane_rsrc(vs + (size_t)k * n2, pb), ov, (bc + sBv[b][w0 + 1]) * hts_c * 0.25);
             if (do_w && k <= kmt - 1) {
                 const double de = sD[b][w0 + 1], dn = sD[b][w0 - WX], dne = sD[b][w0 - WX + 1];
                 const double be = sBv[b][w0 + 1], bn = sBv[b][w0 - WX], bne = sBv[b][w0 - WX + 1];
@@ -140,19 +156,17 @@ __global__ __launch_bounds__(256) void k_prelude_col(KGrid g, KPar p, const doub
                                   (fe * be + fc * bc) * hts_c - (fne * bne + fn * bn) * hts_n) *
                                  tr;
                 wka[k] = d;
-                w = w - g.dzp[k] * d * 1.0;
+                w = w - sDZ[k] * d * 1.0;
             }
         }
     }
     if (!do_w) return;
     // upwell_3 / upwell_4 (tracer.cpp:1045-1079)
-    const double ohbt = g.ohbt[q];
-    const double sc = 1.0 / (1.0 + stf_pre[q] * ohbt);
-    double prev = ws[q];
+    const double sc = 1.0 / (1.0 + stfp * ohbt);
 #pragma unroll
     for (int k = 1; k < KM; ++k) {
         if (k < km) {
-            const double cur = (k < kmt ? 1.0 : 0.0) * (prev + g.dzp[k - 1] * (w * ohbt + wka[k - 1]));
+            const double cur = (k < kmt ? 1.0 : 0.0) * (prev + sDZ[k - 1] * (w * ohbt + wka[k - 1]));
             ws[(size_t)k * n2 + q] = cur * sc;
             prev = cur;
         }

@@ -117,6 +117,35 @@ __device__ __forceinline__ bool xcd_block(const XcdGrid &x, int &bx, int &by) {
     return true;
 }
 
+// Raw buffer access to one plane (up to 2 GiB).  A lane whose access is
+// masked off gets an offset past the end of the buffer: the hardware then
+// returns 0 for its load and drops its store, so no branch surrounds the
+// instruction.  Branch-free loads and stores keep the compiler's count of
+// outstanding vector-memory operations exact, so a wait for an earlier load
+// does not also wait for later stores (on gfx9 one vmcnt covers both).
+typedef int licom_v2i __attribute__((ext_vector_type(2)));
+constexpr uint32_t kBufOff = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint32_t buf_off(bool ok, size_t idx, uint32_t esize) {
+    return ok ? (uint32_t)idx * esize : kBufOff;
+}
+__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ float buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(licom_v2i, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, off, 0, 0);
+}
+// s_waitcnt vmcnt(0) that the compiler's own wait insertion knows about
+__device__ __forceinline__ void wait_all_vmem() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // RAII HIP-event bracket used when ctx->profiling is set.
 struct Timer {
     licom_ctx *ctx;

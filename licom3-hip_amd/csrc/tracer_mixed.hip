@@ -57,11 +57,15 @@ __device__ __forceinline__ void prelude_col_mx_body(const KGrid &g, float oncc,
                                                     float *__restrict__ ws, float *__restrict__ uw,
                                                     float *__restrict__ vs, int bx, int by, int shape) {
     __shared__ float sD[2][400], sBv[2][400];
+    // dzp from LDS: a global load inside the level loop would carry a vmcnt
+    // wait that also waits for the level's stores and the next level's loads
+    __shared__ double sDZ[KMAXM];
     const int FX = fd_cols(shape), FY = fd_rows(shape), WX = FX + 1, WN = WX * (FY + 1);
     const int t = threadIdx.x, tx = t % FX, ty = t / FX;
     const int i0 = bx * FX, j0 = by * FY;
     const int i = i0 + tx, j = j0 + ty;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
+    for (int r = t; r < km; r += 256) sDZ[r] = g.dzp[r];
     const size_t n2 = (size_t)imt * jmt;
     size_t wq[2];
     bool wv[2], wok[2];
@@ -73,18 +77,23 @@ __device__ __forceinline__ void prelude_col_mx_body(const KGrid &g, float oncc,
         wok[u] = wv[u] && jj >= 0 && jj < jmt && ii < imt;
         wq[u] = wok[u] ? (size_t)jj * imt + ii : 0;
     }
+    // window loads and the face-velocity stores go through plane buffers
+    // (masked lanes read 0 / drop their store without a branch)
+    uint32_t wo[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) wo[u] = buf_off(wok[u], wq[u], 4);
+    const size_t pb = n2 * sizeof(float);
     float f_ut[2], f_ul[2], f_vt[2], f_vl[2];
     auto fetch = [&](int k) {
         const size_t o = (size_t)k * n2;
+        const auto r_ut = plane_rsrc(utf + o, pb), r_ul = plane_rsrc(utl + o, pb);
+        const auto r_vt = plane_rsrc(vtf + o, pb), r_vl = plane_rsrc(vtl + o, pb);
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            f_ut[u] = f_ul[u] = f_vt[u] = f_vl[u] = 0.0f;
-            if (wok[u]) {
-                f_ut[u] = utf[o + wq[u]];
-                f_ul[u] = utl[o + wq[u]];
-                f_vt[u] = vtf[o + wq[u]];
-                f_vl[u] = vtl[o + wq[u]];
-            }
+            f_ut[u] = buf_load(r_ut, wo[u], 0.0f);
+            f_ul[u] = buf_load(r_ul, wo[u], 0.0f);
+            f_vt[u] = buf_load(r_vt, wo[u], 0.0f);
+            f_vl[u] = buf_load(r_vl, wo[u], 0.0f);
         }
     };
     fetch(0);
@@ -95,8 +104,12 @@ __device__ __forceinline__ void prelude_col_mx_body(const KGrid &g, float oncc,
     const bool do_w = do_u && do_v;
     const float htw_c = in ? (float)g.htw[q] : 0.0f, hts_c = in ? (float)g.hts[q] : 0.0f;
     float fc = 0, fe = 0, fn = 0, fne = 0, htw_e = 0, hts_n = 0, tr = 0;
+    double ohbt = 0, stfp = 0, prev = 0;
     int kmt = 0;
     if (do_w) {
+        ohbt = g.ohbt[q];
+        stfp = stf_pre[q];
+        prev = (double)ws[q];
         const size_t qe = q + 1, qn = q - imt, qne = q - imt + 1;
         fc = (float)(1.0 + work_u[q] * g.ohbu[q]);
         fe = (float)(1.0 + work_u[qe] * g.ohbu[qe]);
@@ -107,6 +120,10 @@ __device__ __forceinline__ void prelude_col_mx_body(const KGrid &g, float oncc,
         tr = (float)g.tarea_r[q];
         kmt = g.kmt[q];
     }
+    const uint32_t ou = buf_off(do_u, q, 4), ov = buf_off(do_v, q, 4);
+    // everything loaded so far completes here, so that no wait for it stays
+    // inside the level loop
+    wait_all_vmem();
     const int w0 = (ty + 1) * WX + tx;
     float wka[KM];
     double w = 0.0;
@@ -125,9 +142,8 @@ __device__ __forceinline__ void prelude_col_mx_body(const KGrid &g, float oncc,
             if (k + 1 < km) fetch(k + 1);
             __syncthreads();
             const float dc = sD[b][w0], bc = sBv[b][w0];
-            const size_t c = (size_t)k * n2 + q;
-            if (do_u) uw[c] = (sD[b][w0 - WX] + dc) * htw_c * 0.25f;
-            if (do_v) vs[c] = (bc + sBv[b][w0 + 1]) * hts_c * 0.25f;
+            buf_store(plane_rsrc(uw + (size_t)k * n2, pb), ou, (sD[b][w0 - WX] + dc) * htw_c * 0.25f);
+            buf_store(plane_rsrc(vs + (size_t)k * n2, pb), ov, (bc + sBv[b][w0 + 1]) * hts_c * 0.25f);
             if (do_w && k <= kmt - 1) {
                 const float de = sD[b][w0 + 1], dn = sD[b][w0 - WX], dne = sD[b][w0 - WX + 1];
                 const float be = sBv[b][w0 + 1], bn = sBv[b][w0 - WX], bne = sBv[b][w0 - WX + 1];
@@ -136,18 +152,16 @@ __device__ __forceinline__ void prelude_col_mx_body(const KGrid &g, float oncc,
                                  (fe * be + fc * bc) * hts_c - (fne * bne + fn * bn) * hts_n) *
                                 tr;
                 wka[k] = d;
-                w = w - g.dzp[k] * (double)d;
+                w = w - sDZ[k] * (double)d;
             }
         }
     }
     if (!do_w) return;
-    const double ohbt = g.ohbt[q];
-    const double sc = 1.0 / (1.0 + stf_pre[q] * ohbt);
-    double prev = (double)ws[q];
+    const double sc = 1.0 / (1.0 + stfp * ohbt);
 #pragma unroll
     for (int k = 1; k < KM; ++k) {
         if (k < km) {
-            const double cur = (k < kmt ? 1.0 : 0.0) * (prev + g.dzp[k - 1] * (w * ohbt + (double)wka[k - 1]));
+            const double cur = (k < kmt ? 1.0 : 0.0) * (prev + sDZ[k - 1] * (w * ohbt + (double)wka[k - 1]));
             ws[(size_t)k * n2 + q] = (float)(cur * sc);
             prev = cur;
         }

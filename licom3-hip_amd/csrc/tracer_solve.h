@@ -10,7 +10,10 @@ namespace licom {
 namespace {
 
 // ---------------------------------------------------------------- solve
-// Two lanes per column: lane parity = tracer; 32 columns along i per wave.
+// Two lanes per column, 32 columns along i per wave: lanes 0-31 hold T and
+// lanes 32-63 S of the same 32 columns, so each quarter of a wave reads or
+// writes 16 consecutive values of one tracer (a whole 128-byte line of an
+// FP64 array) rather than 8 of each.
 //
 // invtrit (tracer.cpp:923-978) eliminates from the bottom and substitutes
 // from the top.  Holding e(k), f(k) for the whole column would take 2*km
@@ -27,7 +30,7 @@ constexpr int SEG = 4;
 // touched, so the mask plus on-demand recomputation reproduces its decisions.
 // The two lanes of a column run this together: each holds its own tracer's
 // column X (the state just written, stride n2) and they swap values with
-// __shfl_xor.  Returns the mixed level range in lo/hi.
+// __shfl_xor over 32 lanes.  Returns the mixed level range in lo/hi.
 __device__ __noinline__ void convect_sparse(const double *C, const double *to, const double *so,
                                             const double *dzp, int km, int kcon, uint64_t mask,
                                             int n, double *X, size_t n2, int &lo, int &hi) {
@@ -42,7 +45,7 @@ __device__ __noinline__ void convect_sparse(const double *C, const double *to, c
         for (;;) {   // CONV_2
             if (lconb != kcon) {
                 const double v = X[(size_t)lconb * n2];
-                const double vo = __shfl_xor(v, 1), mo = __shfl_xor(tramix, 1);
+                const double vo = __shfl_xor(v, 32), mo = __shfl_xor(tramix, 32);
                 const double Tm = n == 0 ? tramix : mo, Sm = n == 0 ? mo : tramix;
                 const double Tv = n == 0 ? v : vo, Sv = n == 0 ? vo : v;
                 const double rl = dev::dens(C, km, lconb, Tm - to[lconb], Sm - so[lconb]);
@@ -57,7 +60,7 @@ __device__ __noinline__ void convect_sparse(const double *C, const double *to, c
             }
             if (lcona > 1) {
                 const double v = X[(size_t)(lcona - 2) * n2];
-                const double vo = __shfl_xor(v, 1), mo = __shfl_xor(tramix, 1);
+                const double vo = __shfl_xor(v, 32), mo = __shfl_xor(tramix, 32);
                 const double Tm = n == 0 ? tramix : mo, Sm = n == 0 ? mo : tramix;
                 const double Tv = n == 0 ? v : vo, Sv = n == 0 ? vo : v;
                 const double rl = dev::dens(C, km, lcona - 1, Tv - to[lcona - 1], Sv - so[lcona - 1]);
@@ -108,7 +111,7 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
                                            DG *__restrict__ dt_conv, int *mix_count, int bx,
                                            int by, int shape) {
     constexpr int NSEG = (KM + SEG - 1) / SEG;
-    const int t = threadIdx.x, n = t & 1, col = t >> 1;
+    const int t = threadIdx.x, n = (t >> 5) & 1, col = (t >> 6) * 32 + (t & 31);
     const int imt = g.imt, jmt = g.jmt, km = g.km;
     // The per-level constants (odzp, odzt, the density coefficients, to, so)
     // are read from LDS: as global loads after the first stores they cannot
@@ -135,14 +138,35 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
     const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
     const size_t q = act ? (size_t)gj * imt + gi : 0;
     const int kz = act ? g.kmt[q] : 0;
-    const double *O = ori + n * n3 + q;
-    const AK *A = akt + n * n3 + q;
     double *TT = at + n * n3 + q, *BB = atb + n * (n3 + n2) + n2 + q;
     const double *odzp = sZP, *odzt = sZT;
     const double dts = p.dts;
     const double stf = act ? stf_n[n * n2 + q] : 0.0;
     const int hmix = p.hmix;
     const double dwm = p.dwndmix;
+    // With FP32 diagnostics (kBuf) every 3-D access of the substitution goes
+    // through a buffer over one plane of this tracer: masked lanes (land,
+    // outside the tile, beyond km) get an out-of-range offset and load 0 /
+    // drop their store with no branch, so the waits for a prefetched segment
+    // never include the stores that followed it; a NULL diagnostic gets an
+    // empty buffer.  The FP64 form moves enough bytes to be bound by HBM
+    // bandwidth and runs faster with plain conditional accesses.
+    constexpr bool kBuf = sizeof(DG) == 4;
+    const double *Ob = ori + n * n3;
+    const AK *Ab = akt + n * n3;
+    double *Tb = at + n * n3, *Bb = atb + n * (n3 + n2) + n2;
+    const size_t pb8 = n2 * sizeof(double), pba = n2 * sizeof(AK), pbg = n2 * sizeof(DG);
+    auto plane = [&](const void *base, size_t esz, int k, size_t bytes) {
+        const int kk = k < km ? k : km - 1;
+        return plane_rsrc(base ? (const char *)base + (size_t)kk * n2 * esz : nullptr, base ? bytes : 0);
+    };
+    const uint32_t q8 = (uint32_t)q * 8u, qak = (uint32_t)q * (uint32_t)sizeof(AK),
+                   qg = (uint32_t)q * (uint32_t)sizeof(DG);
+    auto ld_o = [&](int k, bool ok) { return buf_load(plane(Ob, 8, k, pb8), ok ? q8 : kBufOff, 0.0); };
+    auto ld_t = [&](int k, bool ok) { return buf_load(plane(Tb, 8, k, pb8), ok ? q8 : kBufOff, 0.0); };
+    auto ld_a = [&](int k, bool ok) {
+        return (double)buf_load(plane(Ab, sizeof(AK), k, pba), ok ? qak : kBufOff, AK(0));
+    };
 
     // ---- elimination from the bottom; checkpoints e,f at levels SEG*m (m >= 1)
     double cke[NSEG + 1], ckf[NSEG + 1];
@@ -155,8 +179,8 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
             const int k = SEG * (NSEG - 1) + r;
             pd[r] = pa[r] = 0.0;
             if (k < KM && k < kz) {
-                pd[r] = O[(size_t)k * n2];
-                pa[r] = k >= 1 ? (double)A[(size_t)(k - 1) * n2] : 0.0;
+                pd[r] = Ob[(size_t)k * n2 + q];
+                pa[r] = k >= 1 ? (double)Ab[(size_t)(k - 1) * n2 + q] : 0.0;
             }
         }
         double en = 0.0, fn = 0.0, wk_k = 0.0;
@@ -174,8 +198,8 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
                     const int k = SEG * (m - 1) + r;
                     pd[r] = pa[r] = 0.0;
                     if (k < kz) {
-                        pd[r] = O[(size_t)k * n2];
-                        pa[r] = k >= 1 ? (double)A[(size_t)(k - 1) * n2] : 0.0;
+                        pd[r] = Ob[(size_t)k * n2 + q];
+                        pa[r] = k >= 1 ? (double)Ab[(size_t)(k - 1) * n2 + q] : 0.0;
                     }
                 }
             }
@@ -209,15 +233,23 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
 #pragma unroll
         for (int r = 0; r < SEG; ++r) {
             const int k = SEG * m + r;
-            qd[r] = qa[r] = qt[r] = 0.0;
-            if (k < KM && k < km && act) {
-                qd[r] = O[(size_t)k * n2];
-                qt[r] = TT[(size_t)k * n2];
-                if (k >= 1 && k < kz) qa[r] = (double)A[(size_t)(k - 1) * n2];
+            if constexpr (kBuf) {
+                const bool ok = k < km && act;
+                qd[r] = k < KM ? ld_o(k, ok) : 0.0;
+                qt[r] = k < KM ? ld_t(k, ok) : 0.0;
+                qa[r] = k < KM && k >= 1 ? ld_a(k - 1, k < kz) : 0.0;
+            } else {
+                qd[r] = qa[r] = qt[r] = 0.0;
+                if (k < KM && k < km && act) {
+                    qd[r] = Ob[(size_t)k * n2 + q];
+                    qt[r] = Tb[(size_t)k * n2 + q];
+                    if (k >= 1 && k < kz) qa[r] = (double)Ab[(size_t)(k - 1) * n2 + q];
+                }
             }
         }
         const int kb = SEG * m + SEG - 1;
-        qw = (kb < KM && kb < kz - 1) ? (double)A[(size_t)kb * n2] : 0.0;
+        if constexpr (kBuf) qw = kb < KM ? ld_a(kb, kb < kz - 1) : 0.0;
+        else qw = (kb < KM && kb < kz - 1) ? (double)Ab[(size_t)kb * n2 + q] : 0.0;
     };
     load_seg(0);
 #pragma unroll
@@ -256,27 +288,36 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
 #pragma unroll
         for (int r = 0; r < SEG; ++r) {
             const int k = top + r;
-            if (k < KM && k < km) {
-                const size_t c = (size_t)k * n2;
+            // levels k >= km (KM above km) store nothing and never count as wet
+            if (k < KM) {
                 const bool wet = k < kz;
                 const double o = cd[r];
                 double x = o;
                 if (wet) x = (k == 0) ? (se[0] * stf + sf[0]) * 1.0 : (se[r] * xprev + sf[r]) * 1.0;
                 xprev = x;
                 const double vk = wet ? 1.0 : 0.0;
-                if (act) {
-                    if (dt_diff) {
-                        double v = (x - o) / dts * vk;
-                        if (k == 0) v = v - stf * 0.5 * odzp[0];
-                        dt_diff[n * n3 + q + c] = (DG)v;
-                    }
-                    if (tend) tend[n * n3 + q + c] = (DG)((x - ct[r]) / dts * vk);
-                    if (conv && dt_conv) dt_conv[n * n3 + q + c] = (DG)0.0;
-                    TT[c] = x;
-                    BB[c] = x;
+                double v = (x - o) / dts * vk;
+                if (k == 0) v = v - stf * 0.5 * odzp[0];
+                const double tn = (x - ct[r]) / dts * vk;
+                if constexpr (kBuf) {
+                    const bool st = act && k < km;
+                    const uint32_t og = st ? qg : kBufOff, o8 = st ? q8 : kBufOff;
+                    buf_store(plane(dt_diff ? dt_diff + n * n3 : nullptr, sizeof(DG), k, pbg), og, (DG)v);
+                    buf_store(plane(tend ? tend + n * n3 : nullptr, sizeof(DG), k, pbg), og, (DG)tn);
+                    buf_store(plane(conv && dt_conv ? dt_conv + n * n3 : nullptr, sizeof(DG), k, pbg), og,
+                              (DG)0.0);
+                    buf_store(plane(Tb, 8, k, pb8), o8, x);
+                    buf_store(plane(Bb, 8, k, pb8), o8, x);
+                } else if (act && k < km) {
+                    const size_t c = n * n3 + (size_t)k * n2 + q;
+                    if (dt_diff) dt_diff[c] = (DG)v;
+                    if (tend) tend[c] = (DG)tn;
+                    if (conv && dt_conv) dt_conv[c] = (DG)0.0;
+                    Tb[(size_t)k * n2 + q] = x;
+                    Bb[(size_t)k * n2 + q] = x;
                 }
                 if (conv) {
-                    const double xo = __shfl_xor(x, 1);
+                    const double xo = __shfl_xor(x, 32);
                     const double Tk = n == 0 ? x : xo, Sk = n == 0 ? xo : x;
                     if (k >= 1 && k <= kz - 1) {
                         const double up = dev::dens(C, km, k, Tk - to[k], Sk - so[k]);
@@ -311,7 +352,7 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
 // In-place side effects of the reference on its inputs
 // (tracer.cpp:1088, 1107-1108, 1187, 1315).
 template <int KM, typename AK, typename DG>
-__global__ __launch_bounds__(256) void k_solve(KGrid g, KPar p, int flags, double *__restrict__ at,
+__global__ __launch_bounds__(256, 3) void k_solve(KGrid g, KPar p, int flags, double *__restrict__ at,
                                                double *__restrict__ atb,
                                                const AK *__restrict__ akt,
                                                const double *__restrict__ ori,
