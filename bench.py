@@ -296,20 +296,68 @@ def run_ensemble(args, torch, lt, le):
             "isolated_step_ms": {f"{a}x{b}x{c}": round(1000 * t, 4) for (a, b, c), t in iso.items()}}, iso
 
 
-def run_serving(args, torch, lt, le, offline_value, iso):
-    """Request loop: Poisson arrivals at a fixed fraction of the offline
-    batched capacity, HBM admission under the cap, continuous batching."""
+def mixed_error_study(lt, ls, steps=100):
+    """Error of the mixed form against the FP64 drop-in on the 1 deg grid
+    (360x218x30, del2, dt=3600 s) after `steps` steps, beside the FP64 drop-in's
+    own divergence from an initial T perturbed by N(0, 1e-8 K) on wet cells:
+    the tspas limiter and convadj make discrete decisions at near-ties, so a
+    perturbation of one mixed step's size flips some of them."""
+    case = ls.make_case("C2")
+    pcase = ls.make_case("C2")
+    rng = np.random.default_rng(99)
+    kmt = case.grid["kmt"]
+    wet3 = np.arange(case.km)[:, None, None] < kmt[None]
+    noise = 1e-8 * rng.standard_normal(pcase.state["at"][0].shape) * wet3
+    pcase.state["at"][0] += noise
+    pcase.state["atb"][0, 1:] += noise
+    flags = lt.LICOM_STEP_SIDE_EFFECTS | lt.LICOM_STEP_CONVADJ
+    models = [lt.TracerModel(case), lt.TracerModel(case, precision="mixed"), lt.TracerModel(pcase)]
+    for st in range(steps):
+        for m in models:
+            m.tracer(st, flags)
+    a_ref, a_mix, a_ctl = [m.host_state()["at"] for m in models]
+    for m in models:
+        m.close()
+    wet = wet3[:, 2:-2, 2:-2]
+
+    def err(a):
+        dT = np.abs(a[0] - a_ref[0])[:, 2:-2, 2:-2][wet]
+        dS = np.abs(a[1] - a_ref[1])[:, 2:-2, 2:-2][wet] * 1000.0
+        return {"max_dT_K": float(dT.max()), "rms_dT_K": float(np.sqrt(np.mean(dT * dT))),
+                "max_dS_psu": float(dS.max()), "rms_dS_psu": float(np.sqrt(np.mean(dS * dS))),
+                "wet_fraction_dT_above_1e-4": float(np.mean(dT > 1e-4))}
+
+    dz = np.abs(np.diff(case.grid["zkp"]))[:case.km]
+    w = dz[:, None, None] * case.grid["tarea"][None, 2:-2, 2:-2] * wet
+
+    def heat(a):
+        return float(np.sum(w * a[0][:, 2:-2, 2:-2]))
+
+    h = heat(a_ref)
+    return {"grid": f"{case.nx}x{case.ny}x{case.km}", "steps": steps,
+            "mixed_vs_fp64": dict(err(a_mix), heat_drift_rel=abs(heat(a_mix) - h) / abs(h)),
+            "fp64_perturbed_1e-8K_vs_fp64": dict(err(a_ctl), heat_drift_rel=abs(heat(a_ctl) - h) / abs(h))}
+
+
+def run_serving(args, torch, lt, le, offline_value, iso, load):
+    """Request loop: Poisson arrivals at a fixed fraction `load` of the
+    offline batched capacity, HBM admission under the cap, continuous
+    batching.  Slowdown of a request = its latency / (its steps x the
+    isolated step time of its member shape)."""
     mean_updates = np.mean([st * nx * ny * km * 2 for nx, ny, km, st in le.request_specs(args.requests)])
-    rate = args.load * offline_value / mean_updates
+    rate = load * offline_value / mean_updates
     reqs = le.poisson_requests(args.requests, rate)
-    print(f"[bench] serving: {len(reqs)} requests at {rate:.2f}/s", file=sys.stderr, flush=True)
-    loop = le.RequestLoop(hbm_cap_bytes=args.hbm_cap_gb * 1e9, chunk_steps=24)
+    print(f"[bench] serving: {len(reqs)} requests at {rate:.2f}/s (load {load})", file=sys.stderr,
+          flush=True)
+    loop = le.RequestLoop(hbm_cap_bytes=args.hbm_cap_gb * 1e9, chunk_steps=args.chunk_steps)
     stats = loop.run(reqs)
     slow = np.array([(r.finished - r.arrival) / (r.n_steps * iso.get((r.case.nx, r.case.ny, r.case.km),
                                                                       np.nan)) for r in reqs])
     slow = slow[np.isfinite(slow)]
-    stats.update({"arrival_rate_per_s": rate, "load_fraction": args.load,
-                  "hbm_cap_gb": args.hbm_cap_gb, "chunk_steps": 24,
+    last_arrival = max(r.arrival for r in reqs)
+    stats.update({"arrival_rate_per_s": rate, "load_fraction": load,
+                  "hbm_cap_gb": args.hbm_cap_gb,
+                  "drain_after_last_arrival_s": stats["makespan_s"] - last_arrival,
                   "vs_offline": stats["cell_updates_per_s"] / offline_value,
                   "p50_slowdown_vs_isolated": float(np.percentile(slow, 50)) if slow.size else None,
                   "p99_slowdown_vs_isolated": float(np.percentile(slow, 99)) if slow.size else None})
@@ -330,11 +378,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-error-study", action="store_true")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--ensemble-replays", type=int, default=3)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=64)
-    ap.add_argument("--load", type=float, default=0.8)
+    ap.add_argument("--loads", default="0.25,0.8",
+                    help="request arrival rates as fractions of the offline batched capacity")
+    ap.add_argument("--chunk-steps", type=int, default=8)
     ap.add_argument("--hbm-cap-gb", type=float, default=200.0)
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the HBM traffic summary from two rocprofv3 --pmc runs")
@@ -398,12 +449,18 @@ def main():
     del models
     torch.cuda.empty_cache()
 
+    err_study = None
+    if "mixed" in variants and not args.no_error_study:
+        print("[bench] mixed-form error study (360x218x30, 100 steps)", file=sys.stderr, flush=True)
+        err_study = mixed_error_study(lt, ls)
+
     ens_out, serve_out = None, None
     if not args.no_ensemble:
         import licom_ensemble as le
         ens_out, iso = run_ensemble(args, torch, lt, le)
         if not args.no_serving:
-            serve_out = run_serving(args, torch, lt, le, ens_out["value"], iso)
+            serve_out = [run_serving(args, torch, lt, le, ens_out["value"], iso, float(x))
+                         for x in args.loads.split(",")]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -438,7 +495,7 @@ def main():
                         "speedup_vs_fp64": rm["value"] / r["value"],
                         "roofline": rm["roofline"], "step_roofline": rm["step_roofline"],
                         "kernel_ms": rm["kernel_ms"], "kernel_alg_gbs": rm["kernel_alg_gbs"],
-                        "error": mixed_err}
+                        "error": mixed_err, "error_100_steps_1deg": err_study}
     if ens_out:
         out["ensemble"] = ens_out
     if serve_out:

@@ -217,12 +217,14 @@ class RequestLoop:
     """Continuous batching of tracer-advance requests on one GPU.
 
     Each tick: admit every arrived request whose HBM estimate fits under the
-    cap (arrival order), upload its state, rebuild the batch over the active
-    members, advance all of them by the chunk min(chunk_steps, fewest
-    remaining steps), then retire finished members (their results are copied
-    out and their memory freed).  Latency = completion - arrival."""
+    cap (arrival order) and upload its state; when the set of active members
+    changed, rebuild the batch over them (otherwise the batch is reused);
+    advance all of them by min(chunk_steps, fewest remaining steps); retire
+    finished members (their results are copied out and their memory freed).
+    Short chunks bound how long an arrival waits for admission.  Latency =
+    completion - arrival."""
 
-    def __init__(self, hbm_cap_bytes=200e9, chunk_steps=24, tuner=None, keep_results=False):
+    def __init__(self, hbm_cap_bytes=200e9, chunk_steps=8, tuner=None, keep_results=False):
         self.cap = float(hbm_cap_bytes)
         self.chunk = int(chunk_steps)
         self.tuner = tuner
@@ -238,6 +240,10 @@ class RequestLoop:
         updates = 0
         i = 0
         peak_active = 0
+        ens = None
+        changed = False
+        stream = None
+        rebuilds = 0
         while i < len(pending) or active:
             now = clock() - t0
             while i < len(pending) and pending[i].arrival <= now and used + pending[i].bytes <= self.cap:
@@ -247,20 +253,26 @@ class RequestLoop:
                 r.admitted = clock() - t0
                 used += r.bytes
                 active.append(r)
+                changed = True
             if not active:
                 # idle: wait for the next arrival
                 wait = pending[i].arrival - (clock() - t0)
                 if wait > 0:
-                    time.sleep(min(wait, 0.05))
+                    time.sleep(min(wait, 0.005))
                 continue
             peak_active = max(peak_active, len(active))
-            tuning = [self.tuner.tune(r.case) for r in active] if self.tuner else None
-            ens = Ensemble([r.model for r in active], tuning=tuning)
+            if changed or ens is None:
+                if ens is not None:
+                    ens.close()
+                if stream is None:
+                    stream = torch.cuda.Stream(device=active[0].model.device)
+                tuning = [self.tuner.tune(r.case) for r in active] if self.tuner else None
+                ens = Ensemble([r.model for r in active], tuning=tuning, stream=stream)
+                changed = False
+                rebuilds += 1
             c = min(self.chunk, min(r.n_steps - r.done_steps for r in active))
             ens.step(c)
             ens.sync()
-            ens.close()
-            torch.cuda.synchronize()
             updates += c * sum(r.case.nx * r.case.ny * r.case.km * 2 for r in active)
             t_now = clock() - t0
             still = []
@@ -268,22 +280,33 @@ class RequestLoop:
                 r.done_steps += c
                 if r.done_steps >= r.n_steps:
                     r.finished = t_now
-                    if self.keep_results:
-                        self.results[r.rid] = r.model.host_state()["at"]
-                    r.model.close()
-                    r.model = None
-                    used -= r.bytes
+                    changed = True
                     done.append(r)
                 else:
                     still.append(r)
+            if changed:
+                # the batch holds the retiring members' arrays: close it first
+                ens.close()
+                ens = None
+                for r in active:
+                    if r.finished is not None and r.model is not None:
+                        if self.keep_results:
+                            self.results[r.rid] = r.model.host_state()["at"]
+                        r.model.close()
+                        r.model = None
+                        used -= r.bytes
             active = still
+        if ens is not None:
+            ens.close()
+        torch.cuda.synchronize()
         makespan = clock() - t0
         lat = np.array([r.finished - r.arrival for r in done])
         return {"requests": len(done), "makespan_s": makespan, "cell_updates": updates,
                 "cell_updates_per_s": updates / makespan if makespan > 0 else 0.0,
                 "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
                 "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
-                "peak_active": peak_active}
+                "peak_active": peak_active, "batch_rebuilds": rebuilds,
+                "chunk_steps": self.chunk}
 
 
 def request_specs(n, seed_requests=5002):
