@@ -366,13 +366,13 @@ __device__ __forceinline__ void bar_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n
 // and the del4 second pass read around the tile, and stages every plane
 // (a four-wave workgroup keeps three workgroups resident per CU).
 // Per level k two phases, each closed by a barrier:
-//   P(k): wave 3 first issues the direct-to-LDS loads of the planes P(k+1)
-//         and C(k+1) need; then the tspas predictor of level k+1 and the
-//         del4 first pass of level k at every cell (tile and ring, one cell
-//         per lane).
+//   P(k): wave 3 first issues the direct-to-LDS loads of the planes first
+//         read LA levels later; then the tspas predictor of level k+1 and
+//         the del4 first pass of level k at every cell (tile and ring, one
+//         cell per lane).
 //   C(k): limiter, fluxes and explicit update of level k on the tile (waves
-//         0-2); wave 3 waits for its loads.
-// LDS slots rotate: at x4, u_wface/v_sface x3, ws x4, atb x3.
+//         0-2); wave 3 waits for the loads P(k+1) and C(k+1) read.
+// LDS slots rotate: at and ws x(LA+3), u_wface/v_sface, atb and akt x(LA+2).
 __global__ __launch_bounds__(NT, (LA >= 2 ? 2 : 3)) void k_march(KGrid g, KPar p, MarchArgs a) {
     __shared__ __align__(16) double sAT[NAT][NR];
     __shared__ __align__(16) double sU[NUV][NR];
