@@ -237,6 +237,7 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
     c->jmt = G->ny + 4;
     c->n2 = (size_t)c->imt * c->jmt;
     c->n3 = c->n2 * c->km;
+    c->solve_shape = c->nx >= 512 ? 2 : 0;   // one 128-column row per workgroup on wide grids
     c->p = *P;
     c->device = dev;
     int rc = derive(c, G);

@@ -55,6 +55,12 @@ struct licom_ctx {
     licom_params p;
     int device = 0;
     hipStream_t stream = nullptr;
+    // launch shape of the single-model column solve (solve_rows/solve_cols):
+    // wide grids take one row of 128 columns per workgroup, so that the
+    // lines a row segment shares with its neighbours (the S half of each
+    // 3-D array sits 64 bytes off the T half at odd n3/8) are mostly written
+    // and read by one workgroup
+    int solve_shape = 0;
     licom::KGrid g{};
     licom::KPar kp{};
     double area_t = 0.0;

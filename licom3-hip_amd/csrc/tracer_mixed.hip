@@ -716,10 +716,10 @@ void launch_prelude_solve_mx(licom_ctx *ctx, const licom_fields_mixed *f, const 
                                                          s.work_u, s.stf_pre, f->ws, uw, vs, 0, xg);
     } else {
         Timer t(ctx, "k_solve");
-        const dim3 gr = solve_grid(g.imt, g.jmt, 0);
+        const dim3 gr = solve_grid(g.imt, g.jmt, ctx->solve_shape);
         k_solve<KM, float, float><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
                                                   d ? d->tend : nullptr, d ? d->dt_diff : nullptr,
-                                                  d ? d->dt_conv : nullptr, s.mix_count, 0);
+                                                  d ? d->dt_conv : nullptr, s.mix_count, ctx->solve_shape);
     }
 }
 
