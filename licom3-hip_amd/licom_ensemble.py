@@ -213,38 +213,73 @@ class Request:
         self.bytes = member_bytes(case.nx, case.ny, case.km)
 
 
+class _Lane:
+    """One size class of the request loop: its active members, their batch
+    and a stream of its own, so that classes advance independently."""
+
+    def __init__(self, stream):
+        self.stream = stream
+        self.active = []      # admitted members of this class
+        self.running = []     # the members of the chunk in flight
+        self.ens = None
+        self.changed = False
+        self.busy = False
+        self.chunk = 0
+        self.event = None
+
+
 class RequestLoop:
     """Continuous batching of tracer-advance requests on one GPU.
 
-    Each tick: admit every arrived request whose HBM estimate fits under the
-    cap (arrival order) and upload its state; when the set of active members
-    changed, rebuild the batch over them (otherwise the batch is reused);
-    advance all of them by min(chunk_steps, fewest remaining steps); retire
-    finished members (their results are copied out and their memory freed).
-    Short chunks bound how long an arrival waits for admission.  Latency =
-    completion - arrival."""
+    Requests are sorted into size classes by their cells per step
+    (`class_cells` bounds); each class is a batch on a stream of its own, so a
+    small member never waits for the steps of a large one, and the classes'
+    kernels share the GPU.  The host loop polls: it admits every arrived
+    request whose HBM estimate fits under the cap (arrival order) and uploads
+    its state; for each class whose last chunk finished it retires finished
+    members (results copied out), rebuilds the batch when the membership
+    changed, and launches the next chunk of min(chunk_steps, fewest remaining
+    steps).  Batches and member contexts of finished requests are freed when no
+    class is running (a free synchronises the device) or when they hold more
+    than a quarter of the cap.  Latency = completion - arrival."""
 
-    def __init__(self, hbm_cap_bytes=200e9, chunk_steps=8, tuner=None, keep_results=False):
+    def __init__(self, hbm_cap_bytes=200e9, chunk_steps=8, tuner=None, keep_results=False,
+                 class_cells=(5.0e5, 2.0e6)):
         self.cap = float(hbm_cap_bytes)
         self.chunk = int(chunk_steps)
         self.tuner = tuner
         self.keep_results = keep_results
+        self.class_cells = tuple(class_cells)
         self.results = {}
+
+    def _class(self, r):
+        cells = r.case.nx * r.case.ny * r.case.km
+        for i, b in enumerate(self.class_cells):
+            if cells <= b:
+                return i
+        return len(self.class_cells)
 
     def run(self, requests, clock=time.perf_counter):
         import torch
         pending = sorted(requests, key=lambda r: r.arrival)
-        active, done = [], []
+        done = []
         used = 0.0
         t0 = clock()
         updates = 0
         i = 0
         peak_active = 0
-        ens = None
-        changed = False
-        stream = None
         rebuilds = 0
-        while i < len(pending) or active:
+        lanes = None
+        grave, grave_bytes = [], 0.0
+
+        def flush():
+            nonlocal grave, grave_bytes, used
+            for obj, nbytes in grave:
+                obj.close()
+                used -= nbytes
+            grave, grave_bytes = [], 0.0
+
+        while True:
             now = clock() - t0
             while i < len(pending) and pending[i].arrival <= now and used + pending[i].bytes <= self.cap:
                 r = pending[i]
@@ -252,53 +287,77 @@ class RequestLoop:
                 r.model = lt.TracerModel(r.case, diagnostics=True, precision="mixed")
                 r.admitted = clock() - t0
                 used += r.bytes
-                active.append(r)
-                changed = True
-            if not active:
-                # idle: wait for the next arrival
+                if lanes is None:
+                    lanes = [_Lane(torch.cuda.Stream(device=r.model.device))
+                             for _ in range(len(self.class_cells) + 1)]
+                ln = lanes[self._class(r)]
+                ln.active.append(r)
+                ln.changed = True
+            if lanes is None:
+                if i >= len(pending):
+                    break
+                time.sleep(min(max(pending[i].arrival - (clock() - t0), 0.0), 0.005))
+                continue
+            launched = False
+            for ln in lanes:
+                if ln.busy:
+                    if not ln.event.query():
+                        continue
+                    ln.busy = False
+                    t_now = clock() - t0
+                    updates += ln.chunk * sum(r.case.nx * r.case.ny * r.case.km * 2 for r in ln.running)
+                    finished = []
+                    for r in ln.running:
+                        r.done_steps += ln.chunk
+                        if r.done_steps >= r.n_steps:
+                            finished.append(r)
+                    for r in finished:
+                        r.finished = t_now
+                        done.append(r)
+                        if self.keep_results:
+                            self.results[r.rid] = r.model.host_state()["at"]
+                    if finished:
+                        # the batch holds the finished members' arrays: retire
+                        # it with them
+                        grave.append((ln.ens, 0.0))
+                        ln.ens = None
+                        for r in finished:
+                            grave.append((r.model, r.bytes))
+                            grave_bytes += r.bytes
+                            r.model = None
+                        ln.changed = True
+                    ln.active = [r for r in ln.active if r.finished is None]
+                    ln.running = []
+                if not ln.busy and ln.active:
+                    if ln.changed or ln.ens is None:
+                        if ln.ens is not None:
+                            grave.append((ln.ens, 0.0))
+                        tuning = [self.tuner.tune(r.case) for r in ln.active] if self.tuner else None
+                        ln.ens = Ensemble([r.model for r in ln.active], tuning=tuning, stream=ln.stream)
+                        ln.changed = False
+                        rebuilds += 1
+                    ln.running = list(ln.active)
+                    ln.chunk = min(self.chunk, min(r.n_steps - r.done_steps for r in ln.running))
+                    ln.ens.step(ln.chunk)
+                    ln.event = torch.cuda.Event()
+                    ln.event.record(ln.stream)
+                    ln.busy = True
+                    launched = True
+            peak_active = max(peak_active, sum(len(ln.active) for ln in lanes))
+            idle = not any(ln.busy for ln in lanes)
+            if grave and (idle or grave_bytes > 0.25 * self.cap):
+                torch.cuda.synchronize()
+                flush()
+            if idle and not any(ln.active for ln in lanes):
+                if i >= len(pending):
+                    break
                 wait = pending[i].arrival - (clock() - t0)
                 if wait > 0:
                     time.sleep(min(wait, 0.005))
-                continue
-            peak_active = max(peak_active, len(active))
-            if changed or ens is None:
-                if ens is not None:
-                    ens.close()
-                if stream is None:
-                    stream = torch.cuda.Stream(device=active[0].model.device)
-                tuning = [self.tuner.tune(r.case) for r in active] if self.tuner else None
-                ens = Ensemble([r.model for r in active], tuning=tuning, stream=stream)
-                changed = False
-                rebuilds += 1
-            c = min(self.chunk, min(r.n_steps - r.done_steps for r in active))
-            ens.step(c)
-            ens.sync()
-            updates += c * sum(r.case.nx * r.case.ny * r.case.km * 2 for r in active)
-            t_now = clock() - t0
-            still = []
-            for r in active:
-                r.done_steps += c
-                if r.done_steps >= r.n_steps:
-                    r.finished = t_now
-                    changed = True
-                    done.append(r)
-                else:
-                    still.append(r)
-            if changed:
-                # the batch holds the retiring members' arrays: close it first
-                ens.close()
-                ens = None
-                for r in active:
-                    if r.finished is not None and r.model is not None:
-                        if self.keep_results:
-                            self.results[r.rid] = r.model.host_state()["at"]
-                        r.model.close()
-                        r.model = None
-                        used -= r.bytes
-            active = still
-        if ens is not None:
-            ens.close()
+            elif not launched:
+                time.sleep(5e-5)
         torch.cuda.synchronize()
+        flush()
         makespan = clock() - t0
         lat = np.array([r.finished - r.arrival for r in done])
         return {"requests": len(done), "makespan_s": makespan, "cell_updates": updates,
@@ -306,7 +365,7 @@ class RequestLoop:
                 "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
                 "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
                 "peak_active": peak_active, "batch_rebuilds": rebuilds,
-                "chunk_steps": self.chunk}
+                "chunk_steps": self.chunk, "class_cells": list(self.class_cells)}
 
 
 def request_specs(n, seed_requests=5002):
