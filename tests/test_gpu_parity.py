@@ -153,3 +153,53 @@ def test_derived_masks_and_coefficients_bitwise(need_gpu):
             assert np.array_equal(d[k], o.d[k]), k
         assert d["area_t"] == o.area_t
         m.close()
+
+
+def test_wide_grid_bitwise(need_gpu):
+    """520 columns: the solve takes one 128-column row per workgroup (the
+    launch used for the 0.1 deg grid) and the k-march tiles wrap 33 times
+    along i; two steps with del4, bit for bit."""
+    announce("two steps 520x24x12 del4 (wide-grid launch shapes)")
+    case = ls.make_case("C1", nx=520, ny=24, km=12, seed=21, hmix=4, ah=-1.0e10)
+    case.name = "C1-wide"
+    run_pair(case, 2)
+
+
+def test_full_size_content_and_masks(need_gpu):
+    """At the 0.1 deg workload size (3600x2302x55, del4, dt=600 s, no surface
+    forcing): kmt as exported is the generator's bit for bit, and the T and S
+    contents sum(dzp*tarea*vit*at) are conserved over two steps with convadj
+    to 1e-11 relative (FP64 roundoff of 455.8 M cells; the oracle's own pin
+    of the same property holds to 1e-13 with exact summation)."""
+    import torch
+    announce("full size 3600x2302x55: content conservation over 2 steps")
+    case = ls.make_case("C3")
+    kmt = case.grid["kmt"]
+    tarea = case.grid["tarea"]
+    m = lt.TracerModel(case, diagnostics=False)
+    case.state = None
+    d = m.derived()
+    assert np.array_equal(d["kmt"], kmt)
+    dev = m.state["at"].device
+    km = case.km
+    vit = torch.arange(km, device=dev)[:, None, None] < torch.from_numpy(kmt).to(dev)[None]
+    w = (torch.from_numpy(d["dzp"]).to(dev)[:, None, None] * torch.from_numpy(tarea).to(dev)[None]
+         * vit)[:, 2:-2, 2:-2]
+    del vit
+
+    def content():
+        m.sync()
+        return [float((w * m.state["at"][n, :, 2:-2, 2:-2]).sum()) for n in range(2)]
+
+    before = content()
+    flags = lt.LICOM_STEP_SIDE_EFFECTS | lt.LICOM_STEP_CONVADJ
+    for s in range(2):
+        m.tracer(s, flags)
+    after = content()
+    rel = [abs(a - b) / abs(b) for a, b in zip(after, before)]
+    print(f"[full size] T content {before[0]:.6e} -> {after[0]:.6e} (rel {rel[0]:.2e}), "
+          f"S content {before[1]:.6e} -> {after[1]:.6e} (rel {rel[1]:.2e})", flush=True)
+    m.close()
+    del w, m
+    torch.cuda.empty_cache()
+    assert rel[0] <= 1e-11 and rel[1] <= 1e-11
