@@ -166,6 +166,32 @@ def summarize_trace(trace_dir, workload, command, variant="fp64"):
     print("\n".join(lines))
 
 
+def write_read_bandwidth(torch, nbytes, reps=5):
+    """HBM GB/s of a write-only pass (fill) and of a read-only pass (sum) over
+    `nbytes`: the tracer step writes ~95 GB per step at 0.1 deg, so the write
+    rate bounds it as much as the copy rate does."""
+    x = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    x.fill_(1.0)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        x.fill_(2.0)
+    e1.record()
+    torch.cuda.synchronize()
+    w = nbytes / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
+    y = x.sum()                 # first call sets up the reduction
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        y = x.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    r = nbytes / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
+    del x, y
+    return w, r
+
+
 def cpu_baseline(args):
     """CPU restatement on a 3600 x ny_band x 55 band of the same workload
     (Mercator rows around the equator, del4, dt=600), median of timed steps
@@ -413,6 +439,7 @@ def main():
     import licom_tracer as lt
 
     copy_gbs = lt.stream_copy_bandwidth(COPY_BYTES)
+    fill_gbs, read_gbs = write_read_bandwidth(torch, COPY_BYTES)
     t0 = time.time()
     case = ls.make_case(args.workload)
     print(f"[bench] generated {args.workload} {case.nx}x{case.ny}x{case.km} in "
@@ -487,6 +514,8 @@ def main():
         "kernel_ms": r["kernel_ms"],
         "kernel_alg_gbs": r["kernel_alg_gbs"],
         "stream_copy_gbs": round(copy_gbs, 1),
+        "write_only_gbs": round(fill_gbs, 1),
+        "read_only_gbs": round(read_gbs, 1),
         "cpu_baseline": cpu,
     }
     if "mixed" in runs and main_v != "mixed":
