@@ -94,8 +94,8 @@ __device__ __noinline__ void convect_sparse(const double *C, const double *to, c
 __host__ __device__ constexpr int solve_rows(int shape) { return shape == 0 ? 4 : (shape == 1 ? 2 : 1); }
 __host__ __device__ constexpr int solve_cols(int shape) { return 128 / solve_rows(shape); }
 inline dim3 solve_grid(int imt, int jmt, int shape) {
-    // up to 15 extra leading columns per row from the line alignment
-    return dim3((imt - 4 + 15 + solve_cols(shape) - 1) / solve_cols(shape),
+    // up to 31 extra leading columns per row from the line alignment
+    return dim3((imt - 4 + 31 + solve_cols(shape) - 1) / solve_cols(shape),
                 (jmt - 4 + solve_rows(shape) - 1) / solve_rows(shape));
 }
 
@@ -129,11 +129,12 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
     __syncthreads();
     const int R = solve_rows(shape), NC = solve_cols(shape);
     const int gj = 2 + by * R + col / NC;
-    // Each wave takes 32 consecutive columns of one row, starting on a
-    // 128-byte line of the T arrays (element index a multiple of 16), so its
-    // 256-byte segments of every level are whole cache lines.
+    // Each wave takes 32 consecutive columns of one row, starting at an
+    // element index that is a multiple of 32, so the T half's 256-byte
+    // (FP64) and 128-byte (FP32) segments of every level are whole cache
+    // lines; partial lines cost the L2 a fill before their write-back.
     const long rowbase = (long)gj * imt;
-    const int gi = (int)(((rowbase + 2) & ~15L) + (long)NC * bx + col % NC - rowbase);
+    const int gi = (int)(((rowbase + 2) & ~31L) + (long)NC * bx + col % NC - rowbase);
     const bool act = gi >= 2 && gi < imt - 2 && gj < jmt - 2;
     const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
     const size_t q = act ? (size_t)gj * imt + gi : 0;
