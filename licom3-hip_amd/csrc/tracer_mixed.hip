@@ -183,21 +183,31 @@ __global__ __launch_bounds__(256) void k_prelude_mx(KGrid g, float oncc, const f
 }
 
 // --------------------------------------------------------------- march
-constexpr int BX = 16, BY = 12;                 // tile (waves 0-2), ring (wave 3)
+// A 32x6 tile: each of the three tile waves owns two whole 32-cell rows, so
+// that the level's FP32 stores leave in 128-byte row pieces (the march is
+// bound by its scattered stores, and the 64-byte pieces of a 16-wide tile
+// cost a third more time). Wave 3 is the ring/staging wave: its lane l
+// takes ring cell l (the north and south rows) and, for l < 12, ring cell
+// 64 + l (the west and east columns). Workgroups of four waves: with five
+// or six, residency fell to under two workgroups per CU.
+constexpr int BX = 32, BY = 6;
 constexpr int PX = BX + 2, PY = BY + 2;
 constexpr int RX = BX + 4, RY = BY + 4;
-constexpr int NOWN = BX * BY, NRING = 2 * (BX + BY), NT = 256;
-static_assert(NOWN == 192 && NRING <= 64, "three tile waves and one ring wave");
+constexpr int NOWN = BX * BY, NRING = 2 * (BX + BY), NT = NOWN + 64;
+static_assert(NOWN == 192 && 2 * BX == 64 && NRING > 64 && NRING <= 128,
+              "three tile waves; one ring wave with a second ring cell on its first lanes");
 constexpr int NP = PX * PY, NR = RX * RY;
 constexpr int NCH = NR / 2, NCI = (NCH + 63) / 64;   // 16-byte chunks of an FP64 window
 constexpr int NCF = (NR + 63) / 64;                  // dword loads of an FP32 window
+constexpr int NCA = NOWN / 64;                       // dword loads of the tile's akt
+constexpr int MARCH_BLOCKS = 3;                      // resident workgroups per CU (LDS)
 constexpr float BIG = 1.0e10f;
 // Look-ahead of the staged planes (as in the FP64 k_march): at and ws are
 // issued for level k+LA+2 at the start of P(k); faces, atb, akt for k+LA+1.
 constexpr int LA = 2;
 constexpr int NAT = LA + 3, NW = LA + 3, NUV = LA + 2, NB = LA + 2;
 constexpr int DMA_ATW = NCI + NCF;                  // at (16-byte), ws (dword)
-constexpr int DMA_UVBA = 2 * NCF + NCI + 3;         // u, v (dword), atb (16-byte), akt
+constexpr int DMA_UVBA = 2 * NCF + NCI + NCA;       // u, v (dword), atb (16-byte), akt (dword)
 
 struct MarchArgsMx {
     const double *at, *atb, *stf_n, *err_norm2, *swv, *restore;
@@ -364,22 +374,27 @@ __device__ __forceinline__ void stage_f(const float *src, float *dst, const int 
             dma4(src + off[u], lds_addr(dst + 64 * u));
 }
 
-// akt of the 16x12 tile (read only at the tile's own cells): 192 floats in
-// three dword instructions; element c of the slot is tile lane c.
-__device__ __forceinline__ void stage_tile_f(const float *src, float *dst, const int (&off)[3],
+// akt of the tile (read only at the tile's own cells): NOWN floats in NCA
+// dword instructions; element c of the slot is tile lane c.
+__device__ __forceinline__ void stage_tile_f(const float *src, float *dst, const int (&off)[NCA],
                                              int lane) {
 #pragma unroll
-    for (int u = 0; u < 3; ++u) dma4(src + off[u], lds_addr(dst + 64 * u));
+    for (int u = 0; u < NCA; ++u) dma4(src + off[u], lds_addr(dst + 64 * u));
 }
 
 __device__ __forceinline__ void wait_vmem() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+template <int N>
+__device__ __forceinline__ void wait_vmem_n() {
+    static_assert(N >= 0 && N <= 63, "vmcnt range");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"I"(N) : "memory");
+}
 // Waits for every staged load except the `issued` issued at the start of
-// the current level (direct-to-LDS loads complete in order).
+// the current level (direct-to-LDS loads complete in order): DMA_ATW +
+// DMA_UVBA, DMA_UVBA or none.
 __device__ __forceinline__ void wait_vmem_keep(int issued) {
-    static_assert(DMA_ATW + DMA_UVBA == 24 && DMA_UVBA == 16, "wait counts below");
-    if (LA < 2 || issued == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    else if (issued >= 24) asm volatile("s_waitcnt vmcnt(24)" ::: "memory");
-    else asm volatile("s_waitcnt vmcnt(16)" ::: "memory");
+    if (issued == DMA_ATW + DMA_UVBA) wait_vmem_n<DMA_ATW + DMA_UVBA>();
+    else if (issued == DMA_UVBA) wait_vmem_n<DMA_UVBA>();
+    else wait_vmem_n<0>();
 }
 __device__ __forceinline__ void bar_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
@@ -411,13 +426,11 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
     // merged into one exec-masked loop, and the tile waves' loads and stores
     // then carry vmcnt waits into the staging wave's code)
     const bool ringw = __builtin_amdgcn_readfirstlane(t >> 6) == NOWN / 64;
-    const int rl = t - NOWN;
-    const bool has_cell = !ringw || rl < NRING;
-    const int2 rc = ring_cell(has_cell && ringw ? rl : 0);
+    const int2 rc = ring_cell(ringw ? t - NOWN : 0);   // ring cell `lane` (< 64 < NRING)
     const int pi = ringw ? rc.x : 1 + t % BX, pj = ringw ? rc.y : 1 + t / BX;
     const int gi = gi0 - 1 + pi, gj = gj0 - 1 + pj;
     const int rC = (pj + 1) * RX + (pi + 1), pC = pj * PX + pi;
-    const bool cell_ok = has_cell && gj >= 1 && gj < jmt - 1 && gi >= 1 && gi < imt - 1;
+    const bool cell_ok = gj >= 1 && gj < jmt - 1 && gi >= 1 && gi < imt - 1;
     const bool act = !ringw && gi < imt - 2 && gj < jmt - 2;
     const size_t q = cell_ok ? (size_t)gj * imt + gi : 0;
 
@@ -451,9 +464,9 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
             const int c = min(u * 64 + lane, NR - 1), row = c / RX, col = c - row * RX;
             offf[u] = (int)min(e0 + (long)row * imt + col, (long)n2 - 1);
         }
-        int offa[3];
+        int offa[NCA];
 #pragma unroll
-        for (int u = 0; u < 3; ++u) {
+        for (int u = 0; u < NCA; ++u) {
             const int c = u * 64 + lane, row = c / BX, col = c - row * BX;
             offa[u] = (int)min((long)(gj0 + row) * imt + gi0 + col, (long)n2 - 1);
         }
@@ -467,13 +480,25 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
             stage_d(TB + (size_t)l * n2, sB[l], offd, lane);
             stage_tile_f(A + (size_t)l * n2, sAK[l], offa, lane);
         }
+        // the second ring cell of lanes 0..NRING-65 (west and east columns)
+        const bool hasB = lane < NRING - 64;
+        const int2 rb = ring_cell(hasB ? 64 + lane : 64);
+        const int giB = gi0 - 1 + rb.x, gjB = gj0 - 1 + rb.y;
+        const int rCB = (rb.y + 1) * RX + (rb.x + 1), pCB = rb.y * PX + rb.x;
+        const bool okB = hasB && gjB >= 1 && gjB < jmt - 1 && giB >= 1 && giB < imt - 1;
+        // their metrics live in LDS: in registers they push the persistent
+        // ensemble march over its register budget
+        __shared__ MetricsF sMB[NRING - 64];
+        const MetricsF &MB = sMB[min(lane, NRING - 65)];
+        if (okB) sMB[lane] = load_metrics_f(g, gjB, giB, dts);
         wait_vmem();
         bar_lds();
         {
             LWf tmp;
             const LvM L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
             const bool f0 = cell_ok ? pred_flag_mx(g, rC, L0, sK, 0, sZP, sZT, dtsf, M, tmp) : false;
-            if (has_cell) sFL[0][pC] = f0;
+            sFL[0][pC] = f0;
+            if (hasB) sFL[0][pCB] = okB ? pred_flag_mx(g, rCB, L0, sK, 0, sZP, sZT, dtsf, MB, tmp) : false;
         }
         bar_lds();
         for (int k = 0; k < km; ++k) {
@@ -492,14 +517,23 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
                 stage_tile_f(A + (size_t)(k + LA + 1) * n2, sAK[u2], offa, lane);
                 issued += DMA_UVBA;
             }
+            const LvM L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
             if (k + 1 < km) {
                 LWf tmp;
-                const LvM L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
-                const bool f = cell_ok ? pred_flag_mx(g, rC, L1, sK, k + 1, sZP, sZT, dtsf, M, tmp) : false;
-                if (has_cell) sFL[(k + 1) & 1][pC] = f;
+                sFL[(k + 1) & 1][pC] = cell_ok ? pred_flag_mx(g, rC, L1, sK, k + 1, sZP, sZT, dtsf, M, tmp) : false;
             }
-            if (p.hmix == LICOM_HMIX_DEL4 && has_cell) sD2[pC] = cell_ok ? d2tk_mx(rC, sB[u0], sK, k, M) : 0.0f;
+            if (p.hmix == LICOM_HMIX_DEL4) {
+                sD2[pC] = cell_ok ? d2tk_mx(rC, sB[u0], sK, k, M) : 0.0f;
+                if (hasB) sD2[pCB] = okB ? d2tk_mx(rCB, sB[u0], sK, k, MB) : 0.0f;
+            }
             bar_lds();
+            // C(k): the column cells' flags of level k+1, first read in
+            // C(k+1); the planes they read (at k..k+2, faces and ws of k+1)
+            // are complete and not restaged before the end of level k
+            if (k + 1 < km && hasB) {
+                LWf tmp;
+                sFL[(k + 1) & 1][pCB] = okB ? pred_flag_mx(g, rCB, L1, sK, k + 1, sZP, sZT, dtsf, MB, tmp) : false;
+            }
             wait_vmem_keep(issued);
             bar_lds();
         }
@@ -661,7 +695,7 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
     }
 }
 
-__global__ __launch_bounds__(NT, 3) void k_march_mx(KGrid g, KPar p, MarchArgsMx a) {
+__global__ __launch_bounds__(NT, MARCH_BLOCKS) void k_march_mx(KGrid g, KPar p, MarchArgsMx a) {
     // XCD-aware placement: consecutive blocks of one XCD take the T and S
     // halves of one tile, then the next tile of the same row of tiles.
     const int Lb = blockIdx.x, xcd = Lb & 7, slot = Lb >> 3;
@@ -825,7 +859,7 @@ struct MemberDev {
     int sf_bx, sf_by;                 // 64 x 4 blocks of the 2-D kernels
     int fd_shape, fd_bx, fd_by;       // prelude column blocks
     int sv_shape, sv_bx, sv_by;       // column-solve blocks
-    int ntx, ntiles;                  // 16 x 16 k-march tiles
+    int ntx, ntiles;                  // BX x BY k-march tiles
     int side_vec, side_blocks, side2_blocks;
     int halo_bx;                      // halo blocks per plane
 };
@@ -876,7 +910,7 @@ __global__ __launch_bounds__(256) void kb_prelude(const MemberDev *M, const int 
 }
 
 // Persistent k-march: each workgroup walks the (member, tile, tracer) list.
-__global__ __launch_bounds__(NT, 3) void kb_march(const MemberDev *M, const int *starts, int nm,
+__global__ __launch_bounds__(NT, MARCH_BLOCKS) void kb_march(const MemberDev *M, const int *starts, int nm,
                                                   int total) {
     const int *st_ = starts + ST_MARCH * (nm + 1);
     for (int it = blockIdx.x; it < total; it += gridDim.x) {
