@@ -55,6 +55,11 @@ enum licom_adv { LICOM_ADV_TSPAS = 2 };
 #define LICOM_STEP_STAGED 0x8       /* run one kernel per reference stage
                                        instead of the fused passes (same bits;
                                        a cross-check, not a fast path)        */
+#define LICOM_STEP_ICESNOW 0x10     /* apply icesnow_ between the tracer update
+                                       and convadj, as stepon_ does
+                                       (stepon.cpp:69-75), inside the column
+                                       solve; licomqice goes to the array set
+                                       by licom_tracer_set_icesnow            */
 
 /* Static grid (host pointers, copied by licom_tracer_create).  Replaces the
  * grid/pconst module arrays read through cuda_init.cpp:708-864. */
@@ -226,6 +231,22 @@ int licom_autotune_member(licom_ctx *ctx, const licom_fields_mixed *f, const lic
 /* Convective adjustment alone: convadj_() of convadj.cpp:290-298 over every
  * column, ghost columns included. */
 int licom_convadj(licom_ctx *ctx, const licom_fields *f, const licom_diag *d);
+
+/* Surface freezing adjustment alone: icesnow_() of icesnow.cpp:70-85
+ * (icesnow_cu, icesnow.cpp:15-68) on at [2][km][jmt][imt] over every column,
+ * ghost columns included: a wet column whose surface T is below TBICE
+ * (-1.8 degC, constant_mod.F90:168) is reset to TBICE, its licomqice takes
+ * the cooling and its surface S the rejected salt; stored ice melts against
+ * a warmer surface; T is clamped to TBICE at every level.  licomqice is a
+ * device [jmt][imt] array; reset_qice != 0 zeroes it first, as the
+ * reference's FRC_CORE build does (icesnow.cpp:76-78, def-undef.h:6). */
+int licom_icesnow(licom_ctx *ctx, double *at, double *licomqice, int reset_qice);
+
+/* Output array (device [jmt][imt]) of the icesnow that steps run with
+ * LICOM_STEP_ICESNOW (FRC_CORE semantics: rewritten on every column each
+ * step); NULL detaches it.  Replaces the d_licomqice global
+ * (cuda_data.h, icesnow.cpp:79). */
+int licom_tracer_set_icesnow(licom_ctx *ctx, double *licomqice);
 
 /* Waits for the context's stream. */
 int licom_tracer_sync(licom_ctx *ctx);

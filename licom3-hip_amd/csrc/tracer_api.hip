@@ -360,6 +360,15 @@ int licom_tracer_step(licom_ctx *c, const licom_fields *f, const licom_diag *d, 
     const uintptr_t al = reinterpret_cast<uintptr_t>(f->at) | reinterpret_cast<uintptr_t>(f->atb) |
                          reinterpret_cast<uintptr_t>(f->ws) | reinterpret_cast<uintptr_t>(f->akt);
     if (c->imt % 2 != 0 || (al & 15) != 0) flags |= LICOM_STEP_STAGED;
+    if (flags & LICOM_STEP_ICESNOW) {
+        if (!c->qice) {
+            set_error("LICOM_STEP_ICESNOW needs licom_tracer_set_icesnow first");
+            return LICOM_EINVAL;
+        }
+        // the S left in vtl is the one before icesnow: the per-stage path
+        // runs icesnow after that copy
+        if (flags & LICOM_STEP_VTL_SCRATCH) flags |= LICOM_STEP_STAGED;
+    }
     if (flags & LICOM_STEP_STAGED) {
         // scratch of the per-stage kernels, allocated on first use
         if (!c->s.flags) {
@@ -390,6 +399,10 @@ int licom_tracer_step_mixed(licom_ctx *c, const licom_fields_mixed *f, const lic
         set_error("licom_tracer_step_mixed: VTL_SCRATCH and STAGED are FP64-only");
         return LICOM_ENOTSUP;
     }
+    if ((flags & LICOM_STEP_ICESNOW) && !c->qice) {
+        set_error("LICOM_STEP_ICESNOW needs licom_tracer_set_icesnow first");
+        return LICOM_EINVAL;
+    }
     const uintptr_t al = reinterpret_cast<uintptr_t>(f->at) | reinterpret_cast<uintptr_t>(f->atb);
     if (c->imt % 2 != 0 || (al & 15) != 0) {
         set_error("licom_tracer_step_mixed: needs an even imt and 16-byte aligned at/atb");
@@ -404,6 +417,20 @@ int licom_convadj(licom_ctx *c, const licom_fields *f, const licom_diag *d) {
         return LICOM_EINVAL;
     }
     return launch_convadj(c, f, d);
+}
+
+int licom_icesnow(licom_ctx *c, double *at, double *licomqice, int reset_qice) {
+    if (!c || !at || !licomqice) {
+        set_error("licom_icesnow: NULL argument");
+        return LICOM_EINVAL;
+    }
+    return launch_icesnow(c, at, licomqice, reset_qice);
+}
+
+int licom_tracer_set_icesnow(licom_ctx *c, double *licomqice) {
+    if (!c) return LICOM_EINVAL;
+    c->qice = licomqice;
+    return LICOM_OK;
 }
 
 int licom_tracer_sync(licom_ctx *c) {

@@ -104,6 +104,16 @@ __device__ __forceinline__ double up_zb2(double Tc, double Td, double Wb, double
     return 0.5 * fabs(Wb) * odzpk * (Tc - Td);
 }
 
+// icesnow (icesnow.cpp:15-68): freezing point TBICE and CP
+// (constant_mod.F90:168,164), the latent heat of fusion (shr_const_mod.h:51)
+// and the reference salinities of sea water and sea ice (icesnow.cpp:27-28).
+constexpr double kTbice = -1.8, kCpIce = 3996.0, kLatIce = 3.337e5, kSalOcn = 34.7, kSalIce = 4.0;
+// Surface S change (model units) for tdiff K of freezing, in the reference's
+// operation order (icesnow.cpp:39-40, 55-57).
+__device__ __forceinline__ double ice_salt(double tdiff) {
+    return tdiff * (kSalOcn - kSalIce) * kCpIce / kLatIce * 0.001;
+}
+
 // AKT surface floor: biharmonic build floors T only (tracer.cpp:1315), the
 // Laplacian build every tracer (:1187); only interface 0 is affected.
 __device__ __forceinline__ double floor_akt(double a, int k, int n, int hmix, double dwndmix) {

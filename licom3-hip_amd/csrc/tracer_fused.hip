@@ -741,9 +741,15 @@ void launch_prelude_solve(licom_ctx *ctx, const licom_fields *f, const licom_dia
     } else {
         Timer t(ctx, "k_solve");
         const dim3 gr = solve_grid(g.imt, g.jmt, ctx->solve_shape);
-        k_solve<KM, double, double><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
+        if (flags & LICOM_STEP_ICESNOW)
+            k_solve_ice<KM, double, double><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
                                                   d ? d->tend : nullptr, d ? d->dt_diff : nullptr,
-                                                  d ? d->dt_conv : nullptr, s.mix_count, ctx->solve_shape);
+                                                  d ? d->dt_conv : nullptr, s.mix_count, ctx->solve_shape,
+                                                  ctx->qice);
+        else
+            k_solve<KM, double, double><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
+                                              d ? d->tend : nullptr, d ? d->dt_diff : nullptr,
+                                              d ? d->dt_conv : nullptr, s.mix_count, ctx->solve_shape);
     }
 }
 

@@ -61,6 +61,8 @@ struct licom_ctx {
     // 3-D array sits 64 bytes off the T half at odd n3/8) are mostly written
     // and read by one workgroup
     int solve_shape = 0;
+    // licomqice [jmt][imt] (device) written by steps with LICOM_STEP_ICESNOW
+    double *qice = nullptr;
     licom::KGrid g{};
     licom::KPar kp{};
     double area_t = 0.0;
@@ -89,6 +91,7 @@ int hip_status(hipError_t e, const char *what);
 // tracer_kernels.hip: one kernel per reference stage (cross-check path)
 int launch_step_simple(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags);
 int launch_convadj(licom_ctx *ctx, const licom_fields *f, const licom_diag *d);
+int launch_icesnow(licom_ctx *ctx, double *at, double *qice, int reset);
 // shared pieces of tracer_kernels.hip used by both paths
 int launch_surface(licom_ctx *ctx, const licom_fields *f, const licom_diag *d);
 int launch_halo(licom_ctx *ctx, const licom_fields *f, int flags);

@@ -518,6 +518,40 @@ __global__ void k_convadj(KGrid g, KPar p, double *at, double *atb, double *tend
         }
 }
 
+// icesnow_cu (icesnow.cpp:15-68) on every column, ghosts included; `reset`
+// folds in the FRC_CORE zeroing of licomqice (icesnow.cpp:76-78).  Cells the
+// reference leaves alone are not written.
+__global__ __launch_bounds__(256) void k_icesnow(KGrid g, double *__restrict__ at,
+                                                 double *__restrict__ qice, int reset) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= g.imt || j >= g.jmt) return;
+    const size_t n2 = (size_t)g.imt * g.jmt, n3 = n2 * g.km, q = i2(g, j, i);
+    const double tb = dev::kTbice;
+    double T0 = at[q], S0 = at[n3 + q], qi = reset ? 0.0 : qice[q];
+    const double T0in = T0, S0in = S0;
+    if (T0 < tb && g.kmt[q] > 0) {
+        const double tdiff = tb - T0;
+        qi += tdiff * g.dzp[0] / g.dzp[0];
+        S0 = S0 + dev::ice_salt(tdiff);
+        T0 = tb;
+    }
+    if (qi > 0 && T0 > tb) {
+        const double tdiff = fmin(T0 - tb, qi);
+        qi -= tdiff;
+        T0 -= tdiff;
+        S0 = S0 - dev::ice_salt(tdiff);
+    }
+    if (T0 < tb) T0 = tb;
+    if (T0 != T0in) at[q] = T0;
+    if (S0 != S0in) at[n3 + q] = S0;
+    qice[q] = qi;
+    for (int k = 1; k < g.km; ++k) {
+        double *p = at + (size_t)k * n2 + q;
+        if (*p < tb) *p = tb;
+    }
+}
+
 // 16-byte vector copy used for bandwidth calibration: one element per lane,
 // one pass over the grid (faster on gfx950 than a grid-stride loop).
 __global__ __launch_bounds__(256) void k_stream_copy(const double2 *__restrict__ src,
@@ -583,6 +617,11 @@ int launch_halo(licom_ctx *ctx, const licom_fields *f, int flags) {
         const int cells = 4 * (g.jmt - 4) + 4 * g.imt;
         k_halo<<<dim3((cells + 255) / 256, 4 * g.km), 256, 0, st>>>(g.imt, g.jmt, 4 * g.km, f->at,
                                                                      f->atb, g.km);
+        // licomqice of the fused icesnow: ghost columns copy their interior
+        // columns, closed rows hold 0, as icesnow_cu gives on the refilled
+        // ghost cells of at and kmt
+        if ((flags & LICOM_STEP_ICESNOW) && ctx->qice)
+            k_halo<<<dim3((cells + 255) / 256, 1), 256, 0, st>>>(g.imt, g.jmt, 1, ctx->qice, nullptr, g.km);
     }
     if (flags & LICOM_STEP_VTL_SCRATCH) {
         Timer t(ctx, "k_copy_vtl");
@@ -625,13 +664,20 @@ int launch_step_simple(licom_ctx *ctx, const licom_fields *f, const licom_diag *
                                       d ? d->az : nullptr, d ? d->dz : nullptr,
                                       d ? d->penetrate : nullptr);
     }
+    // with icesnow, the reference's order tracer_ -> icesnow_ -> convadj_
+    // (stepon.cpp:69-75): the column kernel without convadj, the halo, then
+    // icesnow and convadj over every column
+    const bool ice = flags & LICOM_STEP_ICESNOW;
     {
         Timer t(ctx, "k_column");
-        k_column<<<g2, b2, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n, f->utf,
-                                    f->vtf, f->h0f, d ? d->tend : nullptr,
+        k_column<<<g2, b2, 0, st>>>(g, p, ice ? flags & ~LICOM_STEP_CONVADJ : flags, f->at, f->atb, f->akt,
+                                    s.ori, s.stf_n, f->utf, f->vtf, f->h0f, d ? d->tend : nullptr,
                                     d ? d->dt_diff : nullptr, d ? d->dt_conv : nullptr);
     }
-    return launch_halo(ctx, f, flags);
+    if (int rc = launch_halo(ctx, f, flags & ~LICOM_STEP_ICESNOW)) return rc;
+    if (!ice) return LICOM_OK;
+    if (int rc = launch_icesnow(ctx, f->at, ctx->qice, 1)) return rc;
+    return (flags & LICOM_STEP_CONVADJ) ? launch_convadj(ctx, f, d) : LICOM_OK;
 }
 
 int launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t st) {
@@ -644,6 +690,13 @@ int launch_stream_copy(void *dst, const void *src, size_t bytes, hipStream_t st)
     k_stream_copy<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(static_cast<const double2 *>(src),
                                                                static_cast<double2 *>(dst), n);
     return hip_status(hipGetLastError(), "stream copy launch");
+}
+
+int launch_icesnow(licom_ctx *ctx, double *at, double *qice, int reset) {
+    const dim3 b2(64, 4);
+    Timer t(ctx, "k_icesnow");
+    k_icesnow<<<grid2(ctx->imt, ctx->jmt, b2), b2, 0, ctx->stream>>>(ctx->g, at, qice, reset);
+    return hip_status(hipGetLastError(), "icesnow launch");
 }
 
 int launch_convadj(licom_ctx *ctx, const licom_fields *f, const licom_diag *d) {

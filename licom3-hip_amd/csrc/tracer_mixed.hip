@@ -751,9 +751,15 @@ void launch_prelude_solve_mx(licom_ctx *ctx, const licom_fields_mixed *f, const 
     } else {
         Timer t(ctx, "k_solve");
         const dim3 gr = solve_grid(g.imt, g.jmt, ctx->solve_shape);
-        k_solve<KM, float, float><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
+        if (flags & LICOM_STEP_ICESNOW)
+            k_solve_ice<KM, float, float><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
                                                   d ? d->tend : nullptr, d ? d->dt_diff : nullptr,
-                                                  d ? d->dt_conv : nullptr, s.mix_count, ctx->solve_shape);
+                                                  d ? d->dt_conv : nullptr, s.mix_count, ctx->solve_shape,
+                                                  ctx->qice);
+        else
+            k_solve<KM, float, float><<<gr, 256, 0, st>>>(g, p, flags, f->at, f->atb, f->akt, s.ori, s.stf_n,
+                                              d ? d->tend : nullptr, d ? d->dt_diff : nullptr,
+                                              d ? d->dt_conv : nullptr, s.mix_count, ctx->solve_shape);
     }
 }
 
@@ -946,7 +952,7 @@ __global__ __launch_bounds__(256) void kb_solve(const MemberDev *M, const int *s
     LICOM_MEMBER(ST_SOLVE)
     solve_body<KM, float, float>(d.g, d.p, flags, d.at, d.atb, d.akt, d.ori, d.stf_n, d.tend,
                                  d.dt_diff, d.dt_conv, d.mix_count, l % d.sv_bx, l / d.sv_bx,
-                                 d.sv_shape);
+                                 d.sv_shape, nullptr);
 }
 
 __global__ __launch_bounds__(256) void kb_side3(const MemberDev *M, const int *starts, int nm) {
@@ -1040,6 +1046,10 @@ int batch_launch_step(licom_batch *b, int flags) {
 int check_batch_flags(int flags) {
     if (flags & (LICOM_STEP_VTL_SCRATCH | LICOM_STEP_STAGED)) {
         licom::set_error("batch: VTL_SCRATCH and STAGED are FP64-only");
+        return LICOM_ENOTSUP;
+    }
+    if (flags & LICOM_STEP_ICESNOW) {
+        licom::set_error("batch: LICOM_STEP_ICESNOW is not built for batches");
         return LICOM_ENOTSUP;
     }
     return LICOM_OK;

@@ -101,7 +101,8 @@ inline dim3 solve_grid(int imt, int jmt, int shape) {
 
 // AK: element type of akt; DG: element type of the diagnostics (double for
 // the FP64 drop-in, float for the mixed form; the solve itself is FP64).
-template <int KM, typename AK, typename DG>
+// ICE: icesnow between the update and convadj, licomqice to qice.
+template <int KM, typename AK, typename DG, bool ICE = false>
 __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int flags,
                                            double *__restrict__ at, double *__restrict__ atb,
                                            const AK *__restrict__ akt,
@@ -109,7 +110,7 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
                                            const double *__restrict__ stf_n,
                                            DG *__restrict__ tend, DG *__restrict__ dt_diff,
                                            DG *__restrict__ dt_conv, int *mix_count, int bx,
-                                           int by, int shape) {
+                                           int by, int shape, double *__restrict__ qice) {
     constexpr int NSEG = (KM + SEG - 1) / SEG;
     const int t = threadIdx.x, n = (t >> 5) & 1, col = (t >> 6) * 32 + (t & 31);
     const int imt = g.imt, jmt = g.jmt, km = g.km;
@@ -143,6 +144,12 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
     const double *odzp = sZP, *odzt = sZT;
     const double dts = p.dts;
     const double stf = act ? stf_n[n * n2 + q] : 0.0;
+    // icesnow between the update and convadj: dzp(0) for the ice volume,
+    // levels whose state icesnow changed (bit k)
+    constexpr bool ice = ICE;
+    const double dzp0 = ice ? g.dzp[0] : 1.0;
+    uint64_t icemask = 0;
+    double qv = 0.0;
     const int hmix = p.hmix;
     const double dwm = p.dwndmix;
     // With FP32 diagnostics (kBuf) every 3-D access of the substitution goes
@@ -300,6 +307,31 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
                 double v = (x - o) / dts * vk;
                 if (k == 0) v = v - stf * 0.5 * odzp[0];
                 const double tn = (x - ct[r]) / dts * vk;
+                // icesnow_cu (icesnow.cpp:33-67) on the updated column: the
+                // surface freezing of a wet column (T lane) with its rejected
+                // salt (S lane, through the lane pair), then the TBICE clamp
+                // of T at every level; atb keeps the tracer_ state x, as the
+                // reference's icesnow leaves atb alone.  Its melt branch
+                // (icesnow.cpp:47-59) needs licomqice > 0 on entry, which the
+                // FRC_CORE reset rules out.
+                double xs = x;
+                if constexpr (ice) {
+                    if (k == 0) {
+                        double td = 0.0;
+                        if (n == 0 && kz > 0 && x < dev::kTbice) {
+                            td = dev::kTbice - x;
+                            xs = dev::kTbice;
+                        }
+                        const double tdo = __shfl_xor(td, 32);
+                        if (n == 1 && tdo > 0.0) xs = x + dev::ice_salt(tdo);
+                        qv = td > 0.0 ? 0.0 + td * dzp0 / dzp0 : 0.0;
+                    }
+                    if (n == 0 && xs < dev::kTbice) xs = dev::kTbice;
+                    if (xs != x && k < km) icemask |= 1ull << k;
+                    // keep the mask materialised level by level (sunk to its
+                    // use in the tail, it keeps every level's values live)
+                    asm volatile("" : "+v"(icemask));
+                }
                 if constexpr (kBuf) {
                     const bool st = act && k < km;
                     const uint32_t og = st ? qg : kBufOff, o8 = st ? q8 : kBufOff;
@@ -307,19 +339,19 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
                     buf_store(plane(tend ? tend + n * n3 : nullptr, sizeof(DG), k, pbg), og, (DG)tn);
                     buf_store(plane(conv && dt_conv ? dt_conv + n * n3 : nullptr, sizeof(DG), k, pbg), og,
                               (DG)0.0);
-                    buf_store(plane(Tb, 8, k, pb8), o8, x);
+                    buf_store(plane(Tb, 8, k, pb8), o8, xs);
                     buf_store(plane(Bb, 8, k, pb8), o8, x);
                 } else if (act && k < km) {
                     const size_t c = n * n3 + (size_t)k * n2 + q;
                     if (dt_diff) dt_diff[c] = (DG)v;
                     if (tend) tend[c] = (DG)tn;
                     if (conv && dt_conv) dt_conv[c] = (DG)0.0;
-                    Tb[(size_t)k * n2 + q] = x;
+                    Tb[(size_t)k * n2 + q] = xs;
                     Bb[(size_t)k * n2 + q] = x;
                 }
                 if (conv) {
-                    const double xo = __shfl_xor(x, 32);
-                    const double Tk = n == 0 ? x : xo, Sk = n == 0 ? xo : x;
+                    const double xo = __shfl_xor(xs, 32);
+                    const double Tk = n == 0 ? xs : xo, Sk = n == 0 ? xo : xs;
                     if (k >= 1 && k <= kz - 1) {
                         const double up = dev::dens(C, km, k, Tk - to[k], Sk - so[k]);
                         const double lo = dev::dens(C, km, k, Tp - to[k], Sp - so[k]);
@@ -333,15 +365,40 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
     }
 
     // ---- convadj for statically unstable columns (both lanes of the pair)
-    if (conv && act && mask != 0) {
-        if (n == 0) atomicAdd(mix_count, 1);
-        int lo, hi;
-        convect_sparse(g.eosc, g.to, g.so, g.dzp, km, kz, mask, n, TT, n2, lo, hi);
-        for (int l = lo; l <= hi; ++l) {
+    if constexpr (!ice) {
+        if (conv && act && mask != 0) {
+            if (n == 0) atomicAdd(mix_count, 1);
+            int lo, hi;
+            convect_sparse(g.eosc, g.to, g.so, g.dzp, km, kz, mask, n, TT, n2, lo, hi);
+            for (int l = lo; l <= hi; ++l) {
+                const size_t c = (size_t)l * n2;
+                const double y = TT[c], b = BB[c];
+                if (y != b) {
+                    const double dc = (y - b) / dts * 1.0;
+                    if (dt_conv) dt_conv[n * n3 + q + c] = (DG)dc;
+                    if (tend) tend[n * n3 + q + c] = (DG)((double)tend[n * n3 + q + c] + dc);
+                    BB[c] = y;
+                }
+            }
+        }
+        return;
+    }
+    if (n == 0 && act) qice[q] = qv;
+    // with icesnow, convadj's dt_conv = (at - atb)/dts, tend += dt_conv,
+    // atb = at (convadj.cpp:259-273) also covers the levels icesnow changed
+    // (every other level has at == atb here)
+    if (conv && act && (mask != 0 || icemask != 0)) {
+        int lo = km, hi = -1;
+        if (mask != 0) {   // the same in both lanes of the pair
+            if (n == 0) atomicAdd(mix_count, 1);
+            convect_sparse(g.eosc, g.to, g.so, g.dzp, km, kz, mask, n, TT, n2, lo, hi);
+        }
+        for (int l = 0; l < km; ++l) {
+            if ((l < lo || l > hi) && !((icemask >> l) & 1ull)) continue;
             const size_t c = (size_t)l * n2;
             const double y = TT[c], b = BB[c];
             if (y != b) {
-                const double dc = (y - b) / dts * 1.0;
+                const double dc = (y - b) / dts * (l < kz ? 1.0 : 0.0);
                 if (dt_conv) dt_conv[n * n3 + q + c] = (DG)dc;
                 if (tend) tend[n * n3 + q + c] = (DG)((double)tend[n * n3 + q + c] + dc);
                 BB[c] = y;
@@ -361,8 +418,24 @@ __global__ __launch_bounds__(256, 3) void k_solve(KGrid g, KPar p, int flags, do
                                                DG *__restrict__ tend,
                                                DG *__restrict__ dt_diff,
                                                DG *__restrict__ dt_conv, int *mix_count, int shape) {
-    solve_body<KM, AK, DG>(g, p, flags, at, atb, akt, ori, stf_n, tend, dt_diff, dt_conv, mix_count,
-                           blockIdx.x, blockIdx.y, shape);
+    solve_body<KM, AK, DG, false>(g, p, flags, at, atb, akt, ori, stf_n, tend, dt_diff, dt_conv, mix_count,
+                                  blockIdx.x, blockIdx.y, shape, nullptr);
+}
+
+// The same with icesnow between the update and convadj (LICOM_STEP_ICESNOW);
+// its extra live values exceed the three-waves-per-SIMD register budget.
+template <int KM, typename AK, typename DG>
+__global__ __launch_bounds__(256, 2) void k_solve_ice(KGrid g, KPar p, int flags, double *__restrict__ at,
+                                                   double *__restrict__ atb,
+                                                   const AK *__restrict__ akt,
+                                                   const double *__restrict__ ori,
+                                                   const double *__restrict__ stf_n,
+                                                   DG *__restrict__ tend,
+                                                   DG *__restrict__ dt_diff,
+                                                   DG *__restrict__ dt_conv, int *mix_count, int shape,
+                                                   double *__restrict__ qice) {
+    solve_body<KM, AK, DG, true>(g, p, flags, at, atb, akt, ori, stf_n, tend, dt_diff, dt_conv, mix_count,
+                                 blockIdx.x, blockIdx.y, shape, qice);
 }
 
 }  // namespace

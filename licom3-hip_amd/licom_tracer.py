@@ -32,6 +32,7 @@ LICOM_STEP_CONVADJ = 0x1
 LICOM_STEP_SIDE_EFFECTS = 0x2
 LICOM_STEP_VTL_SCRATCH = 0x4
 LICOM_STEP_STAGED = 0x8
+LICOM_STEP_ICESNOW = 0x10
 HMIX_DEL2, HMIX_DEL4 = 2, 4
 ADV_TSPAS = 2
 
@@ -76,7 +77,8 @@ EXPORTED = ["licom_tracer_create", "licom_tracer_set_stream", "licom_tracer_step
             "licom_tracer_step_mixed", "licom_batch_create", "licom_batch_set_stream",
             "licom_batch_set_march_grid", "licom_batch_step", "licom_batch_capture",
             "licom_batch_replay", "licom_batch_sync", "licom_batch_destroy",
-            "licom_autotune_member", "licom_convadj", "licom_tracer_sync", "licom_grid_export", "licom_set_profiling",
+            "licom_autotune_member", "licom_convadj", "licom_icesnow", "licom_tracer_set_icesnow",
+            "licom_tracer_sync", "licom_grid_export", "licom_set_profiling",
             "licom_kernel_times", "licom_scratch_bytes", "licom_last_mix_count",
             "licom_stream_copy", "licom_tracer_destroy", "licom_strerror", "licom_last_error"]
 
@@ -106,6 +108,8 @@ def lib():
         L.licom_tracer_step_mixed.argtypes = [_VP, ctypes.POINTER(Fields), ctypes.POINTER(Diag),
                                               ctypes.c_int]
         L.licom_convadj.argtypes = [_VP, ctypes.POINTER(Fields), ctypes.POINTER(Diag)]
+        L.licom_icesnow.argtypes = [_VP, _VP, _VP, ctypes.c_int]
+        L.licom_tracer_set_icesnow.argtypes = [_VP, _VP]
         L.licom_tracer_sync.argtypes = [_VP]
         L.licom_grid_export.argtypes = [_VP, ctypes.POINTER(GridDerived)]
         L.licom_set_profiling.argtypes = [_VP, ctypes.c_int]
@@ -221,6 +225,11 @@ class TracerModel:
                              dz=z(2, km, jmt, imt), tend=z(2, km, jmt, imt),
                              dt_diff=z(2, km, jmt, imt), dt_conv=z(2, km, jmt, imt),
                              penetrate=z(km, jmt, imt), net=z64(2, jmt, imt), fw_norm2=z64(1))
+        # licomqice of icesnow_ (tracer_mod.F90), written by icesnow() and by
+        # steps with LICOM_STEP_ICESNOW
+        self.licomqice = torch.zeros((jmt, imt), dtype=torch.float64, device=self.device)
+        check(L.licom_tracer_set_icesnow(self.ctx, _VP(self.licomqice.data_ptr())),
+              "licom_tracer_set_icesnow")
         self.ist = 0
 
     # -- C structs -------------------------------------------------------
@@ -259,19 +268,30 @@ class TracerModel:
         F, D = self._fields(), self._diag()
         check(lib().licom_convadj(self.ctx, ctypes.byref(F), ctypes.byref(D)), "licom_convadj")
 
+    def icesnow(self, reset=True):
+        """icesnow_() over every column (icesnow.cpp:70-85): surface freezing
+        adjustment and TBICE clamp of at; licomqice zeroed first when reset
+        (the FRC_CORE build)."""
+        if self.precision == "mixed":
+            raise LicomError("icesnow() alone is FP64-only; use step() with LICOM_STEP_ICESNOW")
+        check(lib().licom_icesnow(self.ctx, _VP(self.state["at"].data_ptr()),
+                                  _VP(self.licomqice.data_ptr()), int(bool(reset))), "licom_icesnow")
+
     def step(self, flags=LICOM_STEP_SIDE_EFFECTS | LICOM_STEP_CONVADJ):
         """tracer_ followed by convadj_ in one launch sequence (the CPU
         driver's per-step order, licom_drv.F90:357-380)."""
         self.tracer(0, flags)
 
-    def stepon(self, nss, convadj_every_step=True):
+    def stepon(self, nss, convadj_every_step=True, icesnow=False):
         """Tracer part of stepon_ (stepon.cpp:57-77): convadj every step, or
-        only at ii == 0 as the GPU driver does."""
+        only at ii == 0 as the GPU driver does; with icesnow, icesnow_
+        between the two (stepon.cpp:71), fused into the step."""
+        ice = LICOM_STEP_ICESNOW if icesnow else 0
         for ii in range(nss):
-            if convadj_every_step:
-                self.tracer(ii, LICOM_STEP_SIDE_EFFECTS | LICOM_STEP_CONVADJ)
+            if convadj_every_step or (icesnow and ii == 0):
+                self.tracer(ii, LICOM_STEP_SIDE_EFFECTS | LICOM_STEP_CONVADJ | ice)
             else:
-                self.tracer(ii, LICOM_STEP_SIDE_EFFECTS)
+                self.tracer(ii, LICOM_STEP_SIDE_EFFECTS | ice)
                 if ii == 0:
                     self.convadj()
 

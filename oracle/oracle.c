@@ -833,3 +833,43 @@ void orc_convadj(const orc_grid *g, const orc_params *p, orc_fields *f,
 #undef AT
         }
 }
+
+/* --------------------------------------------------------------- icesnow */
+/* icesnow_ (icesnow.cpp:70-85) with the active build's FRC_CORE reset of
+ * licomqice (icesnow.cpp:76-78, def-undef.h:6) when reset is nonzero, then
+ * icesnow_cu (icesnow.cpp:15-68) on every column, ghosts included:
+ *   1. a wet column whose surface T is below TBICE freezes back to TBICE;
+ *      licomqice takes tdiff*dzp(0)/dzp(0) and the surface S the rejected
+ *      salt tdiff*(34.7-4.0)*CP/LATICE*0.001 (icesnow.cpp:33-44);
+ *   2. stored ice melts against a surface above TBICE (icesnow.cpp:47-59);
+ *   3. T is clamped to TBICE at every level (icesnow.cpp:61-67).
+ * TBICE = -1.8 and CP = 3996 (constant_mod.F90:164,168), LATICE = 3.337e5
+ * (shr_const_mod.h:51). */
+void orc_icesnow(const orc_grid *g, double *at, double *licomqice, int reset) {
+    const int imt = g->imt, jmt = g->jmt, km = g->km;
+    const size_t n2 = (size_t)imt * jmt, n3 = n2 * km;
+    const double tbice = -1.8, cp = 3996.0, latice = 3.337e5;
+    const double sal_ocn = 34.7, sal_ice = 4.0;
+    if (reset)
+        for (size_t q = 0; q < n2; ++q) licomqice[q] = 0.0;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int j = 0; j < jmt; ++j)
+        for (int i = 0; i < imt; ++i) {
+            const size_t q = I2(j, i);
+            double *T = at + q, *S = at + n3 + q;
+            if (T[0] < tbice && g->kmt[q] > 0) {
+                const double tdiff = tbice - T[0];
+                licomqice[q] += tdiff * g->dzp[0] / g->dzp[0];
+                S[0] += tdiff * (sal_ocn - sal_ice) * cp / latice * 0.001;
+                T[0] = tbice;
+            }
+            if (licomqice[q] > 0 && T[0] > tbice) {
+                const double tdiff = fmin(T[0] - tbice, licomqice[q]);
+                licomqice[q] -= tdiff;
+                T[0] -= tdiff;
+                S[0] -= tdiff * (sal_ocn - sal_ice) * cp / latice * 0.001;
+            }
+            for (int k = 0; k < km; ++k)
+                if (T[(size_t)k * n2] < tbice) T[(size_t)k * n2] = tbice;
+        }
+}

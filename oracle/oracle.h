@@ -12,6 +12,7 @@
  *                          invtrit, operators_div, upwell_2..4)
  *   tracer.cpp:1701-1903  (host sequence of tracer_)
  *   convadj.cpp:29-288    (convadj_cu)
+ *   icesnow.cpp:15-85     (icesnow_cu, icesnow_)
  * and the initialisation of the path's static inputs:
  *   boundary.F90:29-52, grid.F90:1096-1106, grid.F90:1343-1346,
  *   grid.F90:2029-2050, grid.F90:466, hmix_del2.F90:346-385,
@@ -99,6 +100,10 @@ void orc_invtri_column(int kz, const double *wkc_col, const double *odzt,
                        const double *odzp, double dts, double stf, double vit0,
                        double *x /* in: d, out: solution */);
 double orc_dens(const double *eosc, int km, int k, double t, double s);
+/* icesnow_ (icesnow.cpp:15-85): surface freezing adjustment and TBICE clamp of
+ * at [2][km][jmt][imt] on every column; licomqice [jmt][imt] is zeroed first
+ * when reset is nonzero (the FRC_CORE build). */
+void orc_icesnow(const orc_grid *g, double *at, double *licomqice, int reset);
 int orc_nthreads(void);
 
 #ifdef __cplusplus

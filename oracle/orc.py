@@ -69,6 +69,7 @@ def lib():
         _LIB.orc_dens.argtypes = [_DP, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         _LIB.orc_dens.restype = ctypes.c_double
         _LIB.orc_halo_fill_f64.argtypes = [_DP, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        _LIB.orc_icesnow.argtypes = [ctypes.POINTER(OrcGrid), _DP, _DP, ctypes.c_int]
         _LIB.orc_nthreads.restype = ctypes.c_int
         _LIB.orc_advect.argtypes = [ctypes.POINTER(OrcGrid), ctypes.c_double, _DP, _DP, _DP, _DP, _DP]
         _LIB.orc_del2_level.argtypes = [ctypes.POINTER(OrcGrid), _DP, ctypes.c_int, ctypes.c_int,
@@ -163,6 +164,13 @@ class Oracle:
                               ctypes.byref(self._fields(st, forcing)), ctypes.byref(od))
         diag["fw_norm2"] = od.fw_norm2
         return diag
+
+    def icesnow(self, st, licomqice, reset=True):
+        """icesnow_ (icesnow.cpp:15-85) on st["at"] in place; licomqice
+        [jmt][imt] is zeroed first when reset (the FRC_CORE build)."""
+        assert st["at"].flags.c_contiguous and licomqice.flags.c_contiguous
+        lib().orc_icesnow(ctypes.byref(self.g), _d(st["at"]), _d(licomqice), int(bool(reset)))
+        return licomqice
 
     def convadj(self, st, diag=None, **over):
         diag = diag if diag is not None else self.new_diag(self.case)

@@ -26,3 +26,18 @@ def mismatch(a, b):
 def report(name, a, b):
     n = mismatch(a, b)
     return f"{name}: {n} mismatches, rel err {rel_err(a, b):.3e}"
+
+
+TBICE = -1.8   # freezing point of sea water (constant_mod.F90:168)
+
+
+def plant_cold(case, st, seed, frac=0.05):
+    """Cools a random `frac` of the cells of T (every column, ghosts and land
+    included) to between 1.2 and 0.01 K below TBICE, and a larger share of
+    the surface, so that each branch of icesnow has cases.  In place."""
+    rng = np.random.default_rng(seed)
+    T = st["at"][0]
+    cold = rng.random(T.shape) < frac
+    cold[0] |= rng.random(T.shape[1:]) < 4 * frac
+    T[cold] = TBICE - rng.uniform(0.01, 1.2, int(cold.sum()))
+    return st

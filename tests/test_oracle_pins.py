@@ -436,3 +436,66 @@ def test_sssnorm_removes_the_net_freshwater_flux():
     st2 = c.copy_state()
     o.step(st2, convadj=0, sssnorm=0)
     assert abs(orc.content(c, o, st2)[1] - before) > 1e3 * abs(after - before)
+
+
+# ------------------------------------------------------------------ icesnow
+# Salt rejected per kelvin of surface freezing, in model units ((S-35)/1000):
+# (34.7 - 4.0) psu * CP / LATICE / 1000, constants of constant_mod.F90:164 and
+# shr_const_mod.h:51, evaluated here in another order.
+ICE_SALT_PER_K = 30.7 / 3.337e5 * 3996.0 * 1.0e-3
+
+
+def _ice_case(seed=3):
+    from helpers import plant_cold
+    c = ls.make_case("C1", nx=24, ny=12, km=8, seed=seed, depth_min=-1500.0, sigma=2.0)
+    st = plant_cold(c, c.copy_state(), seed)
+    return c, orc.Oracle(c), st
+
+
+def test_icesnow_clamps_and_freezes_wet_surface_cells():
+    from helpers import TBICE
+    c, o, st = _ice_case()
+    T0, S0 = st["at"][0].copy(), st["at"][1].copy()
+    qice = o.icesnow(st, np.full((c.jmt, c.imt), 7.0), reset=True)
+    T, S = st["at"]
+    kmt = o.d["kmt"] if "kmt" in o.d else c.grid["kmt"]
+    wet = kmt > 0
+    froze = (T0[0] < TBICE) & wet
+    assert froze.sum() > 10 and ((T0[0] < TBICE) & ~wet).sum() > 10
+    # every level: T = max(T0, TBICE); S below the surface untouched
+    np.testing.assert_array_equal(T, np.maximum(T0, TBICE))
+    np.testing.assert_array_equal(S[1:], S0[1:])
+    # frozen wet cells: ice = the cooling below TBICE (x*dzp/dzp: 1 ulp)
+    td = TBICE - T0[0][froze]
+    np.testing.assert_allclose(qice[froze], td, rtol=2.3e-16, atol=0)
+    np.testing.assert_allclose(S[0][froze] - S0[0][froze], td * ICE_SALT_PER_K, rtol=1e-9, atol=0)
+    # no ice and no salt anywhere else (land cells only clamp)
+    assert not np.any(qice[~froze])
+    np.testing.assert_array_equal(S[0][~froze], S0[0][~froze])
+
+
+def test_icesnow_melts_stored_ice_against_a_warm_surface():
+    from helpers import TBICE
+    c, o, st = _ice_case(4)
+    st["at"][0] = np.maximum(st["at"][0], TBICE + 0.5)     # nothing freezes
+    st["at"][0][0, 5, 6], st["at"][0][0, 7, 9] = -1.0, -1.0
+    S0 = st["at"][1][0].copy()
+    q = np.zeros((c.jmt, c.imt))
+    q[5, 6], q[7, 9] = 0.5, 2.0
+    o.icesnow(st, q, reset=False)
+    T = st["at"][0][0]
+    # melt min(T - TBICE, ice): 0.5 K of the 0.8 K, then all 0.8 K of 2.0
+    assert T[5, 6] == pytest.approx(-1.5, abs=1e-15) and q[5, 6] == 0.0
+    assert T[7, 9] == pytest.approx(TBICE, abs=1e-15) and q[7, 9] == pytest.approx(1.2, abs=1e-15)
+    assert S0[5, 6] - st["at"][1][0][5, 6] == pytest.approx(0.5 * ICE_SALT_PER_K, rel=1e-9)
+    assert S0[7, 9] - st["at"][1][0][7, 9] == pytest.approx(0.8 * ICE_SALT_PER_K, rel=1e-9)
+
+
+def test_icesnow_is_idempotent_with_reset():
+    c, o, st = _ice_case(5)
+    q = o.icesnow(st, np.zeros((c.jmt, c.imt)), reset=True)
+    assert np.any(q)
+    at1 = st["at"].copy()
+    q2 = o.icesnow(st, q, reset=True)
+    np.testing.assert_array_equal(st["at"], at1)
+    assert not np.any(q2)
