@@ -220,6 +220,46 @@ def cpu_baseline(args):
                       f"workload, median of {len(times)} steps after 1 warm-up"}
 
 
+def run_icesnow(model, args, torch, lt, case, plain):
+    """icesnow_ at the workload size: the stand-alone licom_icesnow (every
+    column; algorithmic bytes: T at every level, kmt, surface S, licomqice
+    written) and the step with LICOM_STEP_ICESNOW against the plain step
+    (`plain`: the timed FP64 run)."""
+    n2 = case.imt * case.jmt
+    alg = n2 * (8 * case.km + 4 + 8 + 8)
+    model.icesnow(reset=True)
+    model.sync()
+    model.kernel_times()
+    model.set_profiling(True)
+    reps = 10
+    for _ in range(reps):
+        model.icesnow(reset=True)
+    kt = model.kernel_times()
+    ms = kt["k_icesnow"][0] / kt["k_icesnow"][1]
+    flags = lt.LICOM_STEP_SIDE_EFFECTS | lt.LICOM_STEP_CONVADJ | lt.LICOM_STEP_ICESNOW
+    model.tracer(0, flags)
+    model.sync()
+    model.kernel_times()
+    steps = max(2, args.steps // 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        model.tracer(0, flags)
+    model.sync()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    kf = model.kernel_times()
+    model.set_profiling(False)
+    out = {"standalone_ms": round(ms, 4), "algorithmic_bytes": alg,
+           "standalone_alg_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+           "fused_step_ms": round(1000 * sec, 3), "plain_step_ms": round(plain["ms_per_step"], 3)}
+    if "k_solve" in kf and "k_solve" in plain["kernel_ms"]:
+        out["fused_k_solve_ms"] = round(kf["k_solve"][0] / kf["k_solve"][1], 4)
+        out["plain_k_solve_ms"] = plain["kernel_ms"]["k_solve"]
+    print(f"[bench] icesnow: {ms:.3f} ms alone, step {1000 * sec:.2f} ms fused", file=sys.stderr, flush=True)
+    return out
+
+
 def run_variant(model, args, torch, lt, case, updates):
     """Warm-up, then K timed steps of one precision form; per-kernel HIP-event
     times on the library's stream over the timed steps."""
@@ -405,6 +445,7 @@ def main():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-error-study", action="store_true")
+    ap.add_argument("--no-icesnow", action="store_true")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--ensemble-replays", type=int, default=3)
     ap.add_argument("--no-serving", action="store_true")
@@ -471,6 +512,9 @@ def main():
                      "rms_dS_psu": rms[1], "wet_fraction_dT_above_1e-4": frac[0],
                      "wet_fraction_dS_above_1e-4": frac[1],
                      "steps": args.warmup + args.steps, "against": "FP64 drop-in"}
+    ice = None
+    if "fp64" in models and not args.no_icesnow and not args.no_kernel_timing:
+        ice = run_icesnow(models["fp64"], args, torch, lt, case, runs["fp64"])
     for m in models.values():
         m.close()
     del models
@@ -525,6 +569,8 @@ def main():
                         "roofline": rm["roofline"], "step_roofline": rm["step_roofline"],
                         "kernel_ms": rm["kernel_ms"], "kernel_alg_gbs": rm["kernel_alg_gbs"],
                         "error": mixed_err, "error_100_steps_1deg": err_study}
+    if ice:
+        out["icesnow"] = ice
     if ens_out:
         out["ensemble"] = ens_out
     if serve_out:

@@ -24,6 +24,7 @@ module licom_tracer_mod
    integer(c_int), parameter, public :: LICOM_STEP_SIDE_EFFECTS = 2
    integer(c_int), parameter, public :: LICOM_STEP_VTL_SCRATCH = 4
    integer(c_int), parameter, public :: LICOM_STEP_STAGED = 8
+   integer(c_int), parameter, public :: LICOM_STEP_ICESNOW = 16
    integer(c_int), parameter, public :: LICOM_HMIX_DEL2 = 2
    integer(c_int), parameter, public :: LICOM_HMIX_DEL4 = 4
    integer(c_int), parameter, public :: LICOM_ADV_TSPAS = 2
@@ -62,7 +63,8 @@ module licom_tracer_mod
    end type licom_diag
 
    public :: licom_tracer_create, licom_tracer_set_stream, licom_tracer_step, &
-             licom_convadj, licom_tracer_sync, licom_tracer_destroy
+             licom_convadj, licom_icesnow, licom_tracer_set_icesnow, licom_tracer_sync, &
+             licom_tracer_destroy
 
    interface
       integer(c_int) function licom_tracer_create(grid, params, ctx) bind(C, name="licom_tracer_create")
@@ -91,6 +93,18 @@ module licom_tracer_mod
          type(licom_fields), intent(in) :: fields
          type(licom_diag), intent(in) :: diag
       end function licom_convadj
+
+      ! icesnow_ (icesnow.cpp:70-85); at, licomqice are device addresses
+      integer(c_int) function licom_icesnow(ctx, at, licomqice, reset_qice) bind(C, name="licom_icesnow")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, at, licomqice
+         integer(c_int), value :: reset_qice
+      end function licom_icesnow
+
+      integer(c_int) function licom_tracer_set_icesnow(ctx, licomqice) bind(C, name="licom_tracer_set_icesnow")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx, licomqice
+      end function licom_tracer_set_icesnow
 
       integer(c_int) function licom_tracer_sync(ctx) bind(C, name="licom_tracer_sync")
          import :: c_int, c_ptr
