@@ -109,20 +109,45 @@ def test_step_with_icesnow_bitwise(flags, tag, need_gpu):
     m.close()
 
 
+def test_step_with_icesnow_km55_bitwise(need_gpu):
+    """The 55-level instantiation of the fused solve (the 0.1 deg grid's)."""
+    announce("C1-land km55 x2, fused icesnow + convadj")
+    case = cold_case(25, km=55, hmix=4, ah=-1.0e10)
+    o = orc.Oracle(case)
+    st = case.copy_state()
+    od = orc.Oracle.new_diag(case)
+    q = np.zeros((case.jmt, case.imt))
+    m = lt.TracerModel(case)
+    froze = 0
+    for s in range(2):
+        oracle_step(o, st, od, q, True)
+        m.tracer(s, SIDE | CONV | ICE)
+        assert mismatch(m.licomqice.cpu().numpy(), q) == 0, f"licomqice differs after step {s}"
+        froze += int(np.count_nonzero(q))
+    assert froze > 0
+    compare(m, st, od, "C1 km55 icesnow + convadj", exact=True, tol=1e-10)
+    m.close()
+
+
 def test_mixed_fused_icesnow_equals_icesnow_after_the_step(need_gpu):
     """The mixed form's fused icesnow gives the same bits as its step
     followed by the stand-alone icesnow (same arithmetic, other order)."""
-    announce("mixed C1-land x2: fused icesnow vs step then licom_icesnow")
-    case = cold_case(23)
+    announce("mixed C1-land km55 x2: fused icesnow vs step then licom_icesnow")
+    case = cold_case(23, km=55)
     a = lt.TracerModel(case, precision="mixed")
     b = lt.TracerModel(case, precision="mixed")
     qb = b.torch.zeros_like(b.licomqice)
     L = lt.lib()
+    froze = 0
     for s in range(2):
         a.tracer(s, SIDE | ICE)
         b.tracer(s, SIDE)
         lt.check(L.licom_icesnow(b.ctx, ctypes.c_void_p(b.state["at"].data_ptr()),
                                  ctypes.c_void_p(qb.data_ptr()), 1), "licom_icesnow")
+        qa = a.licomqice.cpu().numpy()
+        assert mismatch(qa, qb.cpu().numpy()) == 0, f"licomqice differs after step {s}"
+        froze += int(np.count_nonzero(qa))
+    assert froze > 0
     sa, sb = a.host_state(), b.host_state()
     da, db = a.host_diag(), b.host_diag()
     for k in ("at", "atb"):
@@ -130,8 +155,6 @@ def test_mixed_fused_icesnow_equals_icesnow_after_the_step(need_gpu):
         assert mismatch(sa[k], sb[k]) == 0, k
     for k in ("tend", "dt_diff", "ax", "dz"):
         assert mismatch(inner(da[k]), inner(db[k])) == 0, k
-    qa = a.licomqice.cpu().numpy()
-    assert np.count_nonzero(qa) > 0 and mismatch(qa, qb.cpu().numpy()) == 0
     a.close()
     b.close()
 
