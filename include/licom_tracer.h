@@ -251,6 +251,40 @@ int licom_tracer_set_icesnow(licom_ctx *ctx, double *licomqice);
 /* Waits for the context's stream. */
 int licom_tracer_sync(licom_ctx *ctx);
 
+/* ---- State I/O (host side; no context, no GPU) ----
+ * The restart stream that inirun.F90:598-700 reads: Fortran sequential
+ * unformatted records (4-byte little-endian length, payload, length) of the
+ * global grid nx x ny in double precision, i fastest, j north to south:
+ * h0; u, v, T, S, ws level by level; su, sv, swv, sshf, lthf, fresh; then
+ * (number_month, number_day).  It is the concatenation of the reference's
+ * fort.22 .. fort.27 (ssave-cdf-instant.F90:100-830, COUP off). */
+typedef struct licom_restart {
+    double *h0;                     /* [ny][nx]                              */
+    double *u, *v;                  /* [km][ny][nx]                          */
+    double *at;                     /* [2][km][ny][nx]: T, then S as (S-35)/1000 */
+    double *ws;                     /* [km][ny][nx]                          */
+    double *su, *sv, *swv, *sshf, *lthf, *fresh; /* [ny][nx]                 */
+    int32_t number_month, number_day;
+} licom_restart;
+
+/* Reads a restart into caller-allocated host arrays; a NULL array skips its
+ * records.  A record of another length, a bad trailing length or a short
+ * file gives LICOM_EINVAL with the record index in licom_last_error(). */
+int licom_restart_read(const char *path, int nx, int ny, int km, licom_restart *r);
+
+/* Writes the stream (every field required).  Land is written as 0, as the
+ * reference's writer zeroes it in place (ssave-cdf-instant.F90:173-191): h0,
+ * T, S and ws below kmt, u and v below kmu ([ny][nx] global masks); the
+ * caller's arrays are not modified. */
+int licom_restart_write(const char *path, int nx, int ny, int km, const int32_t *kmt,
+                        const int32_t *kmu, const licom_restart *r);
+
+/* Reads dncoef.h1 as constant_mod.F90:331-364 does (list-directed: a header
+ * line, km values of to; the same for so and po; per level a header line and
+ * C(k,1:9)) and scales C by 1000 (g/cm^3 -> kg/m^3): eosc is [9][km], the
+ * layout of licom_grid.eosc.  po may be NULL. */
+int licom_eos_read(const char *path, int km, double *to, double *so, double *po, double *eosc);
+
 /* Copies the derived masks and coefficients to host buffers. */
 int licom_grid_export(const licom_ctx *ctx, licom_grid_derived *out);
 

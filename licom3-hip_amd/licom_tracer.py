@@ -80,7 +80,17 @@ EXPORTED = ["licom_tracer_create", "licom_tracer_set_stream", "licom_tracer_step
             "licom_autotune_member", "licom_convadj", "licom_icesnow", "licom_tracer_set_icesnow",
             "licom_tracer_sync", "licom_grid_export", "licom_set_profiling",
             "licom_kernel_times", "licom_scratch_bytes", "licom_last_mix_count",
-            "licom_stream_copy", "licom_tracer_destroy", "licom_strerror", "licom_last_error"]
+            "licom_stream_copy", "licom_tracer_destroy", "licom_strerror", "licom_last_error",
+            "licom_restart_read", "licom_restart_write", "licom_eos_read"]
+
+
+class Restart(ctypes.Structure):
+    _fields_ = [(k, _DP) for k in ("h0", "u", "v", "at", "ws", "su", "sv", "swv", "sshf", "lthf",
+                                   "fresh")] + [("number_month", ctypes.c_int32),
+                                                ("number_day", ctypes.c_int32)]
+
+
+RESTART_2D = ("h0", "su", "sv", "swv", "sshf", "lthf", "fresh")
 
 
 class LicomError(RuntimeError):
@@ -123,6 +133,11 @@ def lib():
         L.licom_strerror.argtypes = [ctypes.c_int]
         L.licom_strerror.restype = ctypes.c_char_p
         L.licom_last_error.restype = ctypes.c_char_p
+        L.licom_restart_read.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(Restart)]
+        L.licom_restart_write.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          _IP, _IP, ctypes.POINTER(Restart)]
+        L.licom_eos_read.argtypes = [ctypes.c_char_p, ctypes.c_int, _DP, _DP, _DP, _DP]
         _LIB = L
     return _LIB
 
@@ -143,6 +158,56 @@ def _d(a):
 
 def _ptr(t):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+# -- state I/O (host side) ------------------------------------------------
+def read_restart(path, nx, ny, km):
+    """The restart stream of inirun.F90:598-700 as numpy arrays on the global
+    grid (h0 and the surface fields [ny][nx], u, v, ws [km][ny][nx], at
+    [2][km][ny][nx]) plus number_month, number_day."""
+    out = {k: np.zeros((ny, nx)) for k in RESTART_2D}
+    out.update(u=np.zeros((km, ny, nx)), v=np.zeros((km, ny, nx)), ws=np.zeros((km, ny, nx)),
+               at=np.zeros((2, km, ny, nx)))
+    R = Restart(**{k: _d(v) for k, v in out.items()})
+    check(lib().licom_restart_read(os.fsencode(path), nx, ny, km, ctypes.byref(R)),
+          "licom_restart_read")
+    out["number_month"], out["number_day"] = int(R.number_month), int(R.number_day)
+    return out
+
+
+def write_restart(path, fields, kmt, kmu, number_month=0, number_day=0):
+    """Writes `fields` (the keys read_restart returns) as a restart stream;
+    land (below the global kmt / kmu [ny][nx]) is written as 0."""
+    at = np.ascontiguousarray(fields["at"], np.float64)
+    _, km, ny, nx = at.shape
+    arrs = {k: np.ascontiguousarray(fields[k], np.float64) for k in RESTART_2D + ("u", "v", "ws")}
+    arrs["at"] = at
+    R = Restart(**{k: _d(v) for k, v in arrs.items()}, number_month=int(number_month),
+                number_day=int(number_day))
+    kt = np.ascontiguousarray(kmt, np.int32)
+    ku = np.ascontiguousarray(kmu, np.int32)
+    check(lib().licom_restart_write(os.fsencode(path), nx, ny, km, kt.ctypes.data_as(_IP),
+                                    ku.ctypes.data_as(_IP), ctypes.byref(R)), "licom_restart_write")
+
+
+def read_eos(path, km):
+    """dncoef.h1 (constant_mod.F90:331-364): to, so, po [km] and the density
+    coefficients [9][km] in kg/m^3 (C scaled by 1000), licom_grid.eosc."""
+    to, so, po, c = np.zeros(km), np.zeros(km), np.zeros(km), np.zeros((9, km))
+    check(lib().licom_eos_read(os.fsencode(path), km, _d(to), _d(so), _d(po), _d(c)), "licom_eos_read")
+    return to, so, po, c
+
+
+def global_to_halo(a):
+    """A global-grid array [..., ny, nx] in the engine's layout [..., ny+4,
+    nx+4]: two ghost cells, cyclic in i, 0 in the closed rows, as the
+    reference's halo update after a restart read (inirun.F90:616-667)."""
+    a = np.asarray(a, np.float64)
+    out = np.zeros(a.shape[:-2] + (a.shape[-2] + 4, a.shape[-1] + 4))
+    out[..., 2:-2, 2:-2] = a
+    out[..., 2:-2, 0:2] = a[..., :, -2:]
+    out[..., 2:-2, -2:] = a[..., :, 0:2]
+    return out
 
 
 def make_grid(case, keep):
