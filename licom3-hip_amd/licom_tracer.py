@@ -9,10 +9,16 @@ device allocations and for stream handles).
 
 * ``tracer(ii)``  -> ``extern "C" void tracer_(int ii)``  (tracer.cpp:1701)
 * ``convadj()``   -> ``extern "C" void convadj_()``        (convadj.cpp:290)
-* ``stepon(n)``   -> the tracer/convadj part of ``stepon_`` (stepon.cpp:57-77)
+* ``icesnow()``   -> ``extern "C" void icesnow_()``        (icesnow.cpp:70)
+* ``stepon(n)``   -> the tracer/icesnow/convadj part of ``stepon_``
+  (stepon.cpp:57-77)
 
-There is no CPU fallback: if the library or a GPU is missing every entry
-point raises.
+State I/O on the host (no GPU): ``read_restart`` / ``write_restart`` (the
+restart stream of inirun.F90:598-700), ``read_eos`` (dncoef.h1,
+constant_mod.F90:331-364) and ``global_to_halo``.
+
+There is no CPU fallback for the step: if the library or a GPU is missing
+every model entry point raises.
 """
 from __future__ import annotations
 
