@@ -49,17 +49,6 @@ void halo_fill(std::vector<T> &a, int imt, int jmt) {
 }
 
 template <class T>
-int upload(licom_ctx *c, T **dst, const T *src, size_t n) {
-    void *p = nullptr;
-    int rc = hip_status(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
-    if (rc) return rc;
-    c->allocs.push_back(p);
-    *dst = static_cast<T *>(p);
-    if (src) return hip_status(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
-    return LICOM_OK;
-}
-
-template <class T>
 int scratch(licom_ctx *c, T **dst, size_t n) {
     void *p = nullptr;
     int rc = hip_status(hipMalloc(&p, n * sizeof(T)), "hipMalloc scratch");
@@ -252,17 +241,55 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
     g.jmt = c->jmt;
     g.km = km;
     std::vector<double> quarter(n2, 0.25), pen0(km > 1 ? km - 1 : 1, 0.0);
+    // The static fields and the step scratch share one device allocation: a
+    // context costs one hipMalloc and one clear instead of ~36 allocations
+    // with a memset each (context creation is on the request loop's
+    // admission path).  Parts are planned here and placed by place_parts.
+    struct Part {
+        void *slot;             // address of the pointer to set
+        const void *src;        // host data to copy, or NULL: cleared scratch
+        size_t bytes;
+    };
+    std::vector<Part> parts;
     auto up_d = [&](const double **d, const double *h, size_t n) {
-        double *p = nullptr;
-        int r = upload(c, &p, h, n);
-        *d = p;
-        return r;
+        parts.push_back({(void *)d, h, n * sizeof(double)});
+        return 0;
     };
     auto up_i = [&](const int32_t **d, const int32_t *h, size_t n) {
-        int32_t *p = nullptr;
-        int r = upload(c, &p, h, n);
-        *d = p;
-        return r;
+        parts.push_back({(void *)d, h, n * sizeof(int32_t)});
+        return 0;
+    };
+    auto plan_scratch = [&](auto **d, size_t n) {
+        const size_t b = n * sizeof(**d);
+        parts.push_back({(void *)d, nullptr, b});
+        c->scratch_bytes += b;
+        return 0;
+    };
+    auto place_parts = [&]() -> int {
+        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+        size_t total = 0, zero_from = 0;
+        for (const Part &pt : parts) total += al(pt.bytes);
+        char *base = nullptr;
+        if (int r = hip_status(hipMalloc((void **)&base, total), "hipMalloc context")) return r;
+        c->allocs.push_back(base);
+        size_t off = 0;
+        for (const Part &pt : parts)   // copies first, then one clear of the scratch
+            if (pt.src) {
+                void *v = base + off;
+                std::memcpy(pt.slot, &v, sizeof v);
+                if (int r = hip_status(hipMemcpy(v, pt.src, pt.bytes, hipMemcpyHostToDevice), "hipMemcpy"))
+                    return r;
+                off += al(pt.bytes);
+            }
+        zero_from = off;
+        for (const Part &pt : parts)
+            if (!pt.src) {
+                void *v = base + off;
+                std::memcpy(pt.slot, &v, sizeof v);
+                off += al(pt.bytes);
+            }
+        return off > zero_from ? hip_status(hipMemset(base + zero_from, 0, off - zero_from), "hipMemset scratch")
+                               : (int)LICOM_OK;
     };
     rc = 0;
     rc = rc ? rc : up_i(&g.kmt, c->h_kmt.data(), n2);
@@ -311,17 +338,18 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
         const size_t cells = (size_t)(c->imt - 4) * (c->jmt - 4);
         s.nblk = (int)std::min<size_t>(1024, std::max<size_t>(1, (cells + 2047) / 2048));
     }
-    rc = rc ? rc : scratch(c, &s.stf_pre, n2);
-    rc = rc ? rc : scratch(c, &s.work_u, n2);
-    rc = rc ? rc : scratch(c, &s.stf_n, 2 * n2);
-    rc = rc ? rc : scratch(c, &s.temp11, n2);
-    rc = rc ? rc : scratch(c, &s.partials, (size_t)2 * s.nblk);
-    rc = rc ? rc : scratch(c, &s.err_norm2, 1);
-    rc = rc ? rc : scratch(c, &s.uw, n3);
-    rc = rc ? rc : scratch(c, &s.vs, n3);
-    rc = rc ? rc : scratch(c, &s.ori, 2 * n3);
-    rc = rc ? rc : scratch(c, &s.mix_count, 1);
-    rc = rc ? rc : scratch(c, &s.mix_list, n2);
+    rc = rc ? rc : plan_scratch(&s.stf_pre, n2);
+    rc = rc ? rc : plan_scratch(&s.work_u, n2);
+    rc = rc ? rc : plan_scratch(&s.stf_n, 2 * n2);
+    rc = rc ? rc : plan_scratch(&s.temp11, n2);
+    rc = rc ? rc : plan_scratch(&s.partials, (size_t)2 * s.nblk);
+    rc = rc ? rc : plan_scratch(&s.err_norm2, 1);
+    rc = rc ? rc : plan_scratch(&s.uw, n3);
+    rc = rc ? rc : plan_scratch(&s.vs, n3);
+    rc = rc ? rc : plan_scratch(&s.ori, 2 * n3);
+    rc = rc ? rc : plan_scratch(&s.mix_count, 1);
+    rc = rc ? rc : plan_scratch(&s.mix_list, n2);
+    rc = rc ? rc : place_parts();
     if (rc) {
         licom_tracer_destroy(c);
         return rc;

@@ -235,8 +235,11 @@ class RequestLoop:
     (`class_cells` bounds); each class is a batch on a stream of its own, so a
     small member never waits for the steps of a large one, and the classes'
     kernels share the GPU.  The host loop polls: it admits every arrived
-    request whose HBM estimate fits under the cap (arrival order) and uploads
-    its state; for each class whose last chunk finished it retires finished
+    request whose HBM estimate fits under the cap (arrival order) and hands it
+    to an admission thread, which creates its context and uploads its state on
+    a stream of its own (about 10 ms per member, which would otherwise stall
+    every class's next chunk); members join their class when their state is
+    resident.  For each class whose last chunk finished the loop retires finished
     members (results copied out), rebuilds the batch when the membership
     changed, and launches the next chunk of min(chunk_steps, fewest remaining
     steps).  Batches and member contexts of finished requests are freed when no
@@ -260,6 +263,9 @@ class RequestLoop:
         return len(self.class_cells)
 
     def run(self, requests, clock=time.perf_counter):
+        import queue
+        import threading
+
         import torch
         pending = sorted(requests, key=lambda r: r.arrival)
         done = []
@@ -269,8 +275,31 @@ class RequestLoop:
         i = 0
         peak_active = 0
         rebuilds = 0
-        lanes = None
+        dev = torch.cuda.current_device()
+        lanes = [_Lane(torch.cuda.Stream(device=dev)) for _ in range(len(self.class_cells) + 1)]
         grave, grave_bytes = [], 0.0
+        todo, ready, failures = queue.Queue(), queue.Queue(), []
+        inflight = 0
+
+        def admit_worker():
+            torch.cuda.set_device(dev)
+            side = torch.cuda.Stream(device=dev)
+            while True:
+                r = todo.get()
+                if r is None:
+                    return
+                try:
+                    with torch.cuda.stream(side):
+                        r.model = lt.TracerModel(r.case, diagnostics=True, precision="mixed", stream=side)
+                    side.synchronize()     # state resident before a class stream reads it
+                except Exception as e:     # handed to the loop, which raises it
+                    failures.append(e)
+                    ready.put(None)
+                    return
+                ready.put(r)
+
+        worker = threading.Thread(target=admit_worker, daemon=True)
+        worker.start()
 
         def flush():
             nonlocal grave, grave_bytes, used
@@ -284,20 +313,22 @@ class RequestLoop:
             while i < len(pending) and pending[i].arrival <= now and used + pending[i].bytes <= self.cap:
                 r = pending[i]
                 i += 1
-                r.model = lt.TracerModel(r.case, diagnostics=True, precision="mixed")
-                r.admitted = clock() - t0
                 used += r.bytes
-                if lanes is None:
-                    lanes = [_Lane(torch.cuda.Stream(device=r.model.device))
-                             for _ in range(len(self.class_cells) + 1)]
+                todo.put(r)
+                inflight += 1
+            while True:
+                try:
+                    r = ready.get_nowait()
+                except queue.Empty:
+                    break
+                if r is None:
+                    todo.put(None)
+                    raise failures[0]
+                inflight -= 1
+                r.admitted = clock() - t0
                 ln = lanes[self._class(r)]
                 ln.active.append(r)
                 ln.changed = True
-            if lanes is None:
-                if i >= len(pending):
-                    break
-                time.sleep(min(max(pending[i].arrival - (clock() - t0), 0.0), 0.005))
-                continue
             launched = False
             for ln in lanes:
                 if ln.busy:
@@ -348,7 +379,7 @@ class RequestLoop:
             if grave and (idle or grave_bytes > 0.25 * self.cap):
                 torch.cuda.synchronize()
                 flush()
-            if idle and not any(ln.active for ln in lanes):
+            if idle and not any(ln.active for ln in lanes) and inflight == 0:
                 if i >= len(pending):
                     break
                 wait = pending[i].arrival - (clock() - t0)
@@ -356,6 +387,8 @@ class RequestLoop:
                     time.sleep(min(wait, 0.005))
             elif not launched:
                 time.sleep(5e-5)
+        todo.put(None)
+        worker.join()
         torch.cuda.synchronize()
         flush()
         makespan = clock() - t0
