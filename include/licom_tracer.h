@@ -206,14 +206,16 @@ typedef struct licom_member_tuning {
 /* members[m] with its fields[m] and diags[m] (diags may be NULL, or an entry
  * with NULL pointers); tuning may be NULL (shape 0 everywhere).  All members
  * must live on the current device.  The batch owns a stream of its own
- * (default priority) unless licom_batch_set_stream is called. */
+ * (default priority) unless licom_batch_set_stream is called.  Each member's
+ * licomqice array (licom_tracer_set_icesnow) is taken at creation. */
 int licom_batch_create(licom_ctx *const *members, const licom_fields_mixed *fields,
                        const licom_diag_mixed *diags, const licom_member_tuning *tuning,
                        int n, licom_batch **out);
 int licom_batch_set_stream(licom_batch *b, void *hip_stream);
 /* Workgroups of the persistent k-march; 0 = one workgroup per item. */
 int licom_batch_set_march_grid(licom_batch *b, int workgroups);
-/* Enqueues nsteps steps of every member (flags: CONVADJ, SIDE_EFFECTS). */
+/* Enqueues nsteps steps of every member (flags: CONVADJ, SIDE_EFFECTS,
+ * ICESNOW when every member has a licomqice array). */
 int licom_batch_step(licom_batch *b, int nsteps, int flags);
 /* Captures nsteps steps into a HIP graph (replacing any previous one) and
  * launches nothing; licom_batch_replay launches the graph. */

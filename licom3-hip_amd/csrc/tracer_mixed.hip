@@ -860,6 +860,7 @@ struct MemberDev {
     double *stf_pre, *work_u, *stf_n, *temp11, *partials, *err_norm2, *ori;
     float *uw, *vs;
     int *mix_count;
+    double *qice;                     // licomqice of the fused icesnow, or NULL
     double area_t;
     int nblk;
     int sf_bx, sf_by;                 // 64 x 4 blocks of the 2-D kernels
@@ -867,7 +868,7 @@ struct MemberDev {
     int sv_shape, sv_bx, sv_by;       // column-solve blocks
     int ntx, ntiles;                  // BX x BY k-march tiles
     int side_vec, side_blocks, side2_blocks;
-    int halo_bx;                      // halo blocks per plane
+    int halo_bx;                      // halo blocks per plane (4 km planes + licomqice)
 };
 
 enum { ST_SURF, ST_SSSP, ST_SSSF, ST_FD, ST_WC /* unused */, ST_MARCH, ST_SOLVE, ST_SIDE3, ST_SIDE2, ST_HALO,
@@ -955,6 +956,16 @@ __global__ __launch_bounds__(256) void kb_solve(const MemberDev *M, const int *s
                                  d.sv_shape, nullptr);
 }
 
+// The same with icesnow between the update and convadj (LICOM_STEP_ICESNOW).
+template <int KM>
+__global__ __launch_bounds__(256, 2) void kb_solve_ice(const MemberDev *M, const int *starts, int nm,
+                                                    int flags) {
+    LICOM_MEMBER(ST_SOLVE)
+    solve_body<KM, float, float, true>(d.g, d.p, flags, d.at, d.atb, d.akt, d.ori, d.stf_n, d.tend,
+                                       d.dt_diff, d.dt_conv, d.mix_count, l % d.sv_bx, l / d.sv_bx,
+                                       d.sv_shape, d.qice);
+}
+
 __global__ __launch_bounds__(256) void kb_side3(const MemberDev *M, const int *starts, int nm) {
     LICOM_MEMBER(ST_SIDE3)
     const size_t n3 = (size_t)d.g.imt * d.g.jmt * d.g.km;
@@ -988,10 +999,14 @@ __global__ __launch_bounds__(256) void kb_side2(const MemberDev *M, const int *s
             d.akt[n * n3 + t] = (float)dev::floor_akt((double)d.akt[n * n3 + t], 0, n, d.p.hmix, d.p.dwndmix);
 }
 
-__global__ __launch_bounds__(256) void kb_halo(const MemberDev *M, const int *starts, int nm) {
+// Ghost cells of at/atb; plane 4 km is licomqice when icesnow ran.
+__global__ __launch_bounds__(256) void kb_halo(const MemberDev *M, const int *starts, int nm, int ice) {
     LICOM_MEMBER(ST_HALO)
-    halo_cell(d.g.imt, d.g.jmt, d.g.km, l / d.halo_bx, (l % d.halo_bx) * 256 + (int)threadIdx.x, d.at,
-              d.atb);
+    const int plane = l / d.halo_bx, t = (l % d.halo_bx) * 256 + (int)threadIdx.x;
+    if (plane < 4 * d.g.km)
+        halo_cell(d.g.imt, d.g.jmt, d.g.km, plane, t, d.at, d.atb);
+    else if (ice)
+        halo_cell(d.g.imt, d.g.jmt, d.g.km, 0, t, d.qice, nullptr);
 }
 
 #undef LICOM_MEMBER
@@ -1010,6 +1025,7 @@ struct licom_batch {
     int *d_starts = nullptr;
     int totals[licom::NSTAGE] = {};
     int march_grid = 0;
+    bool all_qice = false;            // every member has a licomqice array
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -1032,25 +1048,31 @@ int batch_launch_step(licom_batch *b, int flags) {
     const int items = b->totals[ST_MARCH];
     const int mg = b->march_grid > 0 ? std::min(b->march_grid, items) : items;
     kb_march<<<(unsigned)mg, NT, 0, st>>>(M, S, nm, items);
-    if (b->kmax <= 30) kb_solve<30><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
-    else if (b->kmax <= 55) kb_solve<55><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
-    else kb_solve<LICOM_MAX_KM><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
+    if (flags & LICOM_STEP_ICESNOW) {
+        if (b->kmax <= 30) kb_solve_ice<30><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
+        else if (b->kmax <= 55) kb_solve_ice<55><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
+        else kb_solve_ice<LICOM_MAX_KM><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
+    } else {
+        if (b->kmax <= 30) kb_solve<30><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
+        else if (b->kmax <= 55) kb_solve<55><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
+        else kb_solve<LICOM_MAX_KM><<<grid(ST_SOLVE), 256, 0, st>>>(M, S, nm, flags);
+    }
     if (flags & LICOM_STEP_SIDE_EFFECTS) {
         kb_side3<<<grid(ST_SIDE3), 256, 0, st>>>(M, S, nm);
         kb_side2<<<grid(ST_SIDE2), 256, 0, st>>>(M, S, nm);
     }
-    kb_halo<<<grid(ST_HALO), 256, 0, st>>>(M, S, nm);
+    kb_halo<<<grid(ST_HALO), 256, 0, st>>>(M, S, nm, (flags & LICOM_STEP_ICESNOW) ? 1 : 0);
     return hip_status(hipGetLastError(), "batch launch");
 }
 
-int check_batch_flags(int flags) {
+int check_batch_flags(const licom_batch *b, int flags) {
     if (flags & (LICOM_STEP_VTL_SCRATCH | LICOM_STEP_STAGED)) {
         licom::set_error("batch: VTL_SCRATCH and STAGED are FP64-only");
         return LICOM_ENOTSUP;
     }
-    if (flags & LICOM_STEP_ICESNOW) {
-        licom::set_error("batch: LICOM_STEP_ICESNOW is not built for batches");
-        return LICOM_ENOTSUP;
+    if ((flags & LICOM_STEP_ICESNOW) && !b->all_qice) {
+        licom::set_error("batch: LICOM_STEP_ICESNOW needs licom_tracer_set_icesnow on every member");
+        return LICOM_EINVAL;
     }
     return LICOM_OK;
 }
@@ -1136,6 +1158,7 @@ extern "C" int licom_batch_create(licom_ctx *const *members, const licom_fields_
         d.uw = reinterpret_cast<float *>(s.uw);
         d.vs = reinterpret_cast<float *>(s.vs);
         d.mix_count = s.mix_count;
+        d.qice = c->qice;
         d.area_t = c->area_t;
         d.nblk = s.nblk;
         const int imt = c->imt, jmt = c->jmt, km = c->km;
@@ -1166,7 +1189,7 @@ extern "C" int licom_batch_create(licom_ctx *const *members, const licom_fields_
                                   (long)d.sv_bx * d.sv_by,
                                   d.side_blocks,
                                   d.side2_blocks,
-                                  (long)d.halo_bx * 4 * km};
+                                  (long)d.halo_bx * (4 * km + 1)};
         for (int st = 0; st < NSTAGE; ++st) {
             const long next = starts[(size_t)st * (n + 1) + m] + cnt[st];
             if (next > 0x7fffffffL) {
@@ -1184,6 +1207,7 @@ extern "C" int licom_batch_create(licom_ctx *const *members, const licom_fields_
     if (!b) return LICOM_ENOMEM;
     b->n = n;
     b->kmax = kmax;
+    b->all_qice = std::all_of(hm.begin(), hm.end(), [](const MemberDev &d) { return d.qice != nullptr; });
     for (int st = 0; st < NSTAGE; ++st) b->totals[st] = starts[(size_t)st * (n + 1) + n];
     int rc = hip_status(hipMalloc(&b->d_members, sizeof(MemberDev) * n), "hipMalloc batch");
     if (!rc) rc = hip_status(hipMalloc(&b->d_starts, sizeof(int) * starts.size()), "hipMalloc batch");
@@ -1219,7 +1243,7 @@ extern "C" int licom_batch_set_march_grid(licom_batch *b, int workgroups) {
 
 extern "C" int licom_batch_step(licom_batch *b, int nsteps, int flags) {
     if (!b || nsteps < 0) return LICOM_EINVAL;
-    if (int rc = check_batch_flags(flags)) return rc;
+    if (int rc = check_batch_flags(b, flags)) return rc;
     for (int s = 0; s < nsteps; ++s)
         if (int rc = batch_launch_step(b, flags)) return rc;
     return LICOM_OK;
@@ -1228,7 +1252,7 @@ extern "C" int licom_batch_step(licom_batch *b, int nsteps, int flags) {
 extern "C" int licom_batch_capture(licom_batch *b, int nsteps, int flags) {
     using namespace licom;
     if (!b || nsteps <= 0) return LICOM_EINVAL;
-    if (int rc = check_batch_flags(flags)) return rc;
+    if (int rc = check_batch_flags(b, flags)) return rc;
     if (!b->stream) {
         set_error("licom_batch_capture: graph capture needs a non-default stream");
         return LICOM_EINVAL;

@@ -125,3 +125,32 @@ def test_request_loop_results_same_alone_and_in_the_mix(need_gpu):
     for r in reqs:
         ref = alone(r.case, r.n_steps)["at"]
         assert np.array_equal(loop.results[r.rid], ref), f"request {r.rid}"
+
+
+def test_batch_with_icesnow_equals_members_alone(need_gpu):
+    """LICOM_STEP_ICESNOW in a batch: each member's state, diagnostics and
+    licomqice equal its own single-model steps with the flag."""
+    from helpers import plant_cold
+    announce("ensemble: 3 members with icesnow batched vs alone, 3 steps")
+    flags = le.MIXED_FLAGS | lt.LICOM_STEP_ICESNOW
+    cs = cases()
+    for k, c in enumerate(cs):
+        plant_cold(c, c.state, 30 + k)
+    ref, qref = [], []
+    for c in cs:
+        m = lt.TracerModel(c, precision="mixed")
+        for s in range(3):
+            m.tracer(s, flags)
+        ref.append(outputs(m))
+        qref.append(m.licomqice.cpu().numpy())
+        m.close()
+    models = [lt.TracerModel(c, precision="mixed") for c in cs]
+    ens = le.Ensemble(models)
+    ens.step(3, flags)
+    ens.sync()
+    for k, m in enumerate(models):
+        assert_same(outputs(m), ref[k], f"member {k}")
+        q = m.licomqice.cpu().numpy()
+        assert np.array_equal(q, qref[k]), f"member {k}: licomqice"
+        m.close()
+    ens.close()
