@@ -226,6 +226,8 @@ class _Lane:
         self.busy = False
         self.chunk = 0
         self.event = None
+        self.single = False   # members one at a time (least remaining steps first)
+        self.running_set = []
 
 
 class RequestLoop:
@@ -247,8 +249,13 @@ class RequestLoop:
     than a quarter of the cap.  Latency = completion - arrival."""
 
     def __init__(self, hbm_cap_bytes=200e9, chunk_steps=8, tuner=None, keep_results=False,
-                 class_cells=(5.0e5, 2.0e6)):
+                 class_cells=(5.0e5, 2.0e6), one_at_a_time=(), one_at_a_time_upto=None):
         self.cap = float(hbm_cap_bytes)
+        # classes whose members run one at a time, least remaining steps first,
+        # instead of advancing together, while at most `one_at_a_time_upto`
+        # requests are active (None: always)
+        self.one_at_a_time = frozenset(one_at_a_time)
+        self.one_upto = one_at_a_time_upto
         self.chunk = int(chunk_steps)
         self.tuner = tuner
         self.keep_results = keep_results
@@ -277,6 +284,8 @@ class RequestLoop:
         rebuilds = 0
         dev = torch.cuda.current_device()
         lanes = [_Lane(torch.cuda.Stream(device=dev)) for _ in range(len(self.class_cells) + 1)]
+        for k, ln in enumerate(lanes):
+            ln.single = k in self.one_at_a_time
         grave, grave_bytes = [], 0.0
         todo, ready, failures = queue.Queue(), queue.Queue(), []
         inflight = 0
@@ -360,14 +369,19 @@ class RequestLoop:
                     ln.active = [r for r in ln.active if r.finished is None]
                     ln.running = []
                 if not ln.busy and ln.active:
-                    if ln.changed or ln.ens is None:
+                    one = ln.single and (self.one_upto is None or
+                                         sum(len(x.active) for x in lanes) <= self.one_upto)
+                    run = ([min(ln.active, key=lambda r: (r.n_steps - r.done_steps, r.arrival))]
+                           if one else list(ln.active))
+                    if ln.changed or ln.ens is None or run != ln.running_set:
                         if ln.ens is not None:
                             grave.append((ln.ens, 0.0))
-                        tuning = [self.tuner.tune(r.case) for r in ln.active] if self.tuner else None
-                        ln.ens = Ensemble([r.model for r in ln.active], tuning=tuning, stream=ln.stream)
+                        tuning = [self.tuner.tune(r.case) for r in run] if self.tuner else None
+                        ln.ens = Ensemble([r.model for r in run], tuning=tuning, stream=ln.stream)
                         ln.changed = False
+                        ln.running_set = run
                         rebuilds += 1
-                    ln.running = list(ln.active)
+                    ln.running = run
                     ln.chunk = min(self.chunk, min(r.n_steps - r.done_steps for r in ln.running))
                     ln.ens.step(ln.chunk)
                     ln.event = torch.cuda.Event()
