@@ -246,7 +246,10 @@ class RequestLoop:
     changed, and launches the next chunk of min(chunk_steps, fewest remaining
     steps).  Batches and member contexts of finished requests are freed when no
     class is running (a free synchronises the device) or when they hold more
-    than a quarter of the cap.  Latency = completion - arrival."""
+    than a quarter of the cap.  Classes listed in `one_at_a_time` run one
+    member at a time, least remaining steps first, instead of advancing their
+    members together (shorter tails at low load, less throughput at high
+    load).  Latency = completion - arrival."""
 
     def __init__(self, hbm_cap_bytes=200e9, chunk_steps=8, tuner=None, keep_results=False,
                  class_cells=(5.0e5, 2.0e6), one_at_a_time=(), one_at_a_time_upto=None):
