@@ -188,3 +188,40 @@ def test_icesnow_flag_needs_an_output_array(need_gpu):
     with pytest.raises(lt.LicomError):
         m.tracer(0, SIDE | ICE)
     m.close()
+
+
+def test_full_size_fused_icesnow_equals_icesnow_after_the_step(need_gpu):
+    """At the 0.1 deg workload size (3600x2302x55, del4, dt=600 s), with 2% of
+    the surface planted below the freezing point: the step with
+    LICOM_STEP_ICESNOW (no convadj) gives the same bits as the plain step
+    followed by licom_icesnow (at, atb, licomqice), and T >= TBICE on every
+    cell afterwards."""
+    import torch
+    announce("full size 3600x2302x55: fused icesnow vs step then licom_icesnow")
+    case = ls.make_case("C3")
+    rng = np.random.default_rng(26)
+    T0 = case.state["at"][0][0]
+    cold = rng.random(T0.shape) < 0.02
+    T0[cold] = TBICE - rng.uniform(0.01, 1.2, int(cold.sum()))
+    m = lt.TracerModel(case, diagnostics=False)
+    case.state = None
+    snap = {k: v.clone() for k, v in m.state.items()}
+    m.tracer(0, SIDE | ICE)
+    m.sync()
+    a_at, a_atb, a_q = m.state["at"].clone(), m.state["atb"].clone(), m.licomqice.clone()
+    for k, v in snap.items():
+        m.state[k].copy_(v)
+    del snap
+    m.tracer(0, SIDE)
+    m.icesnow(reset=True)
+    m.sync()
+    same = [torch.equal(a_at, m.state["at"]), torch.equal(a_atb, m.state["atb"]),
+            torch.equal(a_q, m.licomqice)]
+    frozen = int((a_q > 0).sum())
+    tmin = float(a_at[0].min())
+    print(f"[full size ice] same at/atb/licomqice {same}, {frozen} columns froze, min T {tmin:.6f}",
+          flush=True)
+    m.close()
+    del a_at, a_atb, a_q, m
+    torch.cuda.empty_cache()
+    assert all(same) and frozen > 0 and tmin >= TBICE
