@@ -342,12 +342,16 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
                     buf_store(plane(Tb, 8, k, pb8), o8, xs);
                     buf_store(plane(Bb, 8, k, pb8), o8, x);
                 } else if (act && k < km) {
+                    // non-temporal: the L2 otherwise fills each written line
+                    // from HBM before merging the partial writes (the FP64
+                    // solve fetched ~32 GB more than it reads at 0.1 deg);
+                    // 8% faster here, no change for the FP32 buffer stores
                     const size_t c = n * n3 + (size_t)k * n2 + q;
-                    if (dt_diff) dt_diff[c] = (DG)v;
-                    if (tend) tend[c] = (DG)tn;
-                    if (conv && dt_conv) dt_conv[c] = (DG)0.0;
-                    Tb[(size_t)k * n2 + q] = xs;
-                    Bb[(size_t)k * n2 + q] = x;
+                    if (dt_diff) __builtin_nontemporal_store((DG)v, &dt_diff[c]);
+                    if (tend) __builtin_nontemporal_store((DG)tn, &tend[c]);
+                    if (conv && dt_conv) __builtin_nontemporal_store((DG)0.0, &dt_conv[c]);
+                    __builtin_nontemporal_store(xs, &Tb[(size_t)k * n2 + q]);
+                    __builtin_nontemporal_store(x, &Bb[(size_t)k * n2 + q]);
                 }
                 if (conv) {
                     const double xo = __shfl_xor(xs, 32);
