@@ -683,6 +683,7 @@ void orc_tracer_step(const orc_grid *g, const orc_params *p, orc_fields *f,
                         tf[C3(k)] = tf[C3(k)] + vit[C3(k)] * (R[C3(k)] - AB(k)) * lamda;
                     }
         }
+        if (d->tf) memcpy(d->tf + (size_t)n * n3, tf, n3 * sizeof(double));
         /* tracer_8 + invtrit (tracer.cpp:1629-1653, 923-978) */
 #pragma omp parallel for schedule(static)
         for (int j = 2; j < jmt - 2; ++j)

@@ -78,6 +78,9 @@ typedef struct orc_diag {
     double *penetrate;                    /* [km][jmt][imt]                */
     double *net;                          /* [2][jmt][imt]                 */
     double fw_norm2;                      /* SSSNORM correction applied    */
+    double *tf;                           /* [2][km][jmt][imt] or NULL: the
+                                             explicit tendency entering
+                                             tracer_8 (for the pin tests)  */
 } orc_diag;
 
 void orc_halo_fill_f64(double *a, int imt, int jmt, int nz);

@@ -50,7 +50,8 @@ class OrcFields(ctypes.Structure):
 class OrcDiag(ctypes.Structure):
     _fields_ = [("ax", _DP), ("ay", _DP), ("az", _DP), ("dz", _DP),
                 ("tend", _DP), ("dt_diff", _DP), ("dt_conv", _DP),
-                ("penetrate", _DP), ("net", _DP), ("fw_norm2", ctypes.c_double)]
+                ("penetrate", _DP), ("net", _DP), ("fw_norm2", ctypes.c_double),
+                ("tf", _DP)]
 
 
 def lib():
