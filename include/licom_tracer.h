@@ -51,7 +51,10 @@ enum licom_adv { LICOM_ADV_TSPAS = 2 };
                                        (tracer.cpp:1088,1107-1108,1187,1315) */
 #define LICOM_STEP_VTL_SCRATCH 0x4  /* leave the new S in vtl, as the reference
                                        does with its scratch reuse
-                                       (tracer.cpp:1638,1891)                 */
+                                       (tracer.cpp:1638,1891): the S of
+                                       tracer_, before convadj and icesnow
+                                       (with CONVADJ, convadj then runs as a
+                                       separate pass over every column)       */
 #define LICOM_STEP_STAGED 0x8       /* run one kernel per reference stage
                                        instead of the fused passes (same bits;
                                        a cross-check, not a fast path)        */

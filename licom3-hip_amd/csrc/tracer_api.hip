@@ -396,6 +396,13 @@ int licom_tracer_step(licom_ctx *c, const licom_fields *f, const licom_diag *d, 
         // the S left in vtl is the one before icesnow: the per-stage path
         // runs icesnow after that copy
         if (flags & LICOM_STEP_VTL_SCRATCH) flags |= LICOM_STEP_STAGED;
+    } else if ((flags & LICOM_STEP_VTL_SCRATCH) && (flags & LICOM_STEP_CONVADJ)) {
+        // tracer_ leaves the new S in vtl before convadj_ mixes at and atb
+        // (tracer.cpp:1880-1893; convadj.cpp never touches vtl): step
+        // without convadj, so that the copy into vtl sees the unmixed S,
+        // then convadj over every column as convadj_() does
+        if (int rc = licom_tracer_step(c, f, d, flags & ~LICOM_STEP_CONVADJ)) return rc;
+        return launch_convadj(c, f, d);
     }
     if (flags & LICOM_STEP_STAGED) {
         // scratch of the per-stage kernels, allocated on first use

@@ -202,7 +202,7 @@ class Autotuner:
 
 class Request:
     __slots__ = ("rid", "arrival", "case", "n_steps", "done_steps", "model", "admitted",
-                 "finished", "bytes")
+                 "finished", "bytes", "error")
 
     def __init__(self, rid, arrival, case, n_steps):
         self.rid, self.arrival, self.case, self.n_steps = rid, arrival, case, int(n_steps)
@@ -211,6 +211,7 @@ class Request:
         self.admitted = None
         self.finished = None
         self.bytes = member_bytes(case.nx, case.ny, case.km)
+        self.error = None     # reason the request was rejected, if it was
 
 
 class _Lane:
@@ -249,10 +250,19 @@ class RequestLoop:
     than a quarter of the cap.  Classes listed in `one_at_a_time` run one
     member at a time, least remaining steps first, instead of advancing their
     members together (shorter tails at low load, less throughput at high
-    load).  Latency = completion - arrival."""
+    load).  Latency = completion - arrival.
+
+    Admission needs both the HBM estimate under the cap and, on the device,
+    that much free memory (hipMemGetInfo, less `free_margin_bytes`).  A
+    request that can never be admitted -- its estimate exceeds the cap, or it
+    exceeds the device's free memory while nothing else is resident -- is
+    rejected: it is not run, its `error` says why, and it is listed in
+    `rejected` (and counted in the statistics) instead of blocking the
+    requests behind it."""
 
     def __init__(self, hbm_cap_bytes=200e9, chunk_steps=8, tuner=None, keep_results=False,
-                 class_cells=(5.0e5, 2.0e6), one_at_a_time=(), one_at_a_time_upto=None):
+                 class_cells=(5.0e5, 2.0e6), one_at_a_time=(), one_at_a_time_upto=None,
+                 free_margin_bytes=2.0e9, free_bytes="device"):
         self.cap = float(hbm_cap_bytes)
         # classes whose members run one at a time, least remaining steps first,
         # instead of advancing together, while at most `one_at_a_time_upto`
@@ -264,6 +274,23 @@ class RequestLoop:
         self.keep_results = keep_results
         self.class_cells = tuple(class_cells)
         self.results = {}
+        self.rejected = []
+        self.free_margin = float(free_margin_bytes)
+        # device free-memory probe: "device" (hipMemGetInfo), a callable
+        # returning bytes, or None for no probe
+        self.free_bytes = free_bytes
+
+    def _reject(self, r, why):
+        r.error = why
+        self.rejected.append(r)
+
+    def _device_free(self):
+        if self.free_bytes is None:
+            return None
+        if self.free_bytes == "device":
+            import torch
+            return float(torch.cuda.mem_get_info()[0])
+        return float(self.free_bytes())
 
     def _class(self, r):
         cells = r.case.nx * r.case.ny * r.case.km
@@ -272,12 +299,17 @@ class RequestLoop:
                 return i
         return len(self.class_cells)
 
-    def run(self, requests, clock=time.perf_counter):
+    def run(self, requests, clock=time.perf_counter, sleep=time.sleep):
         import queue
         import threading
 
         import torch
-        pending = sorted(requests, key=lambda r: r.arrival)
+        pending = []
+        for r in sorted(requests, key=lambda r: r.arrival):
+            if r.bytes > self.cap:
+                self._reject(r, f"needs {r.bytes / 1e9:.2f} GB, above the {self.cap / 1e9:.2f} GB cap")
+            else:
+                pending.append(r)
         done = []
         used = 0.0
         t0 = clock()
@@ -285,13 +317,22 @@ class RequestLoop:
         i = 0
         peak_active = 0
         rebuilds = 0
-        dev = torch.cuda.current_device()
-        lanes = [_Lane(torch.cuda.Stream(device=dev)) for _ in range(len(self.class_cells) + 1)]
-        for k, ln in enumerate(lanes):
-            ln.single = k in self.one_at_a_time
+        lanes = []            # size classes, created with the first admission
+        dev = None
         grave, grave_bytes = [], 0.0
         todo, ready, failures = queue.Queue(), queue.Queue(), []
         inflight = 0
+        worker = None
+
+        def start_lanes():
+            nonlocal dev, worker
+            dev = torch.cuda.current_device()
+            for k in range(len(self.class_cells) + 1):
+                ln = _Lane(torch.cuda.Stream(device=dev))
+                ln.single = k in self.one_at_a_time
+                lanes.append(ln)
+            worker = threading.Thread(target=admit_worker, daemon=True)
+            worker.start()
 
         def admit_worker():
             torch.cuda.set_device(dev)
@@ -310,9 +351,6 @@ class RequestLoop:
                     return
                 ready.put(r)
 
-        worker = threading.Thread(target=admit_worker, daemon=True)
-        worker.start()
-
         def flush():
             nonlocal grave, grave_bytes, used
             for obj, nbytes in grave:
@@ -324,6 +362,16 @@ class RequestLoop:
             now = clock() - t0
             while i < len(pending) and pending[i].arrival <= now and used + pending[i].bytes <= self.cap:
                 r = pending[i]
+                free = self._device_free()
+                if free is not None and r.bytes > free - self.free_margin:
+                    if used == 0.0 and not grave and inflight == 0:
+                        self._reject(r, f"needs {r.bytes / 1e9:.2f} GB, the device has "
+                                        f"{free / 1e9:.2f} GB free")
+                        i += 1
+                        continue
+                    break        # waits for resident members to finish
+                if not lanes:
+                    start_lanes()
                 i += 1
                 used += r.bytes
                 todo.put(r)
@@ -400,17 +448,20 @@ class RequestLoop:
                 if i >= len(pending):
                     break
                 wait = pending[i].arrival - (clock() - t0)
-                if wait > 0:
-                    time.sleep(min(wait, 0.005))
+                # never spin: an arrived request that does not fit yet waits
+                # for frees, a future one for its arrival
+                sleep(min(wait, 0.005) if wait > 0 else 5e-4)
             elif not launched:
-                time.sleep(5e-5)
-        todo.put(None)
-        worker.join()
-        torch.cuda.synchronize()
+                sleep(5e-5)
+        if worker is not None:
+            todo.put(None)
+            worker.join()
+            torch.cuda.synchronize()
         flush()
         makespan = clock() - t0
         lat = np.array([r.finished - r.arrival for r in done])
-        return {"requests": len(done), "makespan_s": makespan, "cell_updates": updates,
+        return {"requests": len(done), "rejected": len(self.rejected), "makespan_s": makespan,
+                "cell_updates": updates,
                 "cell_updates_per_s": updates / makespan if makespan > 0 else 0.0,
                 "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
                 "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,

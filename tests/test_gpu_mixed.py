@@ -4,9 +4,10 @@ velocities, akt and diagnostics; FP64 implicit solve, convadj and sums).
 Errors are stated against the FP64 result on the same inputs: the oracle
 itself for small cases, and the FP64 GPU path (bit-identical to the oracle,
 see test_gpu_parity.py) for the 100-step run.  Bounds:
-  * after 100 steps (1 deg grid): max |dT|, max |dS| within 1e-4 (K, psu) or
-    within 3x of the FP64 path's own divergence from a 1e-8 K perturbation
-    of the initial state (limiter decisions flip at near-ties),
+  * after 100 steps (1 deg grid): max |dT|, max |dS| within 3x of the FP64
+    path's own divergence from a 1e-8 K perturbation of the initial state
+    (limiter decisions flip at near-ties); the 1e-4 (K, psu) target is its
+    own test, an expected failure at the measured 1.1e-3 K,
   * heat-content drift relative to the FP64 run <= 1e-9,
   * after one step, max |dT| <= 1e-6 K and the FP32 diagnostics within
     1e-4 of the FP64 ones (relative to each field's maximum)."""
@@ -96,15 +97,10 @@ def test_ten_steps_c1_with_forcing(need_gpu):
     assert eT <= 1e-5 and eS <= 1e-5
 
 
-def test_hundred_steps_c2_error_bound(need_gpu):
-    """100 steps on the 1 deg grid.  The tspas limiter switches a face between
-    the Lax-Wendroff and the upwind flux when the predictor crosses the local
-    bounds; a state perturbation of the size of one mixed step's error
-    (~1e-8 K) flips some of these decisions over 100 steps, and each flip
-    moves T by ~CFL * dT across the face.  The control run is the FP64 path
-    itself started from T perturbed by 1e-8 K: its divergence from the
-    unperturbed FP64 run is the scheme's own sensitivity, and the mixed form
-    must stay within 3x of it (and within 1e-4 K wherever the control does)."""
+@pytest.fixture(scope="module")
+def hundred_steps_c2(need_gpu):
+    """100 steps on the 1 deg grid in three runs: FP64, mixed, and an FP64
+    control started from T perturbed by 1e-8 K."""
     case = ls.make_case("C2")
     announce("mixed vs fp64 C2 100 steps, with a perturbed fp64 control")
     pcase = ls.make_case("C2")
@@ -131,9 +127,31 @@ def test_hundred_steps_c2_error_bound(need_gpu):
     drift = abs(h_mix - h_ref) / abs(h_ref)
     print(f"[mixed C2 x100] max|dT| {eT:.3e} K, max|dS| {eS:.3e} psu, heat drift {drift:.3e}; "
           f"fp64 control from T+1e-8 K: max|dT| {cT:.3e} K, max|dS| {cS:.3e} psu", flush=True)
-    assert eT <= max(1e-4, 3.0 * cT)
-    assert eS <= max(1e-4, 3.0 * cS)
-    assert drift <= 1e-9
+    return dict(eT=eT, eS=eS, cT=cT, cS=cS, drift=drift)
+
+
+def test_hundred_steps_c2_error_bound(hundred_steps_c2):
+    """The tspas limiter switches a face between the Lax-Wendroff and the
+    upwind flux when the predictor crosses the local bounds; a state
+    perturbation of the size of one mixed step's error (~1e-8 K) flips some
+    of these decisions over 100 steps, and each flip moves T by ~CFL * dT
+    across the face.  The control run's divergence from the unperturbed FP64
+    run is the scheme's own sensitivity; the mixed form must stay within 3x
+    of it, and its heat content within 1e-9 of the FP64 run's."""
+    r = hundred_steps_c2
+    assert r["eT"] <= 3.0 * r["cT"]
+    assert r["eS"] <= 3.0 * r["cS"]
+    assert r["drift"] <= 1e-9
+
+
+@pytest.mark.xfail(reason="max |dT| after 100 steps on the 1 deg grid is 1.1e-3 K (measured), "
+                          "above the 1e-4 K target: limiter decisions flip at near-ties",
+                   strict=False)
+def test_hundred_steps_c2_meets_the_1e4_target(hundred_steps_c2):
+    """The target as stated: max |dT| <= 1e-4 K and max |dS| <= 1e-4 psu
+    after 100 steps against FP64."""
+    r = hundred_steps_c2
+    assert r["eT"] <= 1e-4 and r["eS"] <= 1e-4
 
 
 def test_mixed_rejects_fp64_only_flags(need_gpu):
