@@ -258,8 +258,10 @@ int licom_tracer_sync(licom_ctx *ctx);
 
 /* ---- State I/O (host side; no context, no GPU) ----
  * The restart stream that inirun.F90:598-700 reads: Fortran sequential
- * unformatted records (4-byte little-endian length, payload, length) of the
- * global grid nx x ny in double precision, i fastest, j north to south:
+ * unformatted records (4-byte length, payload, length) of the global grid
+ * nx x ny in double precision, i fastest, j north to south, big-endian as
+ * the reference's -convert big_endian build (Makefile:4) reads and writes
+ * them (little_endian != 0: native order):
  * h0; u, v, T, S, ws level by level; su, sv, swv, sshf, lthf, fresh; then
  * (number_month, number_day).  It is the concatenation of the reference's
  * fort.22 .. fort.27 (ssave-cdf-instant.F90:100-830, COUP off). */
@@ -270,6 +272,8 @@ typedef struct licom_restart {
     double *ws;                     /* [km][ny][nx]                          */
     double *su, *sv, *swv, *sshf, *lthf, *fresh; /* [ny][nx]                 */
     int32_t number_month, number_day;
+    int32_t little_endian;          /* 0: big-endian records (the reference's
+                                       build); 1: little-endian             */
 } licom_restart;
 
 /* Reads a restart into caller-allocated host arrays; a NULL array skips its

@@ -3,7 +3,11 @@
 // writes it) and the EOS coefficient table dncoef.h1 (constant_mod.F90:331-364).
 //
 // The restart is a Fortran sequential unformatted stream: each record is a
-// 4-byte little-endian length, the payload, and the length again.  Records,
+// 4-byte length, the payload, and the length again.  The reference compiles
+// every Fortran unit with -convert big_endian (Makefile:4), so its record
+// lengths, doubles and integers are big-endian; that is the default here,
+// and licom_restart.little_endian selects the native order of a build
+// without that flag.  Records,
 // in order (inirun.F90:612-686): h0; u, v, T, S, ws one record per level
 // (km each); su, sv, swv, sshf, lthf, fresh; then (number_month, number_day)
 // as two default integers.  Every 2-D record is the global grid nx x ny in
@@ -32,19 +36,40 @@ struct File {
     }
 };
 
+// Byte order of one stream: big-endian unless the caller asks for native
+// little-endian (the host is little-endian x86-64).
+struct Order {
+    bool swap;
+    int32_t i32(int32_t v) const { return swap ? (int32_t)__builtin_bswap32((uint32_t)v) : v; }
+    void doubles(double *p, size_t n) const {
+        if (!swap) return;
+        for (size_t q = 0; q < n; ++q) {
+            uint64_t u;
+            std::memcpy(&u, p + q, 8);
+            u = __builtin_bswap64(u);
+            std::memcpy(p + q, &u, 8);
+        }
+    }
+    void ints(int32_t *p, size_t n) const {
+        for (size_t q = 0; q < n; ++q) p[q] = i32(p[q]);
+    }
+};
+
 int io_error(const char *what, const char *path) {
     set_error("%s: %s (%s)", what, path, errno ? std::strerror(errno) : "format");
     return LICOM_EINVAL;
 }
 
-// One record: payload of `bytes` bytes into dst (NULL: skipped).
-int read_record(FILE *f, const char *path, void *dst, size_t bytes, int index) {
+// One record: payload of `bytes` bytes into dst (NULL: skipped), in the
+// file's byte order (the caller converts the payload).
+int read_record(FILE *f, const char *path, void *dst, size_t bytes, int index, const Order &o) {
     int32_t head = 0, tail = 0;
     errno = 0;
     if (std::fread(&head, 4, 1, f) != 1) {
         set_error("restart %s: record %d missing (file ends early)", path, index);
         return LICOM_EINVAL;
     }
+    head = o.i32(head);
     if (head < 0 || (size_t)head != bytes) {
         set_error("restart %s: record %d holds %d bytes, expected %zu", path, index, (int)head, bytes);
         return LICOM_EINVAL;
@@ -54,15 +79,16 @@ int read_record(FILE *f, const char *path, void *dst, size_t bytes, int index) {
     } else if (std::fseek(f, (long)bytes, SEEK_CUR) != 0) {
         return io_error("restart: seek", path);
     }
-    if (std::fread(&tail, 4, 1, f) != 1 || tail != head) {
+    if (std::fread(&tail, 4, 1, f) != 1 || o.i32(tail) != head) {
         set_error("restart %s: record %d has a bad trailing length", path, index);
         return LICOM_EINVAL;
     }
     return LICOM_OK;
 }
 
-int write_record(FILE *f, const char *path, const void *src, size_t bytes) {
-    const int32_t n = (int32_t)bytes;
+// src is already in the file's byte order
+int write_record(FILE *f, const char *path, const void *src, size_t bytes, const Order &o) {
+    const int32_t n = o.i32((int32_t)bytes);
     if (std::fwrite(&n, 4, 1, f) != 1 || std::fwrite(src, 1, bytes, f) != bytes ||
         std::fwrite(&n, 4, 1, f) != 1)
         return io_error("restart: write", path);
@@ -151,8 +177,13 @@ int licom_restart_read(const char *path, int nx, int ny, int km, licom_restart *
     F.f = std::fopen(path, "rb");
     if (!F.f) return io_error("licom_restart_read: cannot open", path);
     const size_t n2 = (size_t)nx * ny, b2 = n2 * 8;
+    const Order o{r->little_endian == 0};
     int idx = 0;
-    auto rec2 = [&](double *dst) { return read_record(F.f, path, dst, b2, idx++); };
+    auto rec2 = [&](double *dst) {
+        const int rc = read_record(F.f, path, dst, b2, idx++, o);
+        if (rc == LICOM_OK && dst) o.doubles(dst, n2);
+        return rc;
+    };
     auto rec3 = [&](double *dst) {
         for (int k = 0; k < km; ++k)
             if (int rc = rec2(dst ? dst + (size_t)k * n2 : nullptr)) return rc;
@@ -168,7 +199,8 @@ int licom_restart_read(const char *path, int nx, int ny, int km, licom_restart *
     for (double *p : surf)
         if (int rc = rec2(p)) return rc;
     int32_t date[2] = {0, 0};
-    if (int rc = read_record(F.f, path, date, sizeof date, idx++)) return rc;
+    if (int rc = read_record(F.f, path, date, sizeof date, idx++, o)) return rc;
+    o.ints(date, 2);
     r->number_month = date[0];
     r->number_day = date[1];
     return LICOM_OK;
@@ -187,12 +219,14 @@ int licom_restart_write(const char *path, int nx, int ny, int km, const int32_t 
     F.f = std::fopen(path, "wb");
     if (!F.f) return io_error("licom_restart_write: cannot open", path);
     const size_t n2 = (size_t)nx * ny, b2 = n2 * 8;
+    const Order o{r->little_endian == 0};
     std::vector<double> buf(n2);
     // land is written as 0: h0 and the T-grid fields by vit, u and v by viv
     // (ssave-cdf-instant.F90:173-191); the caller's arrays are not modified
     auto masked = [&](const double *src, const int32_t *kz, int k) {
         for (size_t q = 0; q < n2; ++q) buf[q] = k < kz[q] ? src[q] : 0.0;
-        return write_record(F.f, path, buf.data(), b2);
+        o.doubles(buf.data(), n2);
+        return write_record(F.f, path, buf.data(), b2, o);
     };
     if (int rc = masked(r->h0, kmt, 0)) return rc;
     const double *f3[5] = {r->u, r->v, r->at, r->at + (size_t)km * n2, r->ws};
@@ -201,10 +235,14 @@ int licom_restart_write(const char *path, int nx, int ny, int km, const int32_t 
         for (int k = 0; k < km; ++k)
             if (int rc = masked(f3[a] + (size_t)k * n2, m3[a], k)) return rc;
     const double *surf[6] = {r->su, r->sv, r->swv, r->sshf, r->lthf, r->fresh};
-    for (const double *p : surf)
-        if (int rc = write_record(F.f, path, p, b2)) return rc;
-    const int32_t date[2] = {r->number_month, r->number_day};
-    if (int rc = write_record(F.f, path, date, sizeof date)) return rc;
+    for (const double *p : surf) {
+        std::memcpy(buf.data(), p, b2);
+        o.doubles(buf.data(), n2);
+        if (int rc = write_record(F.f, path, buf.data(), b2, o)) return rc;
+    }
+    int32_t date[2] = {r->number_month, r->number_day};
+    o.ints(date, 2);
+    if (int rc = write_record(F.f, path, date, sizeof date, o)) return rc;
     if (std::fflush(F.f) != 0) return io_error("licom_restart_write: flush", path);
     return LICOM_OK;
 }

@@ -93,7 +93,8 @@ EXPORTED = ["licom_tracer_create", "licom_tracer_set_stream", "licom_tracer_step
 class Restart(ctypes.Structure):
     _fields_ = [(k, _DP) for k in ("h0", "u", "v", "at", "ws", "su", "sv", "swv", "sshf", "lthf",
                                    "fresh")] + [("number_month", ctypes.c_int32),
-                                                ("number_day", ctypes.c_int32)]
+                                                ("number_day", ctypes.c_int32),
+                                                ("little_endian", ctypes.c_int32)]
 
 
 RESTART_2D = ("h0", "su", "sv", "swv", "sshf", "lthf", "fresh")
@@ -167,29 +168,37 @@ def _ptr(t):
 
 
 # -- state I/O (host side) ------------------------------------------------
-def read_restart(path, nx, ny, km):
+def _byte_order(byte_order):
+    if byte_order not in ("big", "little"):
+        raise LicomError(f"byte_order must be 'big' or 'little', not {byte_order!r}")
+    return int(byte_order == "little")
+
+
+def read_restart(path, nx, ny, km, byte_order="big"):
     """The restart stream of inirun.F90:598-700 as numpy arrays on the global
     grid (h0 and the surface fields [ny][nx], u, v, ws [km][ny][nx], at
-    [2][km][ny][nx]) plus number_month, number_day."""
+    [2][km][ny][nx]) plus number_month, number_day.  Records are big-endian
+    by default, as the reference's -convert big_endian build writes them."""
     out = {k: np.zeros((ny, nx)) for k in RESTART_2D}
     out.update(u=np.zeros((km, ny, nx)), v=np.zeros((km, ny, nx)), ws=np.zeros((km, ny, nx)),
                at=np.zeros((2, km, ny, nx)))
-    R = Restart(**{k: _d(v) for k, v in out.items()})
+    R = Restart(**{k: _d(v) for k, v in out.items()}, little_endian=_byte_order(byte_order))
     check(lib().licom_restart_read(os.fsencode(path), nx, ny, km, ctypes.byref(R)),
           "licom_restart_read")
     out["number_month"], out["number_day"] = int(R.number_month), int(R.number_day)
     return out
 
 
-def write_restart(path, fields, kmt, kmu, number_month=0, number_day=0):
-    """Writes `fields` (the keys read_restart returns) as a restart stream;
-    land (below the global kmt / kmu [ny][nx]) is written as 0."""
+def write_restart(path, fields, kmt, kmu, number_month=0, number_day=0, byte_order="big"):
+    """Writes `fields` (the keys read_restart returns) as a restart stream
+    (big-endian by default); land (below the global kmt / kmu [ny][nx]) is
+    written as 0."""
     at = np.ascontiguousarray(fields["at"], np.float64)
     _, km, ny, nx = at.shape
     arrs = {k: np.ascontiguousarray(fields[k], np.float64) for k in RESTART_2D + ("u", "v", "ws")}
     arrs["at"] = at
     R = Restart(**{k: _d(v) for k, v in arrs.items()}, number_month=int(number_month),
-                number_day=int(number_day))
+                number_day=int(number_day), little_endian=_byte_order(byte_order))
     kt = np.ascontiguousarray(kmt, np.int32)
     ku = np.ascontiguousarray(kmu, np.int32)
     check(lib().licom_restart_write(os.fsencode(path), nx, ny, km, kt.ctypes.data_as(_IP),

@@ -2,9 +2,12 @@
 100-830) and dncoef.h1 (constant_mod.F90:331-364), host side, no GPU.
 
 The reader is pinned against byte streams built here with `struct` from the
-Fortran sequential-unformatted rule (4-byte little-endian record length,
-payload, length) in the record order of inirun.F90, not by the library's own
-writer; the writer must then reproduce those bytes exactly."""
+Fortran sequential-unformatted rule (4-byte record length, payload, length)
+in the record order of inirun.F90, not by the library's own writer; the
+writer must then reproduce those bytes exactly.  The reference builds its
+Fortran with -convert big_endian (Makefile:4), so the streams are
+big-endian: record lengths '>i', doubles '>f8', the date '>ii'; the native
+little-endian form is an option."""
 import os
 import struct
 
@@ -15,20 +18,22 @@ import licom_synth as ls
 import licom_tracer as lt
 
 
-def fortran_records(payloads):
+def fortran_records(payloads, order=">"):
     out = bytearray()
     for p in payloads:
-        out += struct.pack("<i", len(p)) + p + struct.pack("<i", len(p))
+        out += struct.pack(order + "i", len(p)) + p + struct.pack(order + "i", len(p))
     return bytes(out)
 
 
-def restart_payloads(f, km, month, day):
+def restart_payloads(f, km, month, day, order=">"):
     """Records in the order inirun.F90:612-686 reads them."""
-    recs = [f["h0"].tobytes()]
+    dt = np.dtype(order + "f8")
+    b = lambda a: np.ascontiguousarray(a).astype(dt).tobytes()  # noqa: E731
+    recs = [b(f["h0"])]
     for a in (f["u"], f["v"], f["at"][0], f["at"][1], f["ws"]):
-        recs += [a[k].tobytes() for k in range(km)]
-    recs += [f[k].tobytes() for k in ("su", "sv", "swv", "sshf", "lthf", "fresh")]
-    recs.append(struct.pack("<ii", month, day))
+        recs += [b(a[k]) for k in range(km)]
+    recs += [b(f[k]) for k in ("su", "sv", "swv", "sshf", "lthf", "fresh")]
+    recs.append(struct.pack(order + "ii", month, day))
     return recs
 
 
@@ -49,6 +54,26 @@ def test_reader_takes_fortran_records_in_inirun_order(tmp_path):
     for k in f:
         np.testing.assert_array_equal(r[k], f[k], err_msg=k)
     assert (r["number_month"], r["number_day"]) == (7, 23)
+    # the same records in the wrong byte order are refused (record length)
+    with pytest.raises(lt.LicomError, match="record 0 holds"):
+        lt.read_restart(str(p), nx, ny, km, byte_order="little")
+
+
+def test_little_endian_option(tmp_path):
+    nx, ny, km = 7, 5, 3
+    f = random_fields(nx, ny, km, seed=6)
+    p = tmp_path / "le"
+    p.write_bytes(fortran_records(restart_payloads(f, km, 2, 28, "<"), "<"))
+    r = lt.read_restart(str(p), nx, ny, km, byte_order="little")
+    for k in f:
+        np.testing.assert_array_equal(r[k], f[k], err_msg=k)
+    assert (r["number_month"], r["number_day"]) == (2, 28)
+    kmt = np.full((ny, nx), km, np.int32)
+    q = tmp_path / "le2"
+    lt.write_restart(str(q), f, kmt, kmt, 2, 28, byte_order="little")
+    assert q.read_bytes() == p.read_bytes()
+    with pytest.raises(lt.LicomError):
+        lt.read_restart(str(p), nx, ny, km, byte_order="middle")
 
 
 def test_writer_reproduces_the_record_stream_and_masks_land(tmp_path):
