@@ -176,7 +176,54 @@ int check_ptr(const void *p, const char *name) {
     return LICOM_OK;
 }
 
+int check_fields(const licom_fields *f) {
+    const char *req[] = {"at", "atb", "akt", "utf", "vtf", "utl", "vtl", "ws", "h0f", "h0l"};
+    const void *ptrs[] = {f->at, f->atb, f->akt, f->utf, f->vtf, f->utl, f->vtl, f->ws, f->h0f, f->h0l};
+    for (int t = 0; t < 10; ++t)
+        if (int rc = check_ptr(ptrs[t], req[t])) return rc;
+    return LICOM_OK;
+}
+
 }  // namespace
+
+// Checks the arguments of an FP64 step and settles its path: STAGED is added
+// where the fused kernels cannot run the layout, and split is set when
+// convadj must run as a separate pass after the step.
+int plan_step(licom_ctx *c, const licom_fields *f, int &flags, bool &split, const char *who) {
+    split = false;
+    if (!c || !f) {
+        set_error("%s: NULL argument", who);
+        return LICOM_EINVAL;
+    }
+    if (int rc = check_fields(f)) return rc;
+    if (c->p.boundary_restore == 1 && !f->restore) {
+        set_error("boundary_restore=1 needs the restore field");
+        return LICOM_EINVAL;
+    }
+    // The fused k-march stages planes in 16-byte chunks: it needs an even imt
+    // and 16-byte aligned at, atb, ws and akt; other layouts take the
+    // per-stage kernels (same results).
+    const uintptr_t al = reinterpret_cast<uintptr_t>(f->at) | reinterpret_cast<uintptr_t>(f->atb) |
+                         reinterpret_cast<uintptr_t>(f->ws) | reinterpret_cast<uintptr_t>(f->akt);
+    if (c->imt % 2 != 0 || (al & 15) != 0) flags |= LICOM_STEP_STAGED;
+    if (flags & LICOM_STEP_ICESNOW) {
+        if (!c->qice) {
+            set_error("LICOM_STEP_ICESNOW needs licom_tracer_set_icesnow first");
+            return LICOM_EINVAL;
+        }
+        // the S left in vtl is the one before icesnow: the per-stage path
+        // runs icesnow after that copy
+        if (flags & LICOM_STEP_VTL_SCRATCH) flags |= LICOM_STEP_STAGED;
+    } else if ((flags & LICOM_STEP_VTL_SCRATCH) && (flags & LICOM_STEP_CONVADJ)) {
+        // tracer_ leaves the new S in vtl before convadj_ mixes at and atb
+        // (tracer.cpp:1880-1893; convadj.cpp never touches vtl): the step
+        // runs without convadj, so that the copy into vtl sees the unmixed
+        // S, then convadj over every column as convadj_() does
+        split = true;
+    }
+    return LICOM_OK;
+}
+
 }  // namespace licom
 
 using namespace licom;
@@ -348,7 +395,6 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
     rc = rc ? rc : plan_scratch(&s.vs, n3);
     rc = rc ? rc : plan_scratch(&s.ori, 2 * n3);
     rc = rc ? rc : plan_scratch(&s.mix_count, 1);
-    rc = rc ? rc : plan_scratch(&s.mix_list, n2);
     rc = rc ? rc : place_parts();
     if (rc) {
         licom_tracer_destroy(c);
@@ -364,56 +410,23 @@ int licom_tracer_set_stream(licom_ctx *c, void *stream) {
     return LICOM_OK;
 }
 
-static int check_fields(const licom_fields *f) {
-    const char *req[] = {"at", "atb", "akt", "utf", "vtf", "utl", "vtl", "ws", "h0f", "h0l"};
-    const void *ptrs[] = {f->at, f->atb, f->akt, f->utf, f->vtf, f->utl, f->vtl, f->ws, f->h0f, f->h0l};
-    for (int t = 0; t < 10; ++t)
-        if (int rc = check_ptr(ptrs[t], req[t])) return rc;
-    return LICOM_OK;
-}
-
 int licom_tracer_step(licom_ctx *c, const licom_fields *f, const licom_diag *d, int flags) {
-    if (!c || !f) {
-        set_error("licom_tracer_step: NULL argument");
-        return LICOM_EINVAL;
-    }
-    if (int rc = check_fields(f)) return rc;
-    if (c->p.boundary_restore == 1 && !f->restore) {
-        set_error("boundary_restore=1 needs the restore field");
-        return LICOM_EINVAL;
-    }
-    // The fused k-march stages planes in 16-byte chunks: it needs an even imt
-    // and 16-byte aligned at, atb, ws and akt; other layouts take the
-    // per-stage kernels (same results).
-    const uintptr_t al = reinterpret_cast<uintptr_t>(f->at) | reinterpret_cast<uintptr_t>(f->atb) |
-                         reinterpret_cast<uintptr_t>(f->ws) | reinterpret_cast<uintptr_t>(f->akt);
-    if (c->imt % 2 != 0 || (al & 15) != 0) flags |= LICOM_STEP_STAGED;
-    if (flags & LICOM_STEP_ICESNOW) {
-        if (!c->qice) {
-            set_error("LICOM_STEP_ICESNOW needs licom_tracer_set_icesnow first");
-            return LICOM_EINVAL;
-        }
-        // the S left in vtl is the one before icesnow: the per-stage path
-        // runs icesnow after that copy
-        if (flags & LICOM_STEP_VTL_SCRATCH) flags |= LICOM_STEP_STAGED;
-    } else if ((flags & LICOM_STEP_VTL_SCRATCH) && (flags & LICOM_STEP_CONVADJ)) {
-        // tracer_ leaves the new S in vtl before convadj_ mixes at and atb
-        // (tracer.cpp:1880-1893; convadj.cpp never touches vtl): step
-        // without convadj, so that the copy into vtl sees the unmixed S,
-        // then convadj over every column as convadj_() does
-        if (int rc = licom_tracer_step(c, f, d, flags & ~LICOM_STEP_CONVADJ)) return rc;
-        return launch_convadj(c, f, d);
-    }
-    if (flags & LICOM_STEP_STAGED) {
+    bool split = false;
+    if (int rc = plan_step(c, f, flags, split, "licom_tracer_step")) return rc;
+    const int run = split ? flags & ~LICOM_STEP_CONVADJ : flags;
+    int rc;
+    if (run & LICOM_STEP_STAGED) {
         // scratch of the per-stage kernels, allocated on first use
         if (!c->s.flags) {
-            int rc = scratch(c, &c->s.flags, 2 * c->n3);
+            rc = scratch(c, &c->s.flags, 2 * c->n3);
             if (!rc && c->p.hmix == LICOM_HMIX_DEL4) rc = scratch(c, &c->s.d2tk, 2 * c->n3);
             if (rc) return rc;
         }
-        return launch_step_simple(c, f, d, flags);
+        rc = launch_step_simple(c, f, d, run);
+    } else {
+        rc = launch_step_fused(c, f, d, run, c->s);
     }
-    return launch_step_fused(c, f, d, flags);
+    return (rc || !split) ? rc : launch_convadj(c, f, d);
 }
 
 int licom_tracer_step_mixed(licom_ctx *c, const licom_fields_mixed *f, const licom_diag_mixed *d,

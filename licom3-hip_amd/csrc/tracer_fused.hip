@@ -728,10 +728,9 @@ void launch_side(licom_ctx *ctx, const licom_fields *f) {
 
 template <int KM>
 void launch_prelude_solve(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags,
-                          bool prelude) {
+                          bool prelude, const Scratch &s) {
     const KGrid &g = ctx->g;
     const KPar &p = ctx->kp;
-    Scratch &s = ctx->s;
     hipStream_t st = ctx->stream;
     if (prelude) {
         Timer t(ctx, "k_prelude");
@@ -754,22 +753,22 @@ void launch_prelude_solve(licom_ctx *ctx, const licom_fields *f, const licom_dia
 }
 
 void launch_column(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags,
-                   bool prelude) {
+                   bool prelude, const Scratch &s) {
     switch (ctx->km) {
-        case 30: launch_prelude_solve<30>(ctx, f, d, flags, prelude); break;
-        case 55: launch_prelude_solve<55>(ctx, f, d, flags, prelude); break;
-        default: launch_prelude_solve<KMAXF>(ctx, f, d, flags, prelude); break;
+        case 30: launch_prelude_solve<30>(ctx, f, d, flags, prelude, s); break;
+        case 55: launch_prelude_solve<55>(ctx, f, d, flags, prelude, s); break;
+        default: launch_prelude_solve<KMAXF>(ctx, f, d, flags, prelude, s); break;
     }
 }
 
 }  // namespace
 
-int launch_step_fused(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags) {
+int launch_step_fused(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags,
+                      const Scratch &s) {
     const KGrid &g = ctx->g;
-    Scratch &s = ctx->s;
     hipStream_t st = ctx->stream;
-    if (int rc = launch_surface(ctx, f, d)) return rc;
-    launch_column(ctx, f, d, flags, true);
+    if (int rc = launch_surface(ctx, f, d, s)) return rc;
+    launch_column(ctx, f, d, flags, true, s);
     {
         Timer t(ctx, "k_march");
         MarchArgs a{};
@@ -795,7 +794,7 @@ int launch_step_fused(licom_ctx *ctx, const licom_fields *f, const licom_diag *d
         a.tiles_per_xcd = (a.ntiles + 7) / 8;
         k_march<<<16 * a.tiles_per_xcd, NT, 0, st>>>(g, ctx->kp, a);
     }
-    launch_column(ctx, f, d, flags, false);
+    launch_column(ctx, f, d, flags, false, s);
     if (flags & LICOM_STEP_SIDE_EFFECTS) launch_side(ctx, f);
     return launch_halo(ctx, f, flags);
 }

@@ -37,8 +37,7 @@ struct Scratch {
     uint8_t *flags = nullptr;    // [2][km][jmt][imt] tspas limiter flags
     double *d2tk = nullptr;      // [2][km][jmt][imt] del4 first pass
     double *ori = nullptr;       // [2][km][jmt][imt] explicit update
-    int *mix_count = nullptr;    // [1] columns queued for convective mixing
-    int *mix_list = nullptr;     // [jmt*imt] their 2-D indices
+    int *mix_count = nullptr;    // [1] columns convadj mixed in the last step
     int nblk = 0;
 };
 
@@ -86,6 +85,8 @@ namespace licom {
 
 void set_error(const char *fmt, ...);
 int hip_status(hipError_t e, const char *what);
+// tracer_api.hip: argument checks and path choice of an FP64 step
+int plan_step(licom_ctx *c, const licom_fields *f, int &flags, bool &split, const char *who);
 
 // Kernel launchers.  All enqueue on ctx->stream.
 // tracer_kernels.hip: one kernel per reference stage (cross-check path)
@@ -93,10 +94,12 @@ int launch_step_simple(licom_ctx *ctx, const licom_fields *f, const licom_diag *
 int launch_convadj(licom_ctx *ctx, const licom_fields *f, const licom_diag *d);
 int launch_icesnow(licom_ctx *ctx, double *at, double *qice, int reset);
 // shared pieces of tracer_kernels.hip used by both paths
-int launch_surface(licom_ctx *ctx, const licom_fields *f, const licom_diag *d);
+int launch_surface(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, const Scratch &s);
 int launch_halo(licom_ctx *ctx, const licom_fields *f, int flags);
 // tracer_fused.hip: prelude / k-march / column solve (production path)
-int launch_step_fused(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags);
+// s: the step's scratch (the context's, or the no-allocation entry point's)
+int launch_step_fused(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, int flags,
+                      const Scratch &s);
 // tracer_mixed.hip: the mixed-precision form
 int launch_step_mixed(licom_ctx *ctx, const licom_fields_mixed *f, const licom_diag_mixed *d,
                       int flags);

@@ -588,10 +588,9 @@ Timer::~Timer() {
     }
 }
 
-int launch_surface(licom_ctx *ctx, const licom_fields *f, const licom_diag *d) {
+int launch_surface(licom_ctx *ctx, const licom_fields *f, const licom_diag *d, const Scratch &s) {
     const KGrid &g = ctx->g;
     const KPar &p = ctx->kp;
-    Scratch &s = ctx->s;
     hipStream_t st = ctx->stream;
     const dim3 b2(64, 4);
     {
@@ -639,7 +638,7 @@ int launch_step_simple(licom_ctx *ctx, const licom_fields *f, const licom_diag *
     const dim3 g2 = grid2(g.imt, g.jmt, b2);
     const dim3 g3(g2.x, g2.y, g.km);
     const dim3 g3i((g.imt - 4 + 63) / 64, (g.jmt - 4 + 3) / 4, g.km);
-    if (int rc = launch_surface(ctx, f, d)) return rc;
+    if (int rc = launch_surface(ctx, f, d, s)) return rc;
     {
         Timer t(ctx, "k_faces");
         k_faces<<<g3, b3, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl, s.uw, s.vs);

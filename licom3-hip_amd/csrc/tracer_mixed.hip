@@ -796,7 +796,7 @@ int launch_step_mixed(licom_ctx *ctx, const licom_fields_mixed *f, const licom_d
         d64.net = d->net;
         d64.fw_norm2 = d->fw_norm2;
     }
-    if (int rc = launch_surface(ctx, &f64, &d64)) return rc;
+    if (int rc = launch_surface(ctx, &f64, &d64, ctx->s)) return rc;
     launch_column_mx(ctx, f, d, flags, true);
     {
         Timer t(ctx, "k_march");

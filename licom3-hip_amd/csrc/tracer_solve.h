@@ -411,8 +411,10 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
     }
 }
 
-// In-place side effects of the reference on its inputs
-// (tracer.cpp:1088, 1107-1108, 1187, 1315).
+// The single-model solve.  Its workgroups keep the hardware's round-robin
+// order over the XCDs (neighbouring row chunks on different XCDs): handing
+// each XCD a contiguous run of chunks instead made the 0.1 deg solve 45%
+// slower (25.7 vs 17.7 ms).
 template <int KM, typename AK, typename DG>
 __global__ __launch_bounds__(256, 3) void k_solve(KGrid g, KPar p, int flags, double *__restrict__ at,
                                                double *__restrict__ atb,
