@@ -153,6 +153,20 @@ int licom_tracer_set_stream(licom_ctx *ctx, void *hip_stream);
 int licom_tracer_step(licom_ctx *ctx, const licom_fields *f,
                       const licom_diag *d, int flags);
 
+/* The same step as licom_tracer_step, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops): it
+ * calls no hipMalloc, hipFree or allocating routine, and keeps the step's
+ * scratch in a static device array of 160 MiB per device instead of the
+ * context's allocation.  Grids whose scratch, 32*imt*jmt*km + 44*imt*jmt
+ * bytes, exceeds it (e.g. larger than 360 x 218 x 55) get LICOM_EINVAL;
+ * LICOM_STEP_STAGED, layouts that need the per-stage kernels (odd imt,
+ * arrays not 16-byte aligned) and ICESNOW with VTL_SCRATCH get
+ * LICOM_ENOTSUP.  Every call on a device shares that device's array:
+ * calls that overlap on one device race.  No counterpart in the reference
+ * (tracer.cpp:1701 allocates its scratch once at start-up, cuda_init.cpp). */
+int licom_tracer_step_noalloc(licom_ctx *ctx, const licom_fields *f,
+                              const licom_diag *d, int flags);
+
 /* Mixed-precision form of the step: the tracer state at/atb and every
  * 2-D field stay FP64; velocities, ws, akt and the 3-D diagnostics are FP32.
  * Advective fluxes, the del4/del2 stencils and the explicit vertical

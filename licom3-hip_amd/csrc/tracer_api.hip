@@ -263,6 +263,9 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
         return LICOM_ENODEV;
     }
     if (int rc = hip_status(hipGetDevice(&dev), "hipGetDevice")) return rc;
+    // the no-allocation step's static scratch: its device address, once per
+    // device (a failure only disables licom_tracer_step_noalloc)
+    (void)noalloc_init(dev);
 
     licom_ctx *c = new (std::nothrow) licom_ctx();
     if (!c) return LICOM_ENOMEM;

@@ -87,6 +87,10 @@ void set_error(const char *fmt, ...);
 int hip_status(hipError_t e, const char *what);
 // tracer_api.hip: argument checks and path choice of an FP64 step
 int plan_step(licom_ctx *c, const licom_fields *f, int &flags, bool &split, const char *who);
+// tracer_noalloc.hip: device address of the no-allocation step's static
+// scratch, looked up once per device (called by licom_tracer_create with
+// the device current)
+int noalloc_init(int device);
 
 // Kernel launchers.  All enqueue on ctx->stream.
 // tracer_kernels.hip: one kernel per reference stage (cross-check path)

@@ -80,6 +80,7 @@ class GridDerived(ctypes.Structure):
 
 
 EXPORTED = ["licom_tracer_create", "licom_tracer_set_stream", "licom_tracer_step",
+            "licom_tracer_step_noalloc",
             "licom_tracer_step_mixed", "licom_batch_create", "licom_batch_set_stream",
             "licom_batch_set_march_grid", "licom_batch_step", "licom_batch_capture",
             "licom_batch_replay", "licom_batch_sync", "licom_batch_destroy",
@@ -121,6 +122,7 @@ def lib():
         L.licom_tracer_set_stream.argtypes = [_VP, _VP]
         L.licom_tracer_step.argtypes = [_VP, ctypes.POINTER(Fields), ctypes.POINTER(Diag),
                                         ctypes.c_int]
+        L.licom_tracer_step_noalloc.argtypes = L.licom_tracer_step.argtypes
         # licom_fields_mixed / licom_diag_mixed have the same pointer layout
         L.licom_tracer_step_mixed.argtypes = [_VP, ctypes.POINTER(Fields), ctypes.POINTER(Diag),
                                               ctypes.c_int]
