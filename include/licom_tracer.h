@@ -41,7 +41,10 @@ enum licom_status {
 };
 
 enum licom_hmix { LICOM_HMIX_DEL2 = 2, LICOM_HMIX_DEL4 = 4 };
-enum licom_adv { LICOM_ADV_TSPAS = 2 };
+/* advection schemes, numbered as the reference's Str code
+ * (tracer.cpp:1712-1714): centered and flux are advection_tracer_1
+ * (tracer.cpp:120-289), tspas advection_tracer_2/3 (:291-921) */
+enum licom_adv { LICOM_ADV_CENTERED = 0, LICOM_ADV_FLUX = 1, LICOM_ADV_TSPAS = 2 };
 
 /* licom_tracer_step flags */
 #define LICOM_STEP_CONVADJ 0x1      /* run convadj after the tracer update    */
@@ -81,6 +84,9 @@ typedef struct licom_grid {
                                  for the lat-lon 0.25 (grid.F90:1654-1657)     */
     const double *to, *so;    /* [km] EOS reference T and S                    */
     const double *eosc;       /* [9][km] = Fortran C(km,9), kg/m^3             */
+    const double *dxu, *dyu;  /* [jmt][imt] U-cell widths: the flux scheme's
+                                 face transports (tracer.cpp:106-117); may be
+                                 NULL for centered and tspas                   */
 } licom_grid;
 
 typedef struct licom_params {
@@ -91,9 +97,12 @@ typedef struct licom_params {
     double od0, od0cp;        /* 1/D0, 1/(D0*CP)                              */
     double ah;                /* AH_HOR (del2) or AH_BIHAR (del4)             */
     int hmix;                 /* LICOM_HMIX_DEL2 / LICOM_HMIX_DEL4            */
-    int adv;                  /* LICOM_ADV_TSPAS                              */
+    int adv;                  /* LICOM_ADV_CENTERED / _FLUX / _TSPAS          */
     int boundary_restore;     /* 1: restoring term of tracer_7                 */
     int sssnorm;              /* 1: global freshwater normalisation of S       */
+    int nss;                  /* pconst_mod nss: the centered scheme adds its
+                                 advective increments / nss to ax, ay, az
+                                 (tracer.cpp:140-159); >= 1 for centered      */
 } licom_params;
 
 /* Device arrays of one step (the d_* globals of cuda_data.h). */
@@ -113,7 +122,10 @@ typedef struct licom_fields {
 
 /* Device diagnostics; any pointer may be NULL (not produced).  Values are
  * written on the physical cells [2,jmt-2) x [2,imt-2); ghost cells are left
- * untouched. */
+ * untouched.  tspas overwrites ax, ay, az with the step's advective
+ * tendencies (tracer.cpp:664,809,917); centered adds them divided by nss
+ * (an accumulation over the caller's diagnostic period, tracer.cpp:140-205);
+ * flux leaves them alone (tracer.cpp:258-288). */
 typedef struct licom_diag {
     double *ax, *ay, *az;     /* [2][km][jmt][imt] advective tendencies        */
     double *dz;               /* [2][km][jmt][imt] explicit vertical diffusion */
@@ -198,7 +210,8 @@ typedef struct licom_diag_mixed {
 
 /* The step in the mixed-precision form; flags as licom_tracer_step except
  * LICOM_STEP_VTL_SCRATCH and LICOM_STEP_STAGED (LICOM_ENOTSUP).  Needs an
- * even imt and 16-byte aligned at/atb. */
+ * even imt and 16-byte aligned at/atb; tspas only (other schemes:
+ * LICOM_ENOTSUP, as for batches). */
 int licom_tracer_step_mixed(licom_ctx *ctx, const licom_fields_mixed *f,
                             const licom_diag_mixed *d, int flags);
 

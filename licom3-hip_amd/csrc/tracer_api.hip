@@ -249,9 +249,17 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
         set_error("hmix must be 2 (del2) or 4 (del4), got %d", P->hmix);
         return LICOM_EINVAL;
     }
-    if (P->adv != LICOM_ADV_TSPAS) {
-        set_error("only the tspas advection scheme is built (adv=%d)", P->adv);
-        return LICOM_ENOTSUP;
+    if (P->adv != LICOM_ADV_TSPAS && P->adv != LICOM_ADV_CENTERED && P->adv != LICOM_ADV_FLUX) {
+        set_error("adv must be 0 (centered), 1 (flux) or 2 (tspas), got %d", P->adv);
+        return LICOM_EINVAL;
+    }
+    if (P->adv == LICOM_ADV_FLUX && (!G->dxu || !G->dyu)) {
+        set_error("the flux scheme needs the U-cell widths dxu and dyu");
+        return LICOM_EINVAL;
+    }
+    if (P->adv == LICOM_ADV_CENTERED && P->nss < 1) {
+        set_error("the centered scheme divides its diagnostics by nss >= 1 (nss=%d)", P->nss);
+        return LICOM_EINVAL;
     }
     if (!(P->dts > 0.0)) {
         set_error("dts must be positive");
@@ -291,6 +299,9 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
     g.jmt = c->jmt;
     g.km = km;
     std::vector<double> quarter(n2, 0.25), pen0(km > 1 ? km - 1 : 1, 0.0);
+    // the U-cell widths only feed the flux scheme's faces
+    const bool flux = P->adv == LICOM_ADV_FLUX;
+    std::vector<double> zero2(flux ? 0 : n2, 0.0);
     // The static fields and the step scratch share one device allocation: a
     // context costs one hipMalloc and one clear instead of ~36 allocations
     // with a memset each (context creation is on the request loop's
@@ -368,6 +379,8 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
     rc = rc ? rc : up_d(&g.to, G->to, km);
     rc = rc ? rc : up_d(&g.so, G->so, km);
     rc = rc ? rc : up_d(&g.eosc, G->eosc, (size_t)9 * km);
+    rc = rc ? rc : up_d(&g.dxu, flux ? G->dxu : zero2.data(), n2);
+    rc = rc ? rc : up_d(&g.dyu, flux ? G->dyu : zero2.data(), n2);
 
     KPar &k = c->kp;
     k.dts = P->dts;
@@ -381,6 +394,8 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
     k.hmix = P->hmix;
     k.boundary_restore = P->boundary_restore;
     k.sssnorm = P->sssnorm;
+    k.adv = P->adv;
+    k.nss = P->nss >= 1 ? P->nss : 1;
 
     Scratch &s = c->s;
     // SSSNORM partial-sum blocks: about 8 physical cells per lane, at most 1024
@@ -448,6 +463,10 @@ int licom_tracer_step_mixed(licom_ctx *c, const licom_fields_mixed *f, const lic
     }
     if (flags & (LICOM_STEP_VTL_SCRATCH | LICOM_STEP_STAGED)) {
         set_error("licom_tracer_step_mixed: VTL_SCRATCH and STAGED are FP64-only");
+        return LICOM_ENOTSUP;
+    }
+    if (c->kp.adv != LICOM_ADV_TSPAS) {
+        set_error("licom_tracer_step_mixed: the mixed form is built for tspas only (adv=%d)", c->kp.adv);
         return LICOM_ENOTSUP;
     }
     if ((flags & LICOM_STEP_ICESNOW) && !c->qice) {

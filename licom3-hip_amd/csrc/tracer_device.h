@@ -90,6 +90,47 @@ __device__ __forceinline__ double predictor(const PTerms &t, double Tc, double d
     return Tc + (xx + yy + zz) * dts;
 }
 
+// advection_tracer_1 (tracer.cpp:120-289) at one cell: the centered
+// (Str=0) or flux (Str=1) advective tendency, from the same neighbours as
+// the tspas terms (Wk/Wb: w at the top and bottom of the cell).  For the
+// centered form hx, hy, vz are the parts the reference adds to ax, ay and
+// subtracts from az, each before its division by nss (tracer.cpp:140-159,
+// 174-205, 224-255); level 0 and km-1 take one vertical term, as in the
+// level-0 / level-(km-1) statements of tracer.cpp:152-159 and 245-255.
+struct CFAdv {
+    double adv, hx, hy, vz;
+};
+__device__ __forceinline__ CFAdv adv_cf(int scheme, double Tc, double Te, double Tw, double Ts,
+                                        double Tn, double Tu, double Td, double Ue, double Uw,
+                                        double Vs, double Vn, double Wk, double Wb, double tr, int k,
+                                        int km, const double *odzp) {
+    CFAdv r;
+    if (scheme == LICOM_ADV_CENTERED) {
+        const double a = (-Uw * (Tc - Tw) - Ue * (Te - Tc) - Vs * (Ts - Tc) - Vn * (Tc - Tn)) * tr;
+        r.hx = (-Uw * (Tc - Tw) - Ue * (Te - Tc)) * tr;
+        r.hy = (-Vs * (Ts - Tc) - Vn * (Tc - Tn)) * tr;
+        double z;
+        if (k == 0) {
+            z = Wb * (Tc - Td);
+        } else if (k == km - 1) {
+            z = Wk * (Tu - Tc);
+        } else {
+            const double z1 = Wk * (Tu - Tc);
+            const double z2 = Wb * (Tc - Td);
+            z = z1 + z2;
+        }
+        r.vz = 0.5 * odzp[k] * z;
+        r.adv = a - 0.5 * odzp[k] * z;
+    } else {
+        const double a = (-Uw * (Tc + Tw) + Ue * (Te + Tc) - Vn * (Tc + Tn) + Vs * (Ts + Tc)) * tr;
+        const double z1 = (k == 0) ? 0.0 : Wk * (Tu + Tc) * 0.5;
+        const double z2 = (k == km - 1) ? 0.0 : Wb * (Tc + Td) * 0.5;
+        r.adv = a - odzp[k] * (z2 - z1);
+        r.hx = r.hy = r.vz = 0.0;
+    }
+    return r;
+}
+
 // First-order upwind replacements of the limiter (tracer.cpp:574-658).
 __device__ __forceinline__ double up_xy1(double Te, double Tc, double Ue, double tr) {
     return -(Te - Tc) * fabs(Ue) * tr;

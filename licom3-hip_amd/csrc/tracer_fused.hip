@@ -46,7 +46,7 @@ constexpr int KMAXF = LICOM_MAX_KM;
 // (1 + work_u*ohbu, upwell_2) are loaded once per column; the divergence
 // profile stays in registers for the column sum and the continuity recursion
 // of upwell_3/4 (tracer.cpp:1029-1080).
-template <int KM>
+template <int KM, bool FLUX>
 __global__ __launch_bounds__(256) void k_prelude_col(KGrid g, KPar p, const double *__restrict__ utf,
                                                      const double *__restrict__ utl,
                                                      const double *__restrict__ vtf,
@@ -106,6 +106,19 @@ __global__ __launch_bounds__(256) void k_prelude_col(KGrid g, KPar p, const doub
     const bool do_v = in && i >= 1 && i < imt - 1;
     const bool do_w = do_u && do_v;
     const double htw_c = in ? g.htw[q] : 0.0, hts_c = in ? g.hts[q] : 0.0;
+    // flux faces (tracer.cpp:106-117): U-cell widths at (j-1,i), (j,i) and
+    // (j,i), (j,i+1)
+    double dyu_n = 0.0, dyu_c = 0.0, dxu_c = 0.0, dxu_e = 0.0;
+    if constexpr (FLUX) {
+        if (do_u) {
+            dyu_n = g.dyu[q - imt];
+            dyu_c = g.dyu[q];
+        }
+        if (do_v) {
+            dxu_c = g.dxu[q];
+            dxu_e = g.dxu[q + 1];
+        }
+    }
     double fc = 0, fe = 0, fn = 0, fne = 0, htw_e = 0, hts_n = 0, tr = 0;
     double ohbt = 0, stfp = 0, prev = 0;
     int kmt = 0;
@@ -145,8 +158,13 @@ __global__ __launch_bounds__(256) void k_prelude_col(KGrid g, KPar p, const doub
             if (k + 1 < km) fetch(k + 1);
             __syncthreads();
             const double dc = sD[b][w0], bc = sBv[b][w0];
-            buf_store(plane_rsrc(uw + (size_t)k * n2, pb), ou, (sD[b][w0 - WX] + dc) * htw_c * 0.25);
-            buf_store(plane_rsrc(vs + (size_t)k * n2, pb), ov, (bc + sBv[b][w0 + 1]) * hts_c * 0.25);
+            if constexpr (FLUX) {
+                buf_store(plane_rsrc(uw + (size_t)k * n2, pb), ou, (sD[b][w0 - WX] * dyu_n + dc * dyu_c) * 0.25);
+                buf_store(plane_rsrc(vs + (size_t)k * n2, pb), ov, (bc * dxu_c + sBv[b][w0 + 1] * dxu_e) * 0.25);
+            } else {
+                buf_store(plane_rsrc(uw + (size_t)k * n2, pb), ou, (sD[b][w0 - WX] + dc) * htw_c * 0.25);
+                buf_store(plane_rsrc(vs + (size_t)k * n2, pb), ov, (bc + sBv[b][w0 + 1]) * hts_c * 0.25);
+            }
             if (do_w && k <= kmt - 1) {
                 const double de = sD[b][w0 + 1], dn = sD[b][w0 - WX], dne = sD[b][w0 - WX + 1];
                 const double be = sBv[b][w0 + 1], bn = sBv[b][w0 - WX], bne = sBv[b][w0 - WX + 1];
@@ -373,7 +391,11 @@ __device__ __forceinline__ void bar_lds() { asm volatile("s_waitcnt lgkmcnt(0)\n
 //   C(k): limiter, fluxes and explicit update of level k on the tile (waves
 //         0-2); wave 3 waits for the loads P(k+1) and C(k+1) read.
 // LDS slots rotate: at and ws x(LA+3), u_wface/v_sface, atb and akt x(LA+2).
+// ADV: the advection scheme (LICOM_ADV_*); centered and flux need no
+// predictor, so their P(k) is the del4 first pass alone.
+template <int ADV>
 __global__ __launch_bounds__(NT, (LA >= 2 ? 2 : 3)) void k_march(KGrid g, KPar p, MarchArgs a) {
+    constexpr bool tspas = ADV == LICOM_ADV_TSPAS;
     __shared__ __align__(16) double sAT[NAT][NR];
     __shared__ __align__(16) double sU[NUV][NR];
     __shared__ __align__(16) double sV[NUV][NR];
@@ -465,7 +487,7 @@ __global__ __launch_bounds__(NT, (LA >= 2 ? 2 : 3)) void k_march(KGrid g, KPar p
         }
         wait_vmem();
         bar_lds();
-        {
+        if constexpr (tspas) {
             LW tmp;
             const Lv L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
             const bool f0 = cell_ok ? pred_flag(g, rC, L0, sK, 0, sZP, sZT, dts, M.tr, M.D1, M.D2, M.D3,
@@ -492,7 +514,7 @@ __global__ __launch_bounds__(NT, (LA >= 2 ? 2 : 3)) void k_march(KGrid g, KPar p
                 stage_tile(A + (size_t)(k + LA + 1) * n2, sAK[u2], offa, lane);
                 issued += DMA_UVBA;
             }
-            if (k + 1 < km) {
+            if (tspas && k + 1 < km) {
                 LW tmp;
                 const Lv L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
                 const bool f =
@@ -517,7 +539,7 @@ __global__ __launch_bounds__(NT, (LA >= 2 ? 2 : 3)) void k_march(KGrid g, KPar p
     __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0)
     bar_lds();
     LW lc{}, ln{};
-    {
+    if constexpr (tspas) {
         const Lv L0{sAT[0], sAT[0], sAT[1], sU[0], sV[0], sW[0], sW[1]};
         sFL[0][pC] = cell_ok ? pred_flag(g, rC, L0, sK, 0, sZP, sZT, dts, M.tr, M.D1, M.D2, M.D3, M.D4,
                                          lc)
@@ -535,7 +557,7 @@ __global__ __launch_bounds__(NT, (LA >= 2 ? 2 : 3)) void k_march(KGrid g, KPar p
         const int u0 = k % NUV, u1 = (k + 1) % NUV;
         // ---- phase P(k)
         const double Acur = act ? dev::floor_akt(sAK[u0][t], k, n, p.hmix, p.dwndmix) : 0.0;
-        if (k + 1 < km) {
+        if (tspas && k + 1 < km) {
             const Lv L1{sAT[s1], sAT[s0], sAT[s2], sU[u1], sV[u1], sW[s1], sW[s2]};
             sFL[(k + 1) & 1][pC] =
                 cell_ok ? pred_flag(g, rC, L1, sK, k + 1, sZP, sZT, dts, M.tr, M.D1, M.D2, M.D3, M.D4,
@@ -557,47 +579,65 @@ __global__ __launch_bounds__(NT, (LA >= 2 ? 2 : 3)) void k_march(KGrid g, KPar p
             const double *U = sU[u0], *V = sV[u0];
             const double Ue = U[r + 1], Uw = U[r], Vs = V[r], Vn = V[r - RX];
             const double Wk = sW[s0][r], Wb = (k + 1 < km) ? sW[s1][r] : 0.0;
-            const bool fc = F[pp], fe = F[pp + 1], fw = F[pp - 1], fs = F[pp + PX], fn = F[pp - PX];
-            const bool fd = (k + 1 < km) ? sFL[(k + 1) & 1][pp] : false;
-            bool up1, up2, up3, up4;
-            if (k == 0 || k == km - 1) {
-                up1 = fc || (fe && gi <= imt - 3);
-                up2 = fc || (fw && gi >= 2);
-                up3 = fc || (fs && gj <= jmt - 3);
-                up4 = fc || (fn && gj >= 2);
-            } else {
-                up1 = (fc || fe) && gi <= imt - 3;
-                up2 = (fc || fw) && gi >= 2;
-                up3 = (fc || fs) && gj <= jmt - 3;
-                up4 = (fc || fn) && gj >= 2;
-            }
-            const double tr = M.tr;
-            dev::PTerms tc;
-            dev::pterms_cheap(Tc, Te, Tw, Ts, Tn, Tu, Td, Ue, Uw, Vs, Vn, Wk, Wb, tr, k, km, sZP, tc);
-            const double xy1 = up1 ? dev::up_xy1(Te, Tc, Ue, tr) : lc.xy1;
-            const double xy2 = up2 ? dev::up_xy2(Tc, Tw, Uw, tr) : lc.xy2;
-            const double xy3 = up3 ? dev::up_xy1(Ts, Tc, Vs, tr) : lc.xy3;
-            const double xy4 = up4 ? dev::up_xy2(Tc, Tn, Vn, tr) : lc.xy4;
-            double zb1, zb2;
-            if (k == 0) {
-                zb1 = 0.0;
-                zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, Wb, sZP[0]) : lc.zb2;
-            } else if (k == km - 1) {
-                zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, Wk, sZP[k]) : lc.zb1;
-                zb2 = 0.0;
-            } else {
-                zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, Wk, sZP[k]) : lc.zb1;
-                zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, Wb, sZP[k]) : lc.zb2;
-            }
-            const double xx = -(tc.x0 + xy1 + xy2 + tc.c1);
-            const double yy = -(tc.y0 + xy3 + xy4 + tc.c2);
-            const double zz = -(tc.za + zb1 + zb2 + tc.zc);
             const size_t c = (size_t)k * n2 + q;
-            if (a.ax) a.ax[n * n3 + c] = xx;
-            if (a.ay) a.ay[n * n3 + c] = yy;
-            if (a.az) a.az[n * n3 + c] = zz;
             const double vk = k < kmt_own ? 1.0 : 0.0;
-            double tf = (xx + yy + zz) * vk;
+            double tf;
+            bool fc = false;
+            if constexpr (!tspas) {
+                // advection_tracer_1 (tracer.cpp:120-289); the centered form
+                // accumulates its diagnostics (the caller's ax/ay/az carry the
+                // sum over the diagnostic period)
+                const dev::CFAdv cf = dev::adv_cf(ADV, Tc, Te, Tw, Ts, Tn, Tu, Td, Ue, Uw, Vs, Vn, Wk, Wb, M.tr,
+                                                  k, km, sZP);
+                if constexpr (ADV == LICOM_ADV_CENTERED) {
+                    const double nss = (double)p.nss;
+                    if (a.ax) a.ax[n * n3 + c] = a.ax[n * n3 + c] + cf.hx / nss;
+                    if (a.ay) a.ay[n * n3 + c] = a.ay[n * n3 + c] + cf.hy / nss;
+                    if (a.az) a.az[n * n3 + c] = a.az[n * n3 + c] - cf.vz / nss;
+                }
+                tf = cf.adv * vk;
+            } else {
+                fc = F[pp];
+                const bool fe = F[pp + 1], fw = F[pp - 1], fs = F[pp + PX], fn = F[pp - PX];
+                const bool fd = (k + 1 < km) ? sFL[(k + 1) & 1][pp] : false;
+                bool up1, up2, up3, up4;
+                if (k == 0 || k == km - 1) {
+                    up1 = fc || (fe && gi <= imt - 3);
+                    up2 = fc || (fw && gi >= 2);
+                    up3 = fc || (fs && gj <= jmt - 3);
+                    up4 = fc || (fn && gj >= 2);
+                } else {
+                    up1 = (fc || fe) && gi <= imt - 3;
+                    up2 = (fc || fw) && gi >= 2;
+                    up3 = (fc || fs) && gj <= jmt - 3;
+                    up4 = (fc || fn) && gj >= 2;
+                }
+                const double tr = M.tr;
+                dev::PTerms tc;
+                dev::pterms_cheap(Tc, Te, Tw, Ts, Tn, Tu, Td, Ue, Uw, Vs, Vn, Wk, Wb, tr, k, km, sZP, tc);
+                const double xy1 = up1 ? dev::up_xy1(Te, Tc, Ue, tr) : lc.xy1;
+                const double xy2 = up2 ? dev::up_xy2(Tc, Tw, Uw, tr) : lc.xy2;
+                const double xy3 = up3 ? dev::up_xy1(Ts, Tc, Vs, tr) : lc.xy3;
+                const double xy4 = up4 ? dev::up_xy2(Tc, Tn, Vn, tr) : lc.xy4;
+                double zb1, zb2;
+                if (k == 0) {
+                    zb1 = 0.0;
+                    zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, Wb, sZP[0]) : lc.zb2;
+                } else if (k == km - 1) {
+                    zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, Wk, sZP[k]) : lc.zb1;
+                    zb2 = 0.0;
+                } else {
+                    zb1 = (fc || fprev) ? dev::up_zb1(Tu, Tc, Wk, sZP[k]) : lc.zb1;
+                    zb2 = (fc || fd) ? dev::up_zb2(Tc, Td, Wb, sZP[k]) : lc.zb2;
+                }
+                const double xx = -(tc.x0 + xy1 + xy2 + tc.c1);
+                const double yy = -(tc.y0 + xy3 + xy4 + tc.c2);
+                const double zz = -(tc.za + zb1 + zb2 + tc.zc);
+                if (a.ax) a.ax[n * n3 + c] = xx;
+                if (a.ay) a.ay[n * n3 + c] = yy;
+                if (a.az) a.az[n * n3 + c] = zz;
+                tf = (xx + yy + zz) * vk;
+            }
             // horizontal mixing (tracer_del4_2 / tracer_4)
             {
                 const bool d4 = p.hmix == LICOM_HMIX_DEL4;
@@ -735,8 +775,12 @@ void launch_prelude_solve(licom_ctx *ctx, const licom_fields *f, const licom_dia
     if (prelude) {
         Timer t(ctx, "k_prelude");
         const XcdGrid xg = xcd_grid((g.imt + 63) / 64, (g.jmt + 3) / 4);
-        k_prelude_col<KM><<<xcd_blocks(xg), 256, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl, s.work_u,
-                                                          s.stf_pre, f->ws, s.uw, s.vs, xg);
+        if (p.adv == LICOM_ADV_FLUX)
+            k_prelude_col<KM, true><<<xcd_blocks(xg), 256, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl,
+                                                                    s.work_u, s.stf_pre, f->ws, s.uw, s.vs, xg);
+        else
+            k_prelude_col<KM, false><<<xcd_blocks(xg), 256, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl,
+                                                                     s.work_u, s.stf_pre, f->ws, s.uw, s.vs, xg);
     } else {
         Timer t(ctx, "k_solve");
         const dim3 gr = solve_grid(g.imt, g.jmt, ctx->solve_shape);
@@ -792,7 +836,12 @@ int launch_step_fused(licom_ctx *ctx, const licom_fields *f, const licom_diag *d
         const int nty = (g.jmt - 4 + BY - 1) / BY;
         a.ntiles = a.ntx * nty;
         a.tiles_per_xcd = (a.ntiles + 7) / 8;
-        k_march<<<16 * a.tiles_per_xcd, NT, 0, st>>>(g, ctx->kp, a);
+        const unsigned nb = 16 * a.tiles_per_xcd;
+        switch (ctx->kp.adv) {
+            case LICOM_ADV_CENTERED: k_march<LICOM_ADV_CENTERED><<<nb, NT, 0, st>>>(g, ctx->kp, a); break;
+            case LICOM_ADV_FLUX: k_march<LICOM_ADV_FLUX><<<nb, NT, 0, st>>>(g, ctx->kp, a); break;
+            default: k_march<LICOM_ADV_TSPAS><<<nb, NT, 0, st>>>(g, ctx->kp, a); break;
+        }
     }
     launch_column(ctx, f, d, flags, false, s);
     if (flags & LICOM_STEP_SIDE_EFFECTS) launch_side(ctx, f);

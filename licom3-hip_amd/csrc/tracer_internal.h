@@ -19,11 +19,14 @@ struct KGrid {
     const double *ohbt, *ohbu, *dtn, *dts, *dte, *dtw, *ahf;
     const double *au0, *aus, *auw, *ausw;
     const double *dzp, *odzp, *odzt, *pen, *to, *so, *eosc;
+    const double *dxu, *dyu;   // U-cell widths (flux scheme faces; zeros otherwise)
 };
 
 struct KPar {
     double dts, onbc, oncc, dwndmix, gamma, od0, od0cp, ah;
     int hmix, boundary_restore, sssnorm;
+    int adv;    // LICOM_ADV_CENTERED / _FLUX / _TSPAS
+    int nss;    // divisor of the centered scheme's ax/ay/az increments
 };
 
 struct Scratch {

@@ -61,7 +61,8 @@ __device__ __forceinline__ double wkd_at(const double *utf, const double *utl, d
     return 0.5 * (utf[c] * oncc) + 0.5 * utl[c];   // tracer_1 (:1107-1119)
 }
 
-// advection_tracer_1_0, tspas branch (tracer.cpp:83-104)
+// advection_tracer_1_0 (tracer.cpp:83-119): centered and tspas faces from
+// hts/htw, flux faces from the U-cell widths dxu/dyu
 __global__ void k_faces(KGrid g, KPar p, const double *__restrict__ utf,
                         const double *__restrict__ utl, const double *__restrict__ vtf,
                         const double *__restrict__ vtl, double *uw, double *vs) {
@@ -71,10 +72,18 @@ __global__ void k_faces(KGrid g, KPar p, const double *__restrict__ utf,
     if (i >= g.imt || j >= g.jmt) return;
     const size_t c = i3(g, k, j, i), q = i2(g, j, i);
     double v = 0.0, u = 0.0;
-    if (i >= 1 && i < g.imt - 1)
-        v = (wkd_at(vtf, vtl, p.oncc, c) + wkd_at(vtf, vtl, p.oncc, c + 1)) * g.hts[q] * 0.25;
-    if (j >= 1 && j < g.jmt - 1)
-        u = (wkd_at(utf, utl, p.oncc, c - g.imt) + wkd_at(utf, utl, p.oncc, c)) * g.htw[q] * 0.25;
+    if (p.adv == LICOM_ADV_FLUX) {
+        if (i >= 1 && i < g.imt - 1)
+            v = (wkd_at(vtf, vtl, p.oncc, c) * g.dxu[q] + wkd_at(vtf, vtl, p.oncc, c + 1) * g.dxu[q + 1]) * 0.25;
+        if (j >= 1 && j < g.jmt - 1)
+            u = (wkd_at(utf, utl, p.oncc, c - g.imt) * g.dyu[q - g.imt] + wkd_at(utf, utl, p.oncc, c) * g.dyu[q]) *
+                0.25;
+    } else {
+        if (i >= 1 && i < g.imt - 1)
+            v = (wkd_at(vtf, vtl, p.oncc, c) + wkd_at(vtf, vtl, p.oncc, c + 1)) * g.hts[q] * 0.25;
+        if (j >= 1 && j < g.jmt - 1)
+            u = (wkd_at(utf, utl, p.oncc, c - g.imt) + wkd_at(utf, utl, p.oncc, c)) * g.htw[q] * 0.25;
+    }
     vs[c] = v;
     uw[c] = u;
 }
@@ -253,6 +262,23 @@ __global__ void k_correct(KGrid g, KPar p, const double *__restrict__ at,
     const double aidif = 0.5;
     for (int n = 0; n < 2; ++n) {
         const double *T = at + n * n3;
+        double adv;
+        if (p.adv != LICOM_ADV_TSPAS) {
+            // advection_tracer_1 (tracer.cpp:120-289)
+            const double Tc = T[c];
+            const double Tu = k > 0 ? T[c - n2] : 0.0, Td = k < km - 1 ? T[c + n2] : 0.0;
+            const double Wb = k < km - 1 ? ws[c + n2] : 0.0;
+            const dev::CFAdv r = dev::adv_cf(p.adv, Tc, T[c + 1], T[c - 1], T[c + sj], T[c - sj], Tu, Td,
+                                             uw[c + 1], uw[c], vs[c], vs[c - sj], ws[c], Wb,
+                                             g.tarea_r[q], k, km, odzp);
+            if (p.adv == LICOM_ADV_CENTERED) {
+                const double nss = (double)p.nss;
+                if (ax) ax[n * n3 + c] = ax[n * n3 + c] + r.hx / nss;
+                if (ay) ay[n * n3 + c] = ay[n * n3 + c] + r.hy / nss;
+                if (az) az[n * n3 + c] = az[n * n3 + c] - r.vz / nss;
+            }
+            adv = r.adv;
+        } else {
         const uint8_t *F = flags + n * n3;
         PTerms t;
         predictor_terms(g, p.dts, T, uw, vs, ws, k, j, i, t);
@@ -289,10 +315,11 @@ __global__ void k_correct(KGrid g, KPar p, const double *__restrict__ at,
         const double xx = -(t.x0 + xy1 + xy2 + t.c1);
         const double yy = -(t.y0 + xy3 + xy4 + t.c2);
         const double zz = -(t.za + zb1 + zb2 + t.zc);
-        const double adv = xx + yy + zz;
+        adv = xx + yy + zz;
         if (ax) ax[n * n3 + c] = xx;
         if (ay) ay[n * n3 + c] = yy;
         if (az) az[n * n3 + c] = zz;
+        }
 
         double tf = adv * vitc;
         // horizontal mixing
@@ -647,7 +674,7 @@ int launch_step_simple(licom_ctx *ctx, const licom_fields *f, const licom_diag *
         Timer t(ctx, "k_ws");
         k_ws<<<g2, b2, 0, st>>>(g, p, f->utf, f->utl, f->vtf, f->vtl, s.work_u, s.stf_pre, f->ws);
     }
-    {
+    if (p.adv == LICOM_ADV_TSPAS) {
         Timer t(ctx, "k_predict");
         k_predict<<<g3, b3, 0, st>>>(g, p, f->at, s.uw, s.vs, f->ws, s.flags);
     }

@@ -1108,6 +1108,11 @@ extern "C" int licom_batch_create(licom_ctx *const *members, const licom_fields_
             set_error("licom_batch_create: member %d has boundary_restore=1 but no restore field", m);
             return LICOM_EINVAL;
         }
+        if (c->kp.adv != LICOM_ADV_TSPAS) {
+            set_error("licom_batch_create: member %d: batches run the mixed tspas form only (adv=%d)", m,
+                      c->kp.adv);
+            return LICOM_ENOTSUP;
+        }
         const licom_member_tuning tu = tuning ? tuning[m] : licom_member_tuning{0, 0};
         if (tu.prelude_shape < 0 || tu.prelude_shape > 2 || tu.solve_shape < 0 || tu.solve_shape > 2) {
             set_error("licom_batch_create: member %d has an unknown launch shape", m);

@@ -27,6 +27,8 @@ module licom_tracer_mod
    integer(c_int), parameter, public :: LICOM_STEP_ICESNOW = 16
    integer(c_int), parameter, public :: LICOM_HMIX_DEL2 = 2
    integer(c_int), parameter, public :: LICOM_HMIX_DEL4 = 4
+   integer(c_int), parameter, public :: LICOM_ADV_CENTERED = 0
+   integer(c_int), parameter, public :: LICOM_ADV_FLUX = 1
    integer(c_int), parameter, public :: LICOM_ADV_TSPAS = 2
 
    type, bind(C), public :: licom_grid
@@ -40,11 +42,13 @@ module licom_tracer_mod
       type(c_ptr) :: au0 = c_null_ptr, aus = c_null_ptr, auw = c_null_ptr, ausw = c_null_ptr
       type(c_ptr) :: to = c_null_ptr, so = c_null_ptr
       type(c_ptr) :: eosc = c_null_ptr         ! C(km,9)
+      type(c_ptr) :: dxu = c_null_ptr, dyu = c_null_ptr   ! (imt,jmt), flux scheme only
    end type licom_grid
 
    type, bind(C), public :: licom_params
       real(c_double) :: dts, onbc, oncc, dwndmix, gamma, od0, od0cp, ah
       integer(c_int) :: hmix, adv, boundary_restore, sssnorm
+      integer(c_int) :: nss                    ! pconst_mod nss (centered diagnostics)
    end type licom_params
 
    type, bind(C), public :: licom_fields

@@ -25,6 +25,8 @@ _LIB = None
 
 D0 = 1026.0
 CP = 3996.0
+# advection schemes by the reference's Str code (tracer.cpp:1712-1714)
+ADV_CENTERED, ADV_FLUX, ADV_TSPAS = 0, 1, 2
 
 
 class SynthCfg(ctypes.Structure):
@@ -133,7 +135,7 @@ class Case:
 def default_params(hmix=2, ah=1.0e3, dts=3600.0):
     return dict(dts=float(dts), onbc=0.25, oncc=0.5, dwndmix=10.0e-4, gamma=0.0,
                 od0=1.0 / D0, od0cp=1.0 / (D0 * CP), ah=float(ah), hmix=int(hmix),
-                boundary_restore=0, sssnorm=1, convadj=1)
+                boundary_restore=0, sssnorm=1, convadj=1, adv=ADV_TSPAS, nss=1)
 
 
 def make_cfg(**kw) -> SynthCfg:

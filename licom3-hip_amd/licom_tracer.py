@@ -40,7 +40,7 @@ LICOM_STEP_VTL_SCRATCH = 0x4
 LICOM_STEP_STAGED = 0x8
 LICOM_STEP_ICESNOW = 0x10
 HMIX_DEL2, HMIX_DEL4 = 2, 4
-ADV_TSPAS = 2
+ADV_CENTERED, ADV_FLUX, ADV_TSPAS = 0, 1, 2   # the reference's Str codes (tracer.cpp:1712-1714)
 
 
 class Grid(ctypes.Structure):
@@ -49,7 +49,7 @@ class Grid(ctypes.Structure):
                 ("kmt", _IP), ("zkp", _DP), ("pen", _DP),
                 ("htw", _DP), ("hts", _DP), ("hun", _DP), ("hue", _DP), ("tarea", _DP),
                 ("au0", _DP), ("aus", _DP), ("auw", _DP), ("ausw", _DP),
-                ("to", _DP), ("so", _DP), ("eosc", _DP)]
+                ("to", _DP), ("so", _DP), ("eosc", _DP), ("dxu", _DP), ("dyu", _DP)]
 
 
 class Params(ctypes.Structure):
@@ -57,7 +57,8 @@ class Params(ctypes.Structure):
                 ("dwndmix", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("od0", ctypes.c_double), ("od0cp", ctypes.c_double), ("ah", ctypes.c_double),
                 ("hmix", ctypes.c_int), ("adv", ctypes.c_int),
-                ("boundary_restore", ctypes.c_int), ("sssnorm", ctypes.c_int)]
+                ("boundary_restore", ctypes.c_int), ("sssnorm", ctypes.c_int),
+                ("nss", ctypes.c_int)]
 
 
 class Fields(ctypes.Structure):
@@ -236,6 +237,9 @@ def make_grid(case, keep):
     for f in ("zkp", "pen", "htw", "hts", "hun", "hue", "tarea", "au0", "aus", "auw", "ausw",
               "to", "so", "eosc"):
         setattr(g, f, _d(cg[f]))
+    for f in ("dxu", "dyu"):
+        if f in cg:
+            setattr(g, f, _d(cg[f]))
     return g
 
 
@@ -244,9 +248,10 @@ def make_params(p):
     for f in ("dts", "onbc", "oncc", "dwndmix", "gamma", "od0", "od0cp", "ah"):
         setattr(P, f, float(p[f]))
     P.hmix = int(p["hmix"])
-    P.adv = ADV_TSPAS
+    P.adv = int(p.get("adv", ADV_TSPAS))
     P.boundary_restore = int(p.get("boundary_restore", 0))
     P.sssnorm = int(p.get("sssnorm", 1))
+    P.nss = int(p.get("nss", 1))
     return P
 
 

@@ -439,6 +439,90 @@ void orc_advect(const orc_grid *g, double dts, const double *T, const double *uw
     free(b);
 }
 
+/* advection_tracer_1_0 (tracer.cpp:83-119, Fortran twin advection.F90:169-188):
+ * face transports from the U-point velocities wkd (u), wkb (v): centered and
+ * tspas (adv 0, 2) scale the sum of the face's two U corners by hts / htw,
+ * flux (adv 1) weights each corner by its U-cell width dxu / dyu.  Faces
+ * outside [1,imt-1) (v) and [1,jmt-1) (u) are left unwritten. */
+void orc_faces(const orc_grid *g, int adv, const double *wkd, const double *wkb, double *uw,
+               double *vs) {
+    const int imt = g->imt, jmt = g->jmt, km = g->km;
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < km; ++k)
+        for (int j = 0; j < jmt; ++j)
+            for (int i = 0; i < imt; ++i) {
+                size_t c = I3(k, j, i);
+                if (adv == 1) {
+                    if (i >= 1 && i < imt - 1)
+                        vs[c] = (wkb[c] * g->dxu[I2(j, i)] + wkb[c + 1] * g->dxu[I2(j, i + 1)]) * 0.25;
+                    if (j >= 1 && j < jmt - 1)
+                        uw[c] = (wkd[I3(k, j - 1, i)] * g->dyu[I2(j - 1, i)] + wkd[c] * g->dyu[I2(j, i)]) * 0.25;
+                } else {
+                    if (i >= 1 && i < imt - 1)
+                        vs[c] = (wkb[c] + wkb[c + 1]) * g->hts[I2(j, i)] * 0.25;
+                    if (j >= 1 && j < jmt - 1)
+                        uw[c] = (wkd[I3(k, j - 1, i)] + wkd[c]) * g->htw[I2(j, i)] * 0.25;
+                }
+            }
+}
+
+/* advection_tracer_1 (tracer.cpp:120-289, Fortran twin advection.F90:190-247):
+ * the centered (adv=0) and flux (adv=1) forms on the physical cells.
+ *   centered: adv_tt = -(u.grad T) with the face transports, vertical terms
+ *             -0.5*odzp*(w_k (T_{k-1}-T_k) + w_{k+1} (T_k-T_{k+1})); ax, ay,
+ *             az accumulate the x, y and z parts divided by nss.
+ *   flux:     adv_tt = (F_{i+1}-F_i + G_j-G_{j-1})*tarea_r with F = u_w (T+T),
+ *             minus odzp*(Z_{k+1}-Z_k) with Z = w (T+T)/2; no diagnostics.
+ * The reference's centered kernel runs its level-0 and level-(km-1)
+ * statements in every k thread (tracer.cpp:137-160 and 210-256 carry no k
+ * guard), so on the GPU those two levels' adv_tt and ax/ay/az increments are
+ * applied a schedule-dependent number of times; this restatement applies
+ * them once, as the Fortran twin does.  Level 0 and km-1 use the HIP
+ * statements' single vertical term (the Fortran's 0 + z2 differs only in the
+ * sign of a zero). */
+void orc_advect_cf(const orc_grid *g, int adv, int nss, const double *T, const double *uw,
+                   const double *vs, const double *ws, double *adv_tt, double *ax, double *ay,
+                   double *az) {
+    const int imt = g->imt, jmt = g->jmt, km = g->km;
+    const double *odzp = g->odzp;
+    const double p5 = 0.5;
+#pragma omp parallel for collapse(2) schedule(static)
+    for (int k = 0; k < km; ++k)
+        for (int j = 2; j < jmt - 2; ++j)
+            for (int i = 2; i < imt - 2; ++i) {
+                const size_t c = I3(k, j, i);
+                const double tr = g->tarea_r[I2(j, i)];
+                const double Tc = T[c], Tw = T[c - 1], Te = T[c + 1];
+                const double Tn = T[I3(k, j - 1, i)], Ts = T[I3(k, j + 1, i)];
+                const double Uw = uw[c], Ue = uw[c + 1];
+                const double Vs = vs[c], Vn = vs[I3(k, j - 1, i)];
+                if (adv == 0) {
+                    double a = (-Uw * (Tc - Tw) - Ue * (Te - Tc) - Vs * (Ts - Tc) - Vn * (Tc - Tn)) * tr;
+                    if (ax) ax[c] = ax[c] + (-Uw * (Tc - Tw) - Ue * (Te - Tc)) * tr / (double)nss;
+                    if (ay) ay[c] = ay[c] + (-Vs * (Ts - Tc) - Vn * (Tc - Tn)) * tr / (double)nss;
+                    double z;
+                    if (k == 0) {
+                        z = ws[I3(1, j, i)] * (Tc - T[I3(1, j, i)]);
+                    } else if (k == km - 1) {
+                        z = ws[c] * (T[I3(k - 1, j, i)] - Tc);
+                    } else {
+                        const double z1 = ws[c] * (T[I3(k - 1, j, i)] - Tc);
+                        const double z2 = ws[I3(k + 1, j, i)] * (Tc - T[I3(k + 1, j, i)]);
+                        z = z1 + z2;
+                    }
+                    a = a - p5 * odzp[k] * z;
+                    if (az) az[c] = az[c] - p5 * odzp[k] * z / (double)nss;
+                    adv_tt[c] = a;
+                } else {
+                    double a = (-Uw * (Tc + Tw) + Ue * (Te + Tc) - Vn * (Tc + Tn) + Vs * (Ts + Tc)) * tr;
+                    const double z1 = (k == 0) ? 0.0 : ws[c] * (T[I3(k - 1, j, i)] + Tc) * p5;
+                    const double z2 =
+                        (k == km - 1) ? 0.0 : ws[I3(k + 1, j, i)] * (Tc + T[I3(k + 1, j, i)]) * p5;
+                    adv_tt[c] = a - odzp[k] * (z2 - z1);
+                }
+            }
+}
+
 void orc_del2_level(const orc_grid *g, const double *X, int k, int use_ahf, double ah,
                     double *out) {
     const int imt = g->imt, jmt = g->jmt;
@@ -546,18 +630,11 @@ void orc_tracer_step(const orc_grid *g, const orc_params *p, orc_fields *f,
         memset(tf, 0, n3 * sizeof(double));
         memset(adv, 0, n3 * sizeof(double));
 
-        /* advection_tracer_1_0, tspas branch (tracer.cpp:83-104) */
-#pragma omp parallel for schedule(static)
-        for (int k = 0; k < km; ++k)
-            for (int j = 0; j < jmt; ++j)
-                for (int i = 0; i < imt; ++i) {
-                    size_t c = I3(k, j, i);
-                    if (i >= 1 && i < imt - 1)
-                        vs[c] = (wkb[c] + wkb[c + 1]) * g->hts[I2(j, i)] * 0.25;
-                    if (j >= 1 && j < jmt - 1)
-                        uw[c] = (wkd[I3(k, j - 1, i)] + wkd[c]) * g->htw[I2(j, i)] * 0.25;
-                }
-        tspas(g, p->dts, T, uw, vs, f->ws, at00, atmax, atmin, adv, ax, ay, az);
+        orc_faces(g, p->adv, wkd, wkb, uw, vs);
+        if (p->adv == 2)
+            tspas(g, p->dts, T, uw, vs, f->ws, at00, atmax, atmin, adv, ax, ay, az);
+        else
+            orc_advect_cf(g, p->adv, p->nss, T, uw, vs, f->ws, adv, ax, ay, az);
 
         /* horizontal mixing + AKT floor (tracer.cpp:1167-1362) */
         if (g->hmix == 4) {

@@ -49,6 +49,8 @@ typedef struct orc_grid {
     double *dtn, *dts, *dte, *dtw, *ahf;         /* [jmt][imt]               */
     double *dzp, *odzp, *odzt;                   /* [km]                     */
     double area_t;    /* ocean surface area over the interior               */
+    /* U-cell widths (caller fills; read by the flux scheme only, else NULL) */
+    double *dxu, *dyu;                           /* [jmt][imt]               */
 } orc_grid;
 
 typedef struct orc_params {
@@ -61,6 +63,9 @@ typedef struct orc_params {
     int boundary_restore;
     int sssnorm;      /* 1: global freshwater-flux normalisation of S       */
     int convadj;      /* 1: run convadj after the tracer step               */
+    int adv;          /* 0 centered, 1 flux, 2 tspas (tracer.cpp:1712-1714)  */
+    int nss;          /* steps per diagnostic period: the centered scheme
+                         accumulates ax/ay/az increments divided by nss      */
 } orc_params;
 
 typedef struct orc_fields {
@@ -95,6 +100,16 @@ void orc_convadj(const orc_grid *g, const orc_params *p, orc_fields *f,
  * with face velocities uw, vs and vertical velocity ws; writes adv_tt. */
 void orc_advect(const orc_grid *g, double dts, const double *T, const double *uw,
                 const double *vs, const double *ws, double *adv_tt);
+/* advection_tracer_1_0 face transports uw, vs [km][jmt][imt] from the U-point
+ * velocities wkd, wkb for scheme adv (0 centered, 1 flux, 2 tspas). */
+void orc_faces(const orc_grid *g, int adv, const double *wkd, const double *wkb, double *uw,
+               double *vs);
+/* centered (adv=0) or flux (adv=1) advection of one tracer (advection_tracer_1,
+ * tracer.cpp:120-289): writes adv_tt on the physical cells; the centered form
+ * also adds its increments / nss to ax, ay, az (any of them may be NULL). */
+void orc_advect_cf(const orc_grid *g, int adv, int nss, const double *T, const double *uw,
+                   const double *vs, const double *ws, double *adv_tt, double *ax, double *ay,
+                   double *az);
 /* One 5-point del2 pass hdtk = fac*(cc*X + sum c*X_nbr) on the physical cells
  * of level k, fac = AH (del2 build) or AHF (first del4 pass). */
 void orc_del2_level(const orc_grid *g, const double *X, int k, int use_ahf, double ah,

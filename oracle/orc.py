@@ -28,7 +28,7 @@ class OrcGrid(ctypes.Structure):
                 ("tarea_r", _DP), ("ohbt", _DP), ("ohbu", _DP),
                 ("dtn", _DP), ("dts", _DP), ("dte", _DP), ("dtw", _DP), ("ahf", _DP),
                 ("dzp", _DP), ("odzp", _DP), ("odzt", _DP),
-                ("area_t", ctypes.c_double)]
+                ("area_t", ctypes.c_double), ("dxu", _DP), ("dyu", _DP)]
 
 
 class OrcParams(ctypes.Structure):
@@ -36,7 +36,7 @@ class OrcParams(ctypes.Structure):
                 ("dwndmix", ctypes.c_double), ("gamma", ctypes.c_double),
                 ("od0", ctypes.c_double), ("od0cp", ctypes.c_double), ("ah", ctypes.c_double),
                 ("boundary_restore", ctypes.c_int), ("sssnorm", ctypes.c_int),
-                ("convadj", ctypes.c_int)]
+                ("convadj", ctypes.c_int), ("adv", ctypes.c_int), ("nss", ctypes.c_int)]
 
 
 class OrcFields(ctypes.Structure):
@@ -73,6 +73,9 @@ def lib():
         _LIB.orc_icesnow.argtypes = [ctypes.POINTER(OrcGrid), _DP, _DP, ctypes.c_int]
         _LIB.orc_nthreads.restype = ctypes.c_int
         _LIB.orc_advect.argtypes = [ctypes.POINTER(OrcGrid), ctypes.c_double, _DP, _DP, _DP, _DP, _DP]
+        _LIB.orc_advect_cf.argtypes = [ctypes.POINTER(OrcGrid), ctypes.c_int, ctypes.c_int,
+                                       _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP]
+        _LIB.orc_faces.argtypes = [ctypes.POINTER(OrcGrid), ctypes.c_int, _DP, _DP, _DP, _DP]
         _LIB.orc_del2_level.argtypes = [ctypes.POINTER(OrcGrid), _DP, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_double, _DP]
     return _LIB
@@ -109,7 +112,7 @@ class Oracle:
         gr.hmix = int(case.params["hmix"])
         gr.kmt = _i(self.kmt)
         for f in ("zkp", "pen", "htw", "hts", "hun", "hue", "tarea", "au0", "aus", "auw", "ausw",
-                  "to", "so", "eosc"):
+                  "to", "so", "eosc", "dxu", "dyu"):
             setattr(gr, f, _d(g[f]))
         gr.kmu = _i(self.d["kmu"])
         for f in ("vit", "viv", "tarea_r", "ohbt", "ohbu", "dtn", "dts", "dte", "dtw", "ahf",
@@ -124,6 +127,22 @@ class Oracle:
         args = [np.ascontiguousarray(a, np.float64) for a in (T, uw, vs, ws)]
         lib().orc_advect(ctypes.byref(self.g), float(dts), *[_d(a) for a in args], _d(adv))
         return adv
+
+    def faces(self, adv, wkd, wkb):
+        """advection_tracer_1_0 face transports (uw, vs) of scheme adv."""
+        uw, vs = np.zeros_like(wkd), np.zeros_like(wkb)
+        a, b = (np.ascontiguousarray(x, np.float64) for x in (wkd, wkb))
+        lib().orc_faces(ctypes.byref(self.g), int(adv), _d(a), _d(b), _d(uw), _d(vs))
+        return uw, vs
+
+    def advect_cf(self, adv, T, uw, vs, ws, nss=1, ax=None, ay=None, az=None):
+        """Centered (adv=0) or flux (adv=1) advection of one tracer; ax, ay,
+        az (or None) accumulate the centered increments / nss in place."""
+        out = np.zeros_like(T)
+        args = [np.ascontiguousarray(a, np.float64) for a in (T, uw, vs, ws)]
+        lib().orc_advect_cf(ctypes.byref(self.g), int(adv), int(nss), *[_d(a) for a in args], _d(out),
+                            _d(ax), _d(ay), _d(az))
+        return out
 
     def del2_level(self, X, k, use_ahf=False, ah=1.0):
         out = np.zeros_like(X)
