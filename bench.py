@@ -193,31 +193,46 @@ def write_read_bandwidth(torch, nbytes, reps=5):
 
 
 def cpu_baseline(args):
-    """CPU restatement on a 3600 x ny_band x 55 band of the same workload
-    (Mercator rows around the equator, del4, dt=600), median of timed steps
-    after one warm-up step."""
+    """CPU restatement (oracle/, C + OpenMP) on bands of the same workload
+    (3600 x ny_band x 55 Mercator rows around the equator, del4, dt=600),
+    median of timed steps after one warm-up step: on every OpenMP thread of
+    this process (the host's CPU share), then on one thread over a narrower
+    band."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import licom_synth as ls
     import orc
-    band = args.cpu_band
-    case = ls.make_case("C3", ny=band)
-    o = orc.Oracle(case)
-    st = case.copy_state()
-    dg = orc.Oracle.new_diag(case)
-    o.step(st, dg)
-    times = []
-    t_end = time.time() + args.cpu_seconds
-    while len(times) < 5 or (time.time() < t_end and len(times) < 20):
-        t0 = time.perf_counter()
+
+    def timed(band, seconds, nmin, nmax):
+        case = ls.make_case("C3", ny=band)
+        o = orc.Oracle(case)
+        st = case.copy_state()
+        dg = orc.Oracle.new_diag(case)
         o.step(st, dg)
-        times.append(time.perf_counter() - t0)
-    times.sort()
-    med = times[len(times) // 2]
-    upd = case.nx * case.ny * case.km * 2
-    return {"value": upd / med, "unit": "tracer cell-updates/s", "cores": orc.lib().orc_nthreads(),
+        times = []
+        t_end = time.time() + seconds
+        while len(times) < nmin or (time.time() < t_end and len(times) < nmax):
+            t0 = time.perf_counter()
+            o.step(st, dg)
+            times.append(time.perf_counter() - t0)
+        times.sort()
+        upd = case.nx * case.ny * case.km * 2
+        return upd / times[len(times) // 2], len(times), case
+
+    threads = orc.lib().orc_nthreads()
+    v_all, n_all, case = timed(args.cpu_band, args.cpu_seconds, 5, 20)
+    orc.lib().orc_set_nthreads(1)
+    band1 = max(4, args.cpu_band // 8)
+    v_one, n_one, case1 = timed(band1, args.cpu_seconds / 2, 3, 10)
+    orc.lib().orc_set_nthreads(threads)
+    return {"value": v_all, "unit": "tracer cell-updates/s", "cores": threads,
             "kind": "port",
-            "sample": f"oracle (C, OpenMP) on a {case.nx}x{case.ny}x{case.km} band of the 0.1 deg "
-                      f"workload, median of {len(times)} steps after 1 warm-up"}
+            "sample": f"oracle (C, OpenMP, {threads} threads) on a {case.nx}x{case.ny}x{case.km} band "
+                      f"of the 0.1 deg workload, median of {n_all} steps after 1 warm-up",
+            "one_core": {"value": v_one, "cores": 1,
+                         "sample": f"the same on 1 thread, {case1.nx}x{case1.ny}x{case1.km} band, "
+                                   f"median of {n_one} steps"},
+            "host_cpus": os.cpu_count(),
+            "threads_note": "OpenMP threads = OMP_NUM_THREADS (the host CPU share given to this job)"}
 
 
 def run_icesnow(model, args, torch, lt, case, plain):
@@ -317,8 +332,8 @@ def run_ensemble(args, torch, lt, le):
     autotuner, 24 steps captured in a HIP graph and replayed."""
     t0 = time.time()
     cases = le.ensemble_cases(args.members)
-    tuner = le.Autotuner(os.path.join("gpurun_out", "autotune_cache.json")
-                         if os.path.isdir("gpurun_out") else None, reps=3)
+    # launch shapes tuned afresh in every run (args.autotune_cache keeps them)
+    tuner = le.Autotuner(args.autotune_cache, reps=5)
     tuning = [tuner.tune(c) for c in cases]
     print(f"[bench] ensemble: {len(cases)} members generated and tuned in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
@@ -447,6 +462,9 @@ def main():
     ap.add_argument("--no-error-study", action="store_true")
     ap.add_argument("--no-icesnow", action="store_true")
     ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--autotune-cache", default=None,
+                    help="JSON file to read and write the ensemble's tuned launch shapes "
+                         "(default: none, shapes are tuned in every run)")
     ap.add_argument("--ensemble-replays", type=int, default=3)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--requests", type=int, default=64)

@@ -255,8 +255,11 @@ int licom_batch_sync(licom_batch *b);
 void licom_batch_destroy(licom_batch *b);
 
 /* Autotuner: times every licom_member_tuning candidate on this member (reps
- * steps each, HIP events, a one-member batch) and returns the fastest with
- * its milliseconds per step.  Advances the member's state. */
+ * steps each, HIP events, a one-member batch per candidate) in three
+ * interleaved rounds, keeps each candidate's fastest round and returns the
+ * fastest candidate with its milliseconds per step -- the default shape
+ * {0, 0} unless another is faster by more than 2% (run-to-run noise is about
+ * 1%).  Advances the member's state. */
 int licom_autotune_member(licom_ctx *ctx, const licom_fields_mixed *f, const licom_diag_mixed *d,
                           int reps, licom_member_tuning *best, double *best_ms);
 

@@ -29,6 +29,14 @@ int orc_nthreads(void) {
 #endif
 }
 
+void orc_set_nthreads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------ halo */
 /* POP_HaloUpdate for one block covering the whole domain with a cyclic
  * east-west boundary and closed north-south boundaries (domain.F90:162-163):

@@ -123,6 +123,7 @@ double orc_dens(const double *eosc, int km, int k, double t, double s);
  * when reset is nonzero (the FRC_CORE build). */
 void orc_icesnow(const orc_grid *g, double *at, double *licomqice, int reset);
 int orc_nthreads(void);
+void orc_set_nthreads(int n);   /* OpenMP threads of the following calls */
 
 #ifdef __cplusplus
 }

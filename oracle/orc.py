@@ -72,6 +72,7 @@ def lib():
         _LIB.orc_halo_fill_f64.argtypes = [_DP, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _LIB.orc_icesnow.argtypes = [ctypes.POINTER(OrcGrid), _DP, _DP, ctypes.c_int]
         _LIB.orc_nthreads.restype = ctypes.c_int
+        _LIB.orc_set_nthreads.argtypes = [ctypes.c_int]
         _LIB.orc_advect.argtypes = [ctypes.POINTER(OrcGrid), ctypes.c_double, _DP, _DP, _DP, _DP, _DP]
         _LIB.orc_advect_cf.argtypes = [ctypes.POINTER(OrcGrid), ctypes.c_int, ctypes.c_int,
                                        _DP, _DP, _DP, _DP, _DP, _DP, _DP, _DP]

@@ -70,6 +70,12 @@ def case_for_shim():
 
 
 def main(mode):
+    if mode == "gpu":
+        # the HIP runtime torch loads must be the process's first, as in every
+        # other caller here (torch initialises the device before the library
+        # is loaded); loading the shim first would bring in a second runtime
+        import torch
+        torch.cuda.init()
     L = load()
     keep = []
     case = case_for_shim()
