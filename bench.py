@@ -338,18 +338,27 @@ def run_ensemble(args, torch, lt, le):
     print(f"[bench] ensemble: {len(cases)} members generated and tuned in {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
     models = [lt.TracerModel(c, diagnostics=True, precision="mixed") for c in cases]
-    ens = le.Ensemble(models, tuning=tuning)
-    ups = ens.cell_updates_per_step()
-    ens.capture(24)
-    ens.replay()               # warm-up
-    ens.sync()
     reps = max(1, args.ensemble_replays)
-    t1 = time.perf_counter()
-    for _ in range(reps):
-        ens.replay()
-    ens.sync()
-    dt = time.perf_counter() - t1
-    ens.close()
+
+    def timed(persistent):
+        # 24 steps captured in a graph: one persistent launch per step, or
+        # one launch per stage and step
+        ens = le.Ensemble(models, tuning=tuning, persistent=persistent)
+        ups_ = ens.cell_updates_per_step()
+        grid = ens.persistent_grid()
+        ens.capture(24)
+        ens.replay()               # warm-up
+        ens.sync()
+        t1 = time.perf_counter()
+        for _ in range(reps):
+            ens.replay()
+        ens.sync()
+        dt_ = time.perf_counter() - t1
+        ens.close()
+        return ups_, dt_, grid
+
+    ups, dt, pgrid = timed(True)
+    _, dt_stages, _ = timed(False)
     # isolated single-member step time per shape (for the serving slowdown)
     iso = {}
     for c in cases:
@@ -370,8 +379,15 @@ def run_ensemble(args, torch, lt, le):
         m.close()
     torch.cuda.empty_cache()
     value = ups * 24 * reps / dt
+    sb = STEP_BYTES_PER_UPDATE["mixed"]
     return {"members": len(cases), "value": value, "unit": "tracer cell-updates/s",
             "ms_per_step": 1000.0 * dt / (24 * reps), "graph_steps": 24, "replays": reps,
+            "launches_per_step": 1, "persistent_workgroups": pgrid,
+            "per_stage_launches": {"ms_per_step": 1000.0 * dt_stages / (24 * reps),
+                                   "value": ups * 24 * reps / dt_stages, "launches_per_step": 9},
+            "step_roofline": {"bytes_per_update": sb, "achieved": round(sb * value / 1e9, 1),
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(sb * value / 1e9 / HBM_PEAK_GBS, 4)},
             "cell_updates_per_step": ups, "dtype": "f64/f32",
             "tuning": {k: v for k, v in tuner.table.items()},
             "isolated_step_ms": {f"{a}x{b}x{c}": round(1000 * t, 4) for (a, b, c), t in iso.items()}}, iso

@@ -244,6 +244,21 @@ int licom_batch_create(licom_ctx *const *members, const licom_fields_mixed *fiel
 int licom_batch_set_stream(licom_batch *b, void *hip_stream);
 /* Workgroups of the persistent k-march; 0 = one workgroup per item. */
 int licom_batch_set_march_grid(licom_batch *b, int workgroups);
+/* on != 0: each step is ONE launch (steps and captured graphs alike): a
+ * grid of co-resident workgroups (occupancy x CUs) walks every stage's work
+ * list, with a grid-wide barrier between dependent phases (surface |
+ * SSSNORM partials | SSSNORM total + prelude | k-march | solve | side effects
+ * + halo).  Same results bit for bit as the default of one launch per stage.
+ * The barrier waits are bounded: if the workgroups cannot all be resident
+ * (other work sharing the device) a wait times out after about a second and
+ * licom_batch_sync returns LICOM_EHIP (the members' state is then
+ * undefined), so run persistent batches on an otherwise idle device. */
+int licom_batch_set_persistent(licom_batch *b, int on);
+/* Workgroups of the persistent step (0 before licom_batch_set_persistent). */
+int licom_batch_persistent_grid(const licom_batch *b);
+/* Test hook: makes the next persistent step's first barrier report a
+ * timeout (every workgroup leaves at once), as licom_batch_sync then shows. */
+int licom_batch_fail_next_barrier(licom_batch *b);
 /* Enqueues nsteps steps of every member (flags: CONVADJ, SIDE_EFFECTS,
  * ICESNOW when every member has a licomqice array). */
 int licom_batch_step(licom_batch *b, int nsteps, int flags);

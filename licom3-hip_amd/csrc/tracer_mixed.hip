@@ -1011,6 +1011,191 @@ __global__ __launch_bounds__(256) void kb_halo(const MemberDev *M, const int *st
 
 #undef LICOM_MEMBER
 
+// ------------------------------------------------- persistent batch step
+// One launch per step: a grid of co-resident workgroups walks every stage's
+// work list (the same items and bodies as the kb_* launches, so the same
+// bits), with a grid-wide barrier between dependent phases:
+//   surface | SSSNORM partials | SSSNORM total + column prelude | k-march |
+//   column solve | side effects + halo.
+// The barrier waits are bounded: a workgroup that waits longer than
+// kBarSpins polls (workgroups that are not co-resident, for example with
+// other work on the device) sets the error flag and leaves, every later
+// barrier then lets all workgroups leave at once, and licom_batch_sync
+// reports the error.  The host can set the flag itself to check that path.
+struct GridBar {
+    unsigned count;   // arrivals at the current barrier
+    unsigned gen;     // barrier generation
+    int err;          // set by a timed-out wait (or by the host)
+    int pad;
+};
+constexpr unsigned long long kBarSpins = 1ull << 24;   // ~1 s of s_sleep 2
+
+struct StageTotals {
+    int t[NSTAGE];
+};
+
+__device__ __forceinline__ bool grid_sync(GridBar *b, unsigned nblk) {
+    __shared__ int s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        if (__hip_atomic_load(&b->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            ok = 0;
+        } else {
+            // the generation is read before arriving: the last arrival may
+            // advance it at any time after its own atomicAdd
+            const unsigned gen = __hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();   // this workgroup's results, before it arrives
+            const unsigned arrived =
+                __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+            if (arrived == nblk) {
+                __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                unsigned long long spins = 0;
+                while (__hip_atomic_load(&b->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                    if (__hip_atomic_load(&b->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
+                        ++spins > kBarSpins) {
+                        __hip_atomic_store(&b->err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __threadfence();   // the other workgroups' results, after the barrier
+        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+template <int KM, bool ICE>
+__global__ __launch_bounds__(256, 2) void kb_step(const MemberDev *M, const int *starts, int nm,
+                                               StageTotals T, int flags, GridBar *bar) {
+    const int G = (int)gridDim.x;
+    const unsigned nblk = gridDim.x;
+    auto member = [&](int stage, int it, int &l) -> const MemberDev & {
+        const int *st_ = starts + stage * (nm + 1);
+        const int m = find_member(st_, nm, it);
+        l = it - st_[m];
+        return M[m];
+    };
+    // surface fluxes, T->U average, SSSNORM integrand
+    for (int it = blockIdx.x; it < T.t[ST_SURF]; it += G) {
+        int l;
+        const MemberDev &d = member(ST_SURF, it, l);
+        const SurfaceIO o{d.h0f, d.h0l, d.atb, d.swv, d.nswv, d.sst, d.seaice, d.sss, d.fresh,
+                          d.stf_pre, d.work_u, d.stf_n, d.temp11, d.net, d.mix_count};
+        const int bx = l % d.sf_bx, by = l / d.sf_bx;
+        surface_cell(d.g, d.p, o, bx * 64 + (int)threadIdx.x % 64, by * 4 + (int)threadIdx.x / 64);
+    }
+    if (!grid_sync(bar, nblk)) return;
+    for (int it = blockIdx.x; it < T.t[ST_SSSP]; it += G) {
+        int l;
+        const MemberDev &d = member(ST_SSSP, it, l);
+        sss_partial_block(d.g, d.temp11, d.partials, l, d.nblk);
+        __syncthreads();
+    }
+    if (!grid_sync(bar, nblk)) return;
+    // SSSNORM total and the column prelude (both need only the surface)
+    for (int it = blockIdx.x; it < T.t[ST_SSSF] + T.t[ST_FD]; it += G) {
+        int l;
+        if (it < T.t[ST_SSSF]) {
+            const MemberDev &d = member(ST_SSSF, it, l);
+            sss_final_block(d.partials, d.nblk, d.area_t, d.p.sssnorm, d.err_norm2, d.fw_norm2);
+        } else {
+            const MemberDev &d = member(ST_FD, it - T.t[ST_SSSF], l);
+            prelude_col_mx_body<KM>(d.g, (float)d.p.oncc, d.utf, d.utl, d.vtf, d.vtl, d.work_u, d.stf_pre,
+                                    d.ws, d.uw, d.vs, l % d.fd_bx, l / d.fd_bx, d.fd_shape);
+        }
+        __syncthreads();
+    }
+    if (!grid_sync(bar, nblk)) return;
+    for (int it = blockIdx.x; it < T.t[ST_MARCH]; it += G) {
+        int l;
+        const MemberDev &d = member(ST_MARCH, it, l);
+        MarchArgsMx a;
+        a.at = d.at;
+        a.atb = d.atb;
+        a.stf_n = d.stf_n;
+        a.err_norm2 = d.err_norm2;
+        a.swv = d.swv;
+        a.restore = d.restore;
+        a.akt = d.akt;
+        a.uw = d.uw;
+        a.vs = d.vs;
+        a.ws = d.ws;
+        a.ori = d.ori;
+        a.ax = d.ax;
+        a.ay = d.ay;
+        a.az = d.az;
+        a.dz = d.dz;
+        a.pen = d.pen;
+        a.ntx = d.ntx;
+        a.ntiles = d.ntiles;
+        a.tiles_per_xcd = 0;
+        march_body(d.g, d.p, a, l >> 1, l & 1);
+        __syncthreads();
+    }
+    if (!grid_sync(bar, nblk)) return;
+    for (int it = blockIdx.x; it < T.t[ST_SOLVE]; it += G) {
+        int l;
+        const MemberDev &d = member(ST_SOLVE, it, l);
+        solve_body<KM, float, float, ICE>(d.g, d.p, flags, d.at, d.atb, d.akt, d.ori, d.stf_n, d.tend,
+                                          d.dt_diff, d.dt_conv, d.mix_count, l % d.sv_bx, l / d.sv_bx,
+                                          d.sv_shape, ICE ? d.qice : nullptr);
+        __syncthreads();
+    }
+    if (!grid_sync(bar, nblk)) return;
+    // side effects and ghost cells (independent of each other)
+    const int nside = (flags & LICOM_STEP_SIDE_EFFECTS) ? T.t[ST_SIDE3] + T.t[ST_SIDE2] : 0;
+    for (int it = blockIdx.x; it < nside + T.t[ST_HALO]; it += G) {
+        int l;
+        const unsigned tid = threadIdx.x;
+        if (it < T.t[ST_SIDE3] && it < nside) {
+            const MemberDev &d = member(ST_SIDE3, it, l);
+            const size_t n3 = (size_t)d.g.imt * d.g.jmt * d.g.km;
+            const size_t t = (size_t)l * 256 + tid;
+            const float oncc = (float)d.p.oncc;
+            if (d.side_vec) {
+                if (t < n3 / 4) {
+                    float4 *u = reinterpret_cast<float4 *>(d.utf), *v = reinterpret_cast<float4 *>(d.vtf);
+                    float4 a = u[t], b = v[t];
+                    a.x *= oncc; a.y *= oncc; a.z *= oncc; a.w *= oncc;
+                    b.x *= oncc; b.y *= oncc; b.z *= oncc; b.w *= oncc;
+                    u[t] = a;
+                    v[t] = b;
+                }
+            } else if (t < n3) {
+                d.utf[t] *= oncc;
+                d.vtf[t] *= oncc;
+            }
+        } else if (it < nside) {
+            const MemberDev &d = member(ST_SIDE2, it - T.t[ST_SIDE3], l);
+            const KGrid &g = d.g;
+            const size_t n2 = (size_t)g.imt * g.jmt, n3 = n2 * g.km;
+            const size_t t = (size_t)l * 256 + tid;
+            if (t < n2) {
+                d.h0f[t] = d.h0f[t] * d.p.onbc;
+                const int j = (int)(t / g.imt), i = (int)(t % g.imt);
+                if (i >= 1 && i < g.imt - 1 && j >= 1 && j < g.jmt - 1)
+                    for (int n = 0; n < 2; ++n)
+                        d.akt[n * n3 + t] =
+                            (float)dev::floor_akt((double)d.akt[n * n3 + t], 0, n, d.p.hmix, d.p.dwndmix);
+            }
+        } else {
+            const MemberDev &d = member(ST_HALO, it - nside, l);
+            const int plane = l / d.halo_bx, t = (l % d.halo_bx) * 256 + (int)tid;
+            if (plane < 4 * d.g.km)
+                halo_cell(d.g.imt, d.g.jmt, d.g.km, plane, t, d.at, d.atb);
+            else if (ICE)
+                halo_cell(d.g.imt, d.g.jmt, d.g.km, 0, t, d.qice, nullptr);
+        }
+    }
+}
+
 }  // namespace
 }  // namespace licom
 
@@ -1028,12 +1213,68 @@ struct licom_batch {
     bool all_qice = false;            // every member has a licomqice array
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // one persistent launch per step (licom_batch_set_persistent)
+    bool persistent = false;
+    int step_grid = 0;                // co-resident workgroups of kb_step
+    licom::GridBar *d_bar = nullptr;
 };
 
 namespace {
 
+int batch_launch_persistent(licom_batch *b, int flags) {
+    using namespace licom;
+    StageTotals T;
+    for (int s = 0; s < NSTAGE; ++s) T.t[s] = b->totals[s];
+    const bool ice = flags & LICOM_STEP_ICESNOW;
+    const dim3 gr((unsigned)b->step_grid);
+    const MemberDev *M = b->d_members;
+    const int *S = b->d_starts;
+    hipStream_t st = b->stream;
+    if (b->kmax <= 30) {
+        if (ice) kb_step<30, true><<<gr, 256, 0, st>>>(M, S, b->n, T, flags, b->d_bar);
+        else kb_step<30, false><<<gr, 256, 0, st>>>(M, S, b->n, T, flags, b->d_bar);
+    } else if (b->kmax <= 55) {
+        if (ice) kb_step<55, true><<<gr, 256, 0, st>>>(M, S, b->n, T, flags, b->d_bar);
+        else kb_step<55, false><<<gr, 256, 0, st>>>(M, S, b->n, T, flags, b->d_bar);
+    } else {
+        if (ice) kb_step<LICOM_MAX_KM, true><<<gr, 256, 0, st>>>(M, S, b->n, T, flags, b->d_bar);
+        else kb_step<LICOM_MAX_KM, false><<<gr, 256, 0, st>>>(M, S, b->n, T, flags, b->d_bar);
+    }
+    return hip_status(hipGetLastError(), "persistent batch launch");
+}
+
+// Workgroups of kb_step that fit on the device at once (the occupancy of the
+// instantiation the batch runs, times the CU count).
+int persistent_grid(licom_batch *b, int &grid) {
+    using namespace licom;
+    int dev = 0, cus = 0, per = 0;
+    if (int rc = hip_status(hipGetDevice(&dev), "hipGetDevice")) return rc;
+    if (int rc = hip_status(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                            "hipDeviceGetAttribute"))
+        return rc;
+    const void *fn = b->kmax <= 30   ? (const void *)kb_step<30, false>
+                     : b->kmax <= 55 ? (const void *)kb_step<55, false>
+                                     : (const void *)kb_step<LICOM_MAX_KM, false>;
+    const void *fi = b->kmax <= 30   ? (const void *)kb_step<30, true>
+                     : b->kmax <= 55 ? (const void *)kb_step<55, true>
+                                     : (const void *)kb_step<LICOM_MAX_KM, true>;
+    int per_i = 0;
+    if (int rc = hip_status(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 256, 0), "occupancy"))
+        return rc;
+    if (int rc = hip_status(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_i, fi, 256, 0), "occupancy"))
+        return rc;
+    per = std::min(per, per_i);
+    if (per < 1 || cus < 1) {
+        set_error("persistent batch step: no workgroup of kb_step fits on a CU");
+        return LICOM_ENOTSUP;
+    }
+    grid = per * cus;
+    return LICOM_OK;
+}
+
 int batch_launch_step(licom_batch *b, int flags) {
     using namespace licom;
+    if (b->persistent) return batch_launch_persistent(b, flags);
     const int nm = b->n;
     hipStream_t st = b->stream;
     const MemberDev *M = b->d_members;
@@ -1290,9 +1531,46 @@ extern "C" int licom_batch_replay(licom_batch *b) {
 }
 
 extern "C" int licom_batch_sync(licom_batch *b) {
+    using namespace licom;
     if (!b) return LICOM_EINVAL;
-    return licom::hip_status(hipStreamSynchronize(b->stream), "hipStreamSynchronize");
+    if (int rc = hip_status(hipStreamSynchronize(b->stream), "hipStreamSynchronize")) return rc;
+    if (b->d_bar) {
+        GridBar h{};
+        if (int rc = hip_status(hipMemcpy(&h, b->d_bar, sizeof h, hipMemcpyDeviceToHost), "hipMemcpy barrier"))
+            return rc;
+        if (h.err) {
+            // the steps since the last sync are incomplete; the barrier starts afresh
+            (void)hipMemset(b->d_bar, 0, sizeof(GridBar));
+            set_error("persistent batch step: a grid barrier wait timed out (workgroups not co-resident, "
+                      "e.g. other work on the device); the members' state is undefined");
+            return LICOM_EHIP;
+        }
+    }
+    return LICOM_OK;
 }
+
+extern "C" int licom_batch_set_persistent(licom_batch *b, int on) {
+    using namespace licom;
+    if (!b) return LICOM_EINVAL;
+    if (on && !b->d_bar) {
+        int grid = 0;
+        if (int rc = persistent_grid(b, grid)) return rc;
+        if (int rc = hip_status(hipMalloc(&b->d_bar, sizeof(GridBar)), "hipMalloc barrier")) return rc;
+        if (int rc = hip_status(hipMemset(b->d_bar, 0, sizeof(GridBar)), "hipMemset barrier")) return rc;
+        b->step_grid = grid;
+    }
+    b->persistent = on != 0;
+    return LICOM_OK;
+}
+
+extern "C" int licom_batch_fail_next_barrier(licom_batch *b) {
+    using namespace licom;
+    if (!b || !b->d_bar) return LICOM_EINVAL;
+    const int one = 1;
+    return hip_status(hipMemcpy(&b->d_bar->err, &one, sizeof one, hipMemcpyHostToDevice), "hipMemcpy barrier");
+}
+
+extern "C" int licom_batch_persistent_grid(const licom_batch *b) { return b ? b->step_grid : 0; }
 
 extern "C" void licom_batch_destroy(licom_batch *b) {
     if (!b) return;
@@ -1301,6 +1579,7 @@ extern "C" void licom_batch_destroy(licom_batch *b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->d_members) (void)hipFree(b->d_members);
     if (b->d_starts) (void)hipFree(b->d_starts);
+    if (b->d_bar) (void)hipFree(b->d_bar);
     if (b->own_stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }

@@ -55,6 +55,9 @@ def _lib():
         L.licom_batch_replay.argtypes = [V]
         L.licom_batch_sync.argtypes = [V]
         L.licom_batch_destroy.argtypes = [V]
+        L.licom_batch_set_persistent.argtypes = [V, ctypes.c_int]
+        L.licom_batch_persistent_grid.argtypes = [V]
+        L.licom_batch_fail_next_barrier.argtypes = [V]
         L.licom_autotune_member.argtypes = [V, ctypes.POINTER(lt.Fields), ctypes.POINTER(lt.Diag),
                                             ctypes.c_int, ctypes.POINTER(MemberTuning),
                                             ctypes.POINTER(ctypes.c_double)]
@@ -63,9 +66,13 @@ def _lib():
 
 
 class Ensemble:
-    """Members (mixed-precision TracerModels) advanced as one batch."""
+    """Members (mixed-precision TracerModels) advanced as one batch.
 
-    def __init__(self, members, tuning=None, march_grid=0, stream=None):
+    persistent=True: one launch per step (licom_batch_set_persistent), a grid
+    of co-resident workgroups with grid-wide barriers between the stages;
+    otherwise one launch per stage.  Same bits either way."""
+
+    def __init__(self, members, tuning=None, march_grid=0, stream=None, persistent=False):
         if not members:
             raise lt.LicomError("Ensemble needs at least one member")
         for m in members:
@@ -94,6 +101,9 @@ class Ensemble:
         self.member_stream = members[0].stream
         if march_grid:
             lt.check(L.licom_batch_set_march_grid(h, int(march_grid)), "licom_batch_set_march_grid")
+        self.persistent = bool(persistent)
+        if self.persistent:
+            lt.check(L.licom_batch_set_persistent(h, 1), "licom_batch_set_persistent")
         self.captured = 0
 
     def _order(self):
@@ -121,6 +131,10 @@ class Ensemble:
         """Waits for the batch's work; afterwards the members' host_state()
         reads the advanced state."""
         lt.check(_lib().licom_batch_sync(self.handle), "licom_batch_sync")
+
+    def persistent_grid(self):
+        """Workgroups of the persistent step (0 unless persistent)."""
+        return int(_lib().licom_batch_persistent_grid(self.handle))
 
     def cell_updates_per_step(self):
         return sum(m.case.nx * m.case.ny * m.case.km * 2 for m in self.members)

@@ -83,7 +83,8 @@ class GridDerived(ctypes.Structure):
 EXPORTED = ["licom_tracer_create", "licom_tracer_set_stream", "licom_tracer_step",
             "licom_tracer_step_noalloc",
             "licom_tracer_step_mixed", "licom_batch_create", "licom_batch_set_stream",
-            "licom_batch_set_march_grid", "licom_batch_step", "licom_batch_capture",
+            "licom_batch_set_march_grid", "licom_batch_set_persistent", "licom_batch_persistent_grid",
+            "licom_batch_fail_next_barrier", "licom_batch_step", "licom_batch_capture",
             "licom_batch_replay", "licom_batch_sync", "licom_batch_destroy",
             "licom_autotune_member", "licom_convadj", "licom_icesnow", "licom_tracer_set_icesnow",
             "licom_tracer_sync", "licom_grid_export", "licom_set_profiling",
