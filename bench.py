@@ -446,13 +446,22 @@ def run_serving(args, torch, lt, le, offline_value, iso, load):
     reqs = le.poisson_requests(args.requests, rate)
     print(f"[bench] serving: {len(reqs)} requests at {rate:.2f}/s (load {load})", file=sys.stderr,
           flush=True)
-    loop = le.RequestLoop(hbm_cap_bytes=args.hbm_cap_gb * 1e9, chunk_steps=args.chunk_steps)
+    loop = le.RequestLoop(hbm_cap_bytes=args.hbm_cap_gb * 1e9, chunk_steps=args.chunk_steps,
+                          policy=args.serve_policy)
     stats = loop.run(reqs)
-    slow = np.array([(r.finished - r.arrival) / (r.n_steps * iso.get((r.case.nx, r.case.ny, r.case.km),
-                                                                      np.nan)) for r in reqs])
-    slow = slow[np.isfinite(slow)]
+    iso_t = lambda r: r.n_steps * iso.get((r.case.nx, r.case.ny, r.case.km), np.nan)  # noqa: E731
+    served = [r for r in reqs if r.finished is not None]
+    slow_of = {r.rid: (r.finished - r.arrival) / iso_t(r) for r in served}
+    slow = np.array([v for v in slow_of.values() if np.isfinite(v)])
+    # where the tail's time goes: the slowest requests' wait for admission
+    # (context creation and upload) and their run in their class
+    worst = sorted(served, key=lambda r: -slow_of[r.rid])[:4]
+    stats["slowest"] = [{"shape": f"{r.case.nx}x{r.case.ny}x{r.case.km}", "steps": r.n_steps,
+                         "isolated_s": round(iso_t(r), 5), "admission_s": round(r.admitted - r.arrival, 5),
+                         "run_s": round(r.finished - r.admitted, 5), "slowdown": round(slow_of[r.rid], 2)}
+                        for r in worst]
     last_arrival = max(r.arrival for r in reqs)
-    stats.update({"arrival_rate_per_s": rate, "load_fraction": load,
+    stats.update({"policy": args.serve_policy, "arrival_rate_per_s": rate, "load_fraction": load,
                   "hbm_cap_gb": args.hbm_cap_gb,
                   "drain_after_last_arrival_s": stats["makespan_s"] - last_arrival,
                   "vs_offline": stats["cell_updates_per_s"] / offline_value,
@@ -486,7 +495,10 @@ def main():
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--loads", default="0.25,0.8",
                     help="request arrival rates as fractions of the offline batched capacity")
-    ap.add_argument("--chunk-steps", type=int, default=8)
+    ap.add_argument("--chunk-steps", type=int, default=4)
+    ap.add_argument("--serve-policy", default="srpt,classes",
+                    help="request loop scheduling, comma-separated: least remaining work first "
+                         "over one batch (srpt), size classes advancing in lock step (classes)")
     ap.add_argument("--hbm-cap-gb", type=float, default=200.0)
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the HBM traffic summary from two rocprofv3 --pmc runs")
@@ -564,8 +576,12 @@ def main():
         import licom_ensemble as le
         ens_out, iso = run_ensemble(args, torch, lt, le)
         if not args.no_serving:
-            serve_out = [run_serving(args, torch, lt, le, ens_out["value"], iso, float(x))
-                         for x in args.loads.split(",")]
+            serve_out = []
+            for pol in args.serve_policy.split(","):
+                args_p = argparse.Namespace(**vars(args))
+                args_p.serve_policy = pol
+                serve_out += [run_serving(args_p, torch, lt, le, ens_out["value"], iso, float(x))
+                              for x in args.loads.split(",")]
 
     cpu = None
     if not args.no_cpu_baseline:

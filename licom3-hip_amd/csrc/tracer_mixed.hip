@@ -1206,6 +1206,7 @@ struct licom_batch {
     int kmax = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    bool stream_set = false;          // licom_batch_set_stream was called
     MemberDev *d_members = nullptr;
     int *d_starts = nullptr;
     int totals[licom::NSTAGE] = {};
@@ -1220,6 +1221,14 @@ struct licom_batch {
 };
 
 namespace {
+
+// The batch's own stream, created on first use unless one was set.
+int ensure_stream(licom_batch *b) {
+    if (b->stream || b->stream_set) return LICOM_OK;
+    const int rc = licom::hip_status(hipStreamCreate(&b->stream), "hipStreamCreate");
+    b->own_stream = rc == LICOM_OK;
+    return rc;
+}
 
 int batch_launch_persistent(licom_batch *b, int flags) {
     using namespace licom;
@@ -1461,10 +1470,8 @@ extern "C" int licom_batch_create(licom_ctx *const *members, const licom_fields_
                              "hipMemcpy batch");
     if (!rc) rc = hip_status(hipMemcpy(b->d_starts, starts.data(), sizeof(int) * starts.size(),
                                        hipMemcpyHostToDevice), "hipMemcpy batch");
-    if (!rc) {
-        rc = hip_status(hipStreamCreate(&b->stream), "hipStreamCreate");
-        b->own_stream = rc == LICOM_OK;
-    }
+    // the batch's own stream is created on first use unless the caller sets
+    // one (the request loop rebuilds batches often and always sets one)
     if (rc) {
         licom_batch_destroy(b);
         return rc;
@@ -1477,6 +1484,7 @@ extern "C" int licom_batch_set_stream(licom_batch *b, void *hip_stream) {
     if (!b) return LICOM_EINVAL;
     if (b->own_stream) (void)hipStreamDestroy(b->stream);
     b->own_stream = false;
+    b->stream_set = true;
     b->stream = static_cast<hipStream_t>(hip_stream);
     return LICOM_OK;
 }
@@ -1490,6 +1498,7 @@ extern "C" int licom_batch_set_march_grid(licom_batch *b, int workgroups) {
 extern "C" int licom_batch_step(licom_batch *b, int nsteps, int flags) {
     if (!b || nsteps < 0) return LICOM_EINVAL;
     if (int rc = check_batch_flags(b, flags)) return rc;
+    if (int rc = ensure_stream(b)) return rc;
     for (int s = 0; s < nsteps; ++s)
         if (int rc = batch_launch_step(b, flags)) return rc;
     return LICOM_OK;
@@ -1499,6 +1508,7 @@ extern "C" int licom_batch_capture(licom_batch *b, int nsteps, int flags) {
     using namespace licom;
     if (!b || nsteps <= 0) return LICOM_EINVAL;
     if (int rc = check_batch_flags(b, flags)) return rc;
+    if (int rc = ensure_stream(b)) return rc;
     if (!b->stream) {
         set_error("licom_batch_capture: graph capture needs a non-default stream");
         return LICOM_EINVAL;

@@ -72,7 +72,8 @@ class Ensemble:
     of co-resident workgroups with grid-wide barriers between the stages;
     otherwise one launch per stage.  Same bits either way."""
 
-    def __init__(self, members, tuning=None, march_grid=0, stream=None, persistent=False):
+    def __init__(self, members, tuning=None, march_grid=0, stream=None, persistent=False,
+                 wait_members=True):
         if not members:
             raise lt.LicomError("Ensemble needs at least one member")
         for m in members:
@@ -98,7 +99,11 @@ class Ensemble:
         s = stream if stream is not None else torch.cuda.Stream(device=members[0].device)
         lt.check(L.licom_batch_set_stream(h, ctypes.c_void_p(s.cuda_stream)), "licom_batch_set_stream")
         self.stream = s
-        self.member_stream = members[0].stream
+        # wait_members: every launch first waits for the members' stream (work
+        # the caller queued there); the request loop, whose members' uploads
+        # are complete when they join, turns this off so that a batch never
+        # waits for the admission of other requests on that stream
+        self.member_stream = members[0].stream if wait_members else None
         if march_grid:
             lt.check(L.licom_batch_set_march_grid(h, int(march_grid)), "licom_batch_set_march_grid")
         self.persistent = bool(persistent)
@@ -107,7 +112,8 @@ class Ensemble:
         self.captured = 0
 
     def _order(self):
-        self.stream.wait_stream(self.member_stream)
+        if self.member_stream is not None:
+            self.stream.wait_stream(self.member_stream)
 
     def step(self, nsteps=1, flags=MIXED_FLAGS):
         self._order()
@@ -276,7 +282,21 @@ class RequestLoop:
 
     def __init__(self, hbm_cap_bytes=200e9, chunk_steps=8, tuner=None, keep_results=False,
                  class_cells=(5.0e5, 2.0e6), one_at_a_time=(), one_at_a_time_upto=None,
-                 free_margin_bytes=2.0e9, free_bytes="device"):
+                 free_margin_bytes=2.0e9, free_bytes="device", policy="classes",
+                 fill_cells=4.0e6, admit_threads=2):
+        if policy not in ("classes", "srpt"):
+            raise ValueError(f"policy must be 'classes' or 'srpt', not {policy!r}")
+        # "srpt": one batch of the resident requests with the least remaining
+        # work (remaining steps x cells per step), as many as fill_cells cells
+        # per step (about what fills the GPU), re-chosen at every chunk
+        # boundary (chunk_steps steps at most), so a short request that
+        # arrives runs at once and leaves as soon as it is done; chunks are
+        # queued two deep while no member of the batch finishes.
+        self.policy = policy
+        self.fill_cells = float(fill_cells)
+        # threads creating the admitted requests' contexts and uploading their
+        # state, each on a stream of its own
+        self.admit_threads = int(admit_threads)
         self.cap = float(hbm_cap_bytes)
         # classes whose members run one at a time, least remaining steps first,
         # instead of advancing together, while at most `one_at_a_time_upto`
@@ -313,6 +333,80 @@ class RequestLoop:
                 return i
         return len(self.class_cells)
 
+    @staticmethod
+    def _remaining(r):
+        return (r.n_steps - r.done_steps) * (r.case.nx * r.case.ny * r.case.km)
+
+    def _srpt_pick(self, active):
+        run, cells = [], 0.0
+        for r in sorted(active, key=lambda r: (self._remaining(r), r.arrival)):
+            c = r.case.nx * r.case.ny * r.case.km
+            if run and cells + c > self.fill_cells:
+                continue
+            run.append(r)
+            cells += c
+        return run
+
+    def _srpt_turn(self, ln, done, grave, clock, t0, torch):
+        """One scheduling turn of the srpt policy on its single lane: retire
+        the chunks that finished, then keep up to two chunks queued.
+        Returns (launched, cell-updates retired, batch rebuilds)."""
+        upd, rb, launched = 0, 0, False
+        # ln.queue: chunks in flight, oldest first: (event, members, steps)
+        q = ln.__dict__.setdefault("queue", [])
+        while q and q[0][0].query():
+            _, members, steps = q.pop(0)
+            t_now = clock() - t0
+            upd += steps * sum(r.case.nx * r.case.ny * r.case.km * 2 for r in members)
+            finished = []
+            for r in members:
+                r.done_steps += steps
+                if r.done_steps >= r.n_steps:
+                    finished.append(r)
+            for r in finished:
+                r.finished = t_now
+                done.append(r)
+                if self.keep_results:
+                    self.results[r.rid] = r.model.host_state()["at"]
+            if finished:
+                grave.append((ln.ens, 0.0))
+                ln.ens = None
+                for r in finished:
+                    grave.append((r.model, r.bytes))
+                    r.model = None
+                ln.active = [r for r in ln.active if r.finished is None]
+                ln.changed = True
+        # queue a chunk when the lane is idle, or a second one behind the
+        # running chunk when no member of it finishes and nothing has joined
+        while ln.active and len(q) < 2:
+            if q:
+                _, members, steps = q[-1]
+                if ln.changed or any(r.done_steps + steps >= r.n_steps for r in members):
+                    break
+                run = members
+                ahead = steps
+            else:
+                run = self._srpt_pick(ln.active)
+                ahead = 0
+            if ln.changed or ln.ens is None or run != ln.running_set:
+                if ln.ens is not None:
+                    grave.append((ln.ens, 0.0))
+                tuning = [self.tuner.tune(r.case) for r in run] if self.tuner else None
+                ln.ens = Ensemble([r.model for r in run], tuning=tuning, stream=ln.stream,
+                                  wait_members=False)
+                ln.changed = False
+                ln.running_set = run
+                rb += 1
+            steps = min(self.chunk, min(r.n_steps - r.done_steps - ahead for r in run))
+            ln.ens.step(steps)
+            ev = torch.cuda.Event()
+            ev.record(ln.stream)
+            q.append((ev, run, steps))
+            launched = True
+        ln.busy = bool(q)
+        ln.running = q[0][1] if q else []
+        return launched, upd, rb
+
     def run(self, requests, clock=time.perf_counter, sleep=time.sleep):
         import queue
         import threading
@@ -341,12 +435,14 @@ class RequestLoop:
         def start_lanes():
             nonlocal dev, worker
             dev = torch.cuda.current_device()
-            for k in range(len(self.class_cells) + 1):
+            nclass = 1 if self.policy == "srpt" else len(self.class_cells) + 1
+            for k in range(nclass):
                 ln = _Lane(torch.cuda.Stream(device=dev))
                 ln.single = k in self.one_at_a_time
                 lanes.append(ln)
-            worker = threading.Thread(target=admit_worker, daemon=True)
-            worker.start()
+            worker = [threading.Thread(target=admit_worker, daemon=True) for _ in range(self.admit_threads)]
+            for w in worker:
+                w.start()
 
         def admit_worker():
             torch.cuda.set_device(dev)
@@ -396,15 +492,22 @@ class RequestLoop:
                 except queue.Empty:
                     break
                 if r is None:
-                    todo.put(None)
+                    for _ in worker:
+                        todo.put(None)
                     raise failures[0]
                 inflight -= 1
                 r.admitted = clock() - t0
-                ln = lanes[self._class(r)]
+                ln = lanes[0 if self.policy == "srpt" else self._class(r)]
                 ln.active.append(r)
                 ln.changed = True
             launched = False
-            for ln in lanes:
+            if self.policy == "srpt" and lanes:
+                ln = lanes[0]
+                launched, upd, rb = self._srpt_turn(ln, done, grave, clock, t0, torch)
+                updates += upd
+                rebuilds += rb
+                grave_bytes = sum(b for _, b in grave)
+            for ln in (lanes if self.policy == "classes" else ()):
                 if ln.busy:
                     if not ln.event.query():
                         continue
@@ -442,7 +545,8 @@ class RequestLoop:
                         if ln.ens is not None:
                             grave.append((ln.ens, 0.0))
                         tuning = [self.tuner.tune(r.case) for r in run] if self.tuner else None
-                        ln.ens = Ensemble([r.model for r in run], tuning=tuning, stream=ln.stream)
+                        ln.ens = Ensemble([r.model for r in run], tuning=tuning, stream=ln.stream,
+                                          wait_members=False)
                         ln.changed = False
                         ln.running_set = run
                         rebuilds += 1
@@ -468,8 +572,10 @@ class RequestLoop:
             elif not launched:
                 sleep(5e-5)
         if worker is not None:
-            todo.put(None)
-            worker.join()
+            for _ in worker:
+                todo.put(None)
+            for w in worker:
+                w.join()
             torch.cuda.synchronize()
         flush()
         makespan = clock() - t0

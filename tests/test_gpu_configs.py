@@ -235,16 +235,17 @@ def test_ensemble_of_64_graph_replay_equals_members_alone(tmp_path, need_gpu):
 
 
 # --------------------------------------------------------- request loop
-def test_request_loop_poisson_requests_same_alone_and_in_the_mix(need_gpu):
+@pytest.mark.parametrize("policy", ["classes", "srpt"])
+def test_request_loop_poisson_requests_same_alone_and_in_the_mix(policy, need_gpu):
     """Requests of the serving workload (domains of the ensemble
     distribution, 24..240 steps, Poisson arrivals) served under a cap that
     holds only part of them at once: every result equals the request run
     alone, bit for bit."""
     reqs = le.poisson_requests(16, rate_per_s=40.0)
     cap = max(0.45 * sum(r.bytes for r in reqs), 1.2 * max(r.bytes for r in reqs))
-    announce(f"request loop: 16 Poisson requests, steps {min(r.n_steps for r in reqs)}.."
+    announce(f"request loop ({policy}): 16 Poisson requests, steps {min(r.n_steps for r in reqs)}.."
              f"{max(r.n_steps for r in reqs)}, cap {cap / 1e9:.2f} GB")
-    loop = le.RequestLoop(hbm_cap_bytes=cap, keep_results=True)
+    loop = le.RequestLoop(hbm_cap_bytes=cap, keep_results=True, policy=policy)
     stats = loop.run(reqs)
     print(f"[loop poisson] {stats}", flush=True)
     assert stats["requests"] == len(reqs)

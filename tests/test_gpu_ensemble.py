@@ -110,15 +110,18 @@ def test_autotuner_picks_a_valid_shape_and_caches(tmp_path, need_gpu):
     assert t2.table[le.Autotuner.key(c)]["ms_per_step"] > 0
 
 
-@pytest.mark.parametrize("single", [(), (0,)], ids=["together", "one-at-a-time"])
-def test_request_loop_results_same_alone_and_in_the_mix(single, need_gpu):
-    announce(f"request loop: 5 requests, continuous batching vs alone, one_at_a_time={single}")
+@pytest.mark.parametrize("single,policy", [((), "classes"), ((0,), "classes"), ((), "srpt")],
+                         ids=["together", "one-at-a-time", "srpt"])
+def test_request_loop_results_same_alone_and_in_the_mix(single, policy, need_gpu):
+    announce(f"request loop ({policy}): 5 requests, continuous batching vs alone, one_at_a_time={single}")
     reqs = []
     for k, (nx, ny, km) in enumerate(SHAPES + ((48, 32, 12), (64, 40, 30))):
         c = le.member_case(4200 + k, nx, ny, km)
-        reqs.append(le.Request(k, 0.002 * k, c, 3 + 2 * k))
+        # all at once, and long enough (a few ms each) that later admissions
+        # find earlier requests still running
+        reqs.append(le.Request(k, 0.0, c, 40 + 10 * k))
     loop = le.RequestLoop(hbm_cap_bytes=2 * le.member_bytes(96, 64, 30), chunk_steps=4,
-                          keep_results=True, one_at_a_time=single)
+                          keep_results=True, one_at_a_time=single, policy=policy)
     stats = loop.run(reqs)
     print(f"[loop] {stats}", flush=True)
     assert stats["requests"] == len(reqs)

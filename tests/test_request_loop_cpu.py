@@ -1,5 +1,7 @@
 """CPU tests of the request loop's admission rules (no GPU: requests that
 can never be admitted are rejected before any device work starts)."""
+import pytest
+
 import licom_ensemble as le
 import licom_synth as ls
 
@@ -47,3 +49,28 @@ def test_request_larger_than_free_device_memory_is_rejected():
     assert stats["rejected"] == 1 and "free" in r.error
     # it waited for its arrival by sleeping, not by spinning
     assert clk.sleeps >= 1 and clk.t >= r.arrival
+
+
+def test_srpt_pick_orders_by_remaining_work_and_fills():
+    """The srpt policy runs the requests with the least remaining work
+    (steps x cells per step) first, as many as fill the GPU's cells per step,
+    and always at least one."""
+    loop = le.RequestLoop(policy="srpt", fill_cells=1.0e6, free_bytes=None)
+
+    class C:
+        def __init__(self, nx, ny, km):
+            self.nx, self.ny, self.km = nx, ny, km
+
+    class R:
+        def __init__(self, rid, shape, steps, done=0, arrival=0.0):
+            self.rid, self.case, self.n_steps, self.done_steps, self.arrival = rid, C(*shape), steps, done, arrival
+
+    small, mid, big = (96, 64, 30), (128, 96, 30), (360, 218, 55)
+    a = R(0, small, 200, done=190)          # 10 steps x 184k cells left
+    b = R(1, mid, 24)                       # 24 x 369k
+    c = R(2, small, 240)                    # 240 x 184k
+    d = R(3, big, 30)                       # 30 x 4.3M: larger than the fill alone
+    assert [r.rid for r in loop._srpt_pick([d, c, b, a])] == [0, 1, 2]
+    assert [r.rid for r in loop._srpt_pick([d])] == [3]
+    with pytest.raises(ValueError):
+        le.RequestLoop(policy="fifo")
