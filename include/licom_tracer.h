@@ -357,6 +357,13 @@ size_t licom_scratch_bytes(const licom_ctx *ctx);
  * convective levels in icmon only under LOWRES, convadj.cpp:247-254). */
 int licom_last_mix_count(licom_ctx *ctx, int *count);
 
+/* Number of tracer columns (one tracer of one column) whose new state held a
+ * NaN or an infinity in the last step (waits for the stream).  A batch
+ * member's count is in its own context, so a member that goes non-finite is
+ * flagged without touching the others (each member has its own arrays).
+ * The reference aborts the run instead (energy_c.c:109-112). */
+int licom_last_nonfinite(licom_ctx *ctx, int *count);
+
 /* Device-to-device streaming copy of `bytes` on `hip_stream`, used to
  * calibrate achievable HBM bandwidth and profiler byte counters.  No
  * reference counterpart. */

@@ -412,7 +412,7 @@ int licom_tracer_create(const licom_grid *G, const licom_params *P, licom_ctx **
     rc = rc ? rc : plan_scratch(&s.uw, n3);
     rc = rc ? rc : plan_scratch(&s.vs, n3);
     rc = rc ? rc : plan_scratch(&s.ori, 2 * n3);
-    rc = rc ? rc : plan_scratch(&s.mix_count, 1);
+    rc = rc ? rc : plan_scratch(&s.mix_count, 2);
     rc = rc ? rc : place_parts();
     if (rc) {
         licom_tracer_destroy(c);
@@ -574,6 +574,13 @@ int licom_last_mix_count(licom_ctx *c, int *count) {
     if (int rc = hip_status(hipStreamSynchronize(c->stream), "sync")) return rc;
     return hip_status(hipMemcpy(count, c->s.mix_count, sizeof(int), hipMemcpyDeviceToHost),
                       "mix count copy");
+}
+
+int licom_last_nonfinite(licom_ctx *c, int *count) {
+    if (!c || !count) return LICOM_EINVAL;
+    if (int rc = hip_status(hipStreamSynchronize(c->stream), "sync")) return rc;
+    return hip_status(hipMemcpy(count, c->s.mix_count + 1, sizeof(int), hipMemcpyDeviceToHost),
+                      "non-finite count copy");
 }
 
 int licom_stream_copy(void *dst, const void *src, size_t bytes, void *stream) {

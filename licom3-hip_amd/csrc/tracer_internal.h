@@ -40,7 +40,8 @@ struct Scratch {
     uint8_t *flags = nullptr;    // [2][km][jmt][imt] tspas limiter flags
     double *d2tk = nullptr;      // [2][km][jmt][imt] del4 first pass
     double *ori = nullptr;       // [2][km][jmt][imt] explicit update
-    int *mix_count = nullptr;    // [1] columns convadj mixed in the last step
+    int *mix_count = nullptr;    // [2] columns convadj mixed in the last step,
+                                 // tracer columns whose new state is not finite
     int nblk = 0;
 };
 

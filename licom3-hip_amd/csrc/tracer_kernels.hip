@@ -444,7 +444,7 @@ __device__ __forceinline__ void convect(const KGrid &g, int kcon, double *T, dou
 __global__ void k_column(KGrid g, KPar p, int flags, double *at, double *atb, double *akt,
                          const double *__restrict__ ori, const double *__restrict__ stf_n,
                          double *utf, double *vtf, double *h0f, double *tend, double *dt_diff,
-                         double *dt_conv) {
+                         double *dt_conv, int *counters) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= g.imt || j >= g.jmt) return;
@@ -484,6 +484,9 @@ __global__ void k_column(KGrid g, KPar p, int flags, double *at, double *atb, do
             }
             if (tend) tend[n * n3 + c] = (x[n][k] - T[c]) / p.dts * vk;
         }
+        bool nonfinite = false;   // counted as in the fused solve
+        for (int k = 0; k < km; ++k) nonfinite |= !__builtin_isfinite(x[n][k]);
+        if (nonfinite && counters) atomicAdd(counters + 1, 1);
     }
     if (flags & LICOM_STEP_CONVADJ) {
         // x[] holds at after tracer_9 (atb == at); convadj mixes a copy
@@ -698,7 +701,7 @@ int launch_step_simple(licom_ctx *ctx, const licom_fields *f, const licom_diag *
         Timer t(ctx, "k_column");
         k_column<<<g2, b2, 0, st>>>(g, p, ice ? flags & ~LICOM_STEP_CONVADJ : flags, f->at, f->atb, f->akt,
                                     s.ori, s.stf_n, f->utf, f->vtf, f->h0f, d ? d->tend : nullptr,
-                                    d ? d->dt_diff : nullptr, d ? d->dt_conv : nullptr);
+                                    d ? d->dt_diff : nullptr, d ? d->dt_conv : nullptr, s.mix_count);
     }
     if (int rc = launch_halo(ctx, f, flags & ~LICOM_STEP_ICESNOW)) return rc;
     if (!ice) return LICOM_OK;

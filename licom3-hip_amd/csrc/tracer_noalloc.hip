@@ -56,7 +56,7 @@ size_t carve(const licom_ctx *c, unsigned char *base, Scratch &s) {
     take(&s.uw, n3 * 8);
     take(&s.vs, n3 * 8);
     take(&s.ori, 2 * n3 * 8);
-    take(&s.mix_count, 4);
+    take(&s.mix_count, 8);
     s.flags = nullptr;
     s.d2tk = nullptr;
     s.nblk = c->s.nblk;

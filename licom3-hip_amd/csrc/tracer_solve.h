@@ -236,6 +236,7 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
     const double *C = sC, *to = sTO, *so = sSO;
     double xprev = 0.0, Tp = 0.0, Sp = 0.0;
     uint64_t mask = 0;
+    bool nonfinite = false;   // the column's new state holds a NaN or an infinity
     double qd[SEG], qa[SEG], qt[SEG], qw = 0.0;
     auto load_seg = [&](int m) {
 #pragma unroll
@@ -303,6 +304,7 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
                 double x = o;
                 if (wet) x = (k == 0) ? (se[0] * stf + sf[0]) * 1.0 : (se[r] * xprev + sf[r]) * 1.0;
                 xprev = x;
+                nonfinite |= !__builtin_isfinite(x);
                 const double vk = wet ? 1.0 : 0.0;
                 double v = (x - o) / dts * vk;
                 if (k == 0) v = v - stf * 0.5 * odzp[0];
@@ -367,6 +369,10 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
             }
         }
     }
+
+    // a member whose state went non-finite is flagged (its counter), not
+    // aborted: other members of a batch never read its arrays
+    if (nonfinite && act) atomicAdd(mix_count + 1, 1);
 
     // ---- convadj for statically unstable columns (both lanes of the pair)
     if constexpr (!ice) {

@@ -25,7 +25,10 @@ struct SurfaceIO {
 
 __device__ __forceinline__ void surface_cell(const KGrid &g, const KPar &p, const SurfaceIO &o, int i,
                                              int j) {
-    if (i == 0 && j == 0 && o.mix_count) *o.mix_count = 0;
+    if (i == 0 && j == 0 && o.mix_count) {
+        o.mix_count[0] = 0;   // columns convadj mixes in this step
+        o.mix_count[1] = 0;   // tracer columns whose new state is not finite
+    }
     if (i >= g.imt || j >= g.jmt) return;
     const size_t q = (size_t)j * g.imt + i, n2 = (size_t)g.imt * g.jmt;
     const double *h0f = o.h0f, *h0l = o.h0l;

@@ -138,6 +138,12 @@ class Ensemble:
         reads the advanced state."""
         lt.check(_lib().licom_batch_sync(self.handle), "licom_batch_sync")
 
+    def nonfinite(self):
+        """Per member: tracer columns whose state went non-finite in the last
+        step (a member is flagged; the others are not affected)."""
+        self.sync()
+        return [m.nonfinite() for m in self.members]
+
     def persistent_grid(self):
         """Workgroups of the persistent step (0 unless persistent)."""
         return int(_lib().licom_batch_persistent_grid(self.handle))
@@ -309,6 +315,7 @@ class RequestLoop:
         self.class_cells = tuple(class_cells)
         self.results = {}
         self.rejected = []
+        self.failed = []      # served requests whose state went non-finite
         self.free_margin = float(free_margin_bytes)
         # device free-memory probe: "device" (hipMemGetInfo), a callable
         # returning bytes, or None for no probe
@@ -332,6 +339,20 @@ class RequestLoop:
             if cells <= b:
                 return i
         return len(self.class_cells)
+
+    def _check_finite(self, r, stream):
+        # a request whose final state is not finite is reported as failed
+        # (its result is still returned); the others are not affected.  The
+        # check runs on the lane's stream, whose chunk has completed, so it
+        # never waits for other work (admissions, other lanes)
+        import torch
+        if r.model is None:
+            return
+        with torch.cuda.stream(stream):
+            ok = bool(torch.isfinite(r.model.state["at"]).all())
+        if not ok:
+            r.error = "the state became non-finite (NaN or infinity)"
+            self.failed.append(r)
 
     @staticmethod
     def _remaining(r):
@@ -366,6 +387,7 @@ class RequestLoop:
             for r in finished:
                 r.finished = t_now
                 done.append(r)
+                self._check_finite(r, ln.stream)
                 if self.keep_results:
                     self.results[r.rid] = r.model.host_state()["at"]
             if finished:
@@ -522,6 +544,7 @@ class RequestLoop:
                     for r in finished:
                         r.finished = t_now
                         done.append(r)
+                        self._check_finite(r, ln.stream)
                         if self.keep_results:
                             self.results[r.rid] = r.model.host_state()["at"]
                     if finished:
@@ -580,7 +603,8 @@ class RequestLoop:
         flush()
         makespan = clock() - t0
         lat = np.array([r.finished - r.arrival for r in done])
-        return {"requests": len(done), "rejected": len(self.rejected), "makespan_s": makespan,
+        return {"requests": len(done), "rejected": len(self.rejected), "nonfinite": len(self.failed),
+                "makespan_s": makespan,
                 "cell_updates": updates,
                 "cell_updates_per_s": updates / makespan if makespan > 0 else 0.0,
                 "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,

@@ -88,7 +88,7 @@ EXPORTED = ["licom_tracer_create", "licom_tracer_set_stream", "licom_tracer_step
             "licom_batch_replay", "licom_batch_sync", "licom_batch_destroy",
             "licom_autotune_member", "licom_convadj", "licom_icesnow", "licom_tracer_set_icesnow",
             "licom_tracer_sync", "licom_grid_export", "licom_set_profiling",
-            "licom_kernel_times", "licom_scratch_bytes", "licom_last_mix_count",
+            "licom_kernel_times", "licom_scratch_bytes", "licom_last_mix_count", "licom_last_nonfinite",
             "licom_stream_copy", "licom_tracer_destroy", "licom_strerror", "licom_last_error",
             "licom_restart_read", "licom_restart_write", "licom_eos_read"]
 
@@ -139,6 +139,7 @@ def lib():
         L.licom_scratch_bytes.argtypes = [_VP]
         L.licom_scratch_bytes.restype = ctypes.c_size_t
         L.licom_last_mix_count.argtypes = [_VP, _IP]
+        L.licom_last_nonfinite.argtypes = [_VP, _IP]
         L.licom_stream_copy.argtypes = [_VP, _VP, ctypes.c_size_t, _VP]
         L.licom_tracer_destroy.argtypes = [_VP]
         L.licom_strerror.argtypes = [ctypes.c_int]
@@ -417,6 +418,13 @@ class TracerModel:
     def mix_count(self):
         n = ctypes.c_int32(0)
         check(lib().licom_last_mix_count(self.ctx, ctypes.byref(n)), "licom_last_mix_count")
+        return int(n.value)
+
+    def nonfinite(self):
+        """Tracer columns whose new state held a NaN or an infinity in the last
+        step (0 for a healthy model)."""
+        n = ctypes.c_int32(0)
+        check(lib().licom_last_nonfinite(self.ctx, ctypes.byref(n)), "licom_last_nonfinite")
         return int(n.value)
 
     def scratch_bytes(self):
