@@ -409,9 +409,25 @@ def mixed_error_study(lt, ls, steps=100):
     pcase.state["atb"][0, 1:] += noise
     flags = lt.LICOM_STEP_SIDE_EFFECTS | lt.LICOM_STEP_CONVADJ
     models = [lt.TracerModel(case), lt.TracerModel(case, precision="mixed"), lt.TracerModel(pcase)]
+    # tspas limiter decisions of every step: cells whose flag differs from the
+    # FP64 run's at the same step (the two runs start from the same state;
+    # from the second step on they also see slightly different states)
+    import torch
+    fl = [m.limiter_flags() for m in models]
+    wet_t = torch.from_numpy(np.broadcast_to(wet3, (2,) + wet3.shape).copy()).to(fl[0].device)
+    wet_t[:, :, :2] = False
+    wet_t[:, :, -2:] = False
+    wet_t[:, :, :, :2] = False
+    wet_t[:, :, :, -2:] = False
+    flips_mix, flips_ctl, flagged = [], [], []
     for st in range(steps):
         for m in models:
             m.tracer(st, flags)
+        torch.cuda.synchronize()
+        flips_mix.append(int(((fl[1] != fl[0]) & wet_t).sum()))
+        flips_ctl.append(int(((fl[2] != fl[0]) & wet_t).sum()))
+        flagged.append(int(((fl[0] != 0) & wet_t).sum()))
+    nwet = int(wet_t.sum())
     a_ref, a_mix, a_ctl = [m.host_state()["at"] for m in models]
     for m in models:
         m.close()
@@ -431,9 +447,19 @@ def mixed_error_study(lt, ls, steps=100):
         return float(np.sum(w * a[0][:, 2:-2, 2:-2]))
 
     h = heat(a_ref)
+
+    def flips(f):
+        return {"first_step": f[0], "mean_per_step": float(np.mean(f)), "max_per_step": int(max(f)),
+                "total": int(sum(f))}
+
+    em = err(a_mix)
     return {"grid": f"{case.nx}x{case.ny}x{case.km}", "steps": steps,
-            "mixed_vs_fp64": dict(err(a_mix), heat_drift_rel=abs(heat(a_mix) - h) / abs(h)),
-            "fp64_perturbed_1e-8K_vs_fp64": dict(err(a_ctl), heat_drift_rel=abs(heat(a_ctl) - h) / abs(h))}
+            "mixed_vs_fp64": dict(em, heat_drift_rel=abs(heat(a_mix) - h) / abs(h),
+                                  limiter_flag_mismatches=flips(flips_mix)),
+            "fp64_perturbed_1e-8K_vs_fp64": dict(err(a_ctl), heat_drift_rel=abs(heat(a_ctl) - h) / abs(h),
+                                                 limiter_flag_mismatches=flips(flips_ctl)),
+            "wet_tracer_cells": nwet, "fp64_flagged_cells_per_step": float(np.mean(flagged)),
+            "target_max_dT_1e-4K_met": bool(em["max_dT_K"] <= 1e-4)}
 
 
 def run_serving(args, torch, lt, le, offline_value, iso, load):

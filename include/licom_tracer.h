@@ -357,6 +357,16 @@ size_t licom_scratch_bytes(const licom_ctx *ctx);
  * convective levels in icmon only under LOWRES, convadj.cpp:247-254). */
 int licom_last_mix_count(licom_ctx *ctx, int *count);
 
+/* Diagnostic output of the tspas limiter's decisions: every step of the
+ * fused FP64 path and of the mixed form writes, for each physical cell of
+ * each tracer, 1 where the cell's predictor left its local bounds (its faces
+ * take the upwind flux, advection_tracer_3, tracer.cpp:549-921) and 0
+ * elsewhere, into flags ([2][km][jmt][imt] bytes, device); NULL stops it.
+ * Used to count the decisions the mixed form takes differently from FP64.
+ * No reference counterpart (the reference keeps them in its at00/atmax/atmin
+ * scratch). */
+int licom_tracer_set_flag_output(licom_ctx *ctx, void *flags);
+
 /* Number of tracer columns (one tracer of one column) whose new state held a
  * NaN or an infinity in the last step (waits for the stream).  A batch
  * member's count is in its own context, so a member that goes non-finite is

@@ -576,6 +576,12 @@ int licom_last_mix_count(licom_ctx *c, int *count) {
                       "mix count copy");
 }
 
+int licom_tracer_set_flag_output(licom_ctx *c, void *flags) {
+    if (!c) return LICOM_EINVAL;
+    c->flags_out = static_cast<uint8_t *>(flags);
+    return LICOM_OK;
+}
+
 int licom_last_nonfinite(licom_ctx *c, int *count) {
     if (!c || !count) return LICOM_EINVAL;
     if (int rc = hip_status(hipStreamSynchronize(c->stream), "sync")) return rc;

@@ -213,6 +213,7 @@ constexpr int DMA_UVBA = 3 * ((NR / 2 + 63) / 64) + 2;     // u, v, atb + akt
 struct MarchArgs {
     const double *at, *atb, *akt, *uw, *vs, *ws, *stf_n, *err_norm2, *swv, *restore;
     double *ori, *ax, *ay, *az, *dz, *pen;
+    uint8_t *flags_out;   // the limiter flag of every physical cell, or NULL
     int ntx, ntiles, tiles_per_xcd;
 };
 
@@ -692,6 +693,7 @@ __global__ __launch_bounds__(NT, (LA >= 2 ? 2 : 3)) void k_march(KGrid g, KPar p
                 dzv = odzp[k] * (wt1 - wt2) * 0.5;
             }
             if (a.dz) a.dz[n * n3 + c] = dzv;
+            if (a.flags_out) a.flags_out[n * n3 + c] = fc;
             // surface fluxes (tracer_5 n=1, tracer_6) and restoring (tracer_7)
             if (k == 0) {
                 if (n == 1) {
@@ -832,6 +834,7 @@ int launch_step_fused(licom_ctx *ctx, const licom_fields *f, const licom_diag *d
         a.az = d ? d->az : nullptr;
         a.dz = d ? d->dz : nullptr;
         a.pen = d ? d->penetrate : nullptr;
+        a.flags_out = ctx->flags_out;
         a.ntx = (g.imt - 4 + BX - 1) / BX;
         const int nty = (g.jmt - 4 + BY - 1) / BY;
         a.ntiles = a.ntx * nty;

@@ -66,6 +66,9 @@ struct licom_ctx {
     int solve_shape = 0;
     // licomqice [jmt][imt] (device) written by steps with LICOM_STEP_ICESNOW
     double *qice = nullptr;
+    // tspas limiter flags [2][km][jmt][imt] (device, 1 byte) written by the
+    // fused FP64 and mixed k-marches when set (licom_tracer_set_flag_output)
+    uint8_t *flags_out = nullptr;
     licom::KGrid g{};
     licom::KPar kp{};
     double area_t = 0.0;

@@ -214,6 +214,7 @@ struct MarchArgsMx {
     const float *akt, *uw, *vs, *ws;
     double *ori;
     float *ax, *ay, *az, *dz, *pen;
+    uint8_t *flags_out = nullptr;   // the limiter flag of every physical cell, or NULL
     int ntx, ntiles, tiles_per_xcd;
 };
 
@@ -674,6 +675,7 @@ __device__ __forceinline__ void march_body(const KGrid &g, const KPar &p, const 
             }
             tf += dzv;
             if (a.dz) a.dz[n * n3 + c] = dzv;
+            if (a.flags_out) a.flags_out[n * n3 + c] = fc;
             if (k == 0 && kmt_own > 0) {
                 tf += (float)(stf_n * 0.5 * odzp0);
                 if (n == 1 && p.sssnorm) tf += (float)enorm * vk;
@@ -817,6 +819,7 @@ int launch_step_mixed(licom_ctx *ctx, const licom_fields_mixed *f, const licom_d
         a.az = d ? d->az : nullptr;
         a.dz = d ? d->dz : nullptr;
         a.pen = d ? d->penetrate : nullptr;
+        a.flags_out = ctx->flags_out;
         a.ntx = (g.imt - 4 + BX - 1) / BX;
         const int nty = (g.jmt - 4 + BY - 1) / BY;
         a.ntiles = a.ntx * nty;

@@ -89,6 +89,7 @@ EXPORTED = ["licom_tracer_create", "licom_tracer_set_stream", "licom_tracer_step
             "licom_autotune_member", "licom_convadj", "licom_icesnow", "licom_tracer_set_icesnow",
             "licom_tracer_sync", "licom_grid_export", "licom_set_profiling",
             "licom_kernel_times", "licom_scratch_bytes", "licom_last_mix_count", "licom_last_nonfinite",
+            "licom_tracer_set_flag_output",
             "licom_stream_copy", "licom_tracer_destroy", "licom_strerror", "licom_last_error",
             "licom_restart_read", "licom_restart_write", "licom_eos_read"]
 
@@ -140,6 +141,7 @@ def lib():
         L.licom_scratch_bytes.restype = ctypes.c_size_t
         L.licom_last_mix_count.argtypes = [_VP, _IP]
         L.licom_last_nonfinite.argtypes = [_VP, _IP]
+        L.licom_tracer_set_flag_output.argtypes = [_VP, _VP]
         L.licom_stream_copy.argtypes = [_VP, _VP, ctypes.c_size_t, _VP]
         L.licom_tracer_destroy.argtypes = [_VP]
         L.licom_strerror.argtypes = [ctypes.c_int]
@@ -419,6 +421,20 @@ class TracerModel:
         n = ctypes.c_int32(0)
         check(lib().licom_last_mix_count(self.ctx, ctypes.byref(n)), "licom_last_mix_count")
         return int(n.value)
+
+    def limiter_flags(self, on=True):
+        """From the next step on, every step writes its tspas limiter flags
+        (uint8 [2][km][jmt][imt], device) into self.flags; on=False stops it."""
+        if on:
+            c = self.case
+            self.flags = self.torch.zeros((2, c.km, c.jmt, c.imt), dtype=self.torch.uint8,
+                                          device=self.device)
+            check(lib().licom_tracer_set_flag_output(self.ctx, _VP(self.flags.data_ptr())),
+                  "licom_tracer_set_flag_output")
+        else:
+            check(lib().licom_tracer_set_flag_output(self.ctx, None), "licom_tracer_set_flag_output")
+            self.flags = None
+        return self.flags
 
     def nonfinite(self):
         """Tracer columns whose new state held a NaN or an infinity in the last
