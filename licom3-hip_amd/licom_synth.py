@@ -186,3 +186,116 @@ def make_case(name: str = "C1", **overrides) -> Case:
     del n2, n3
     return Case(name=name, nx=nx, ny=ny, km=km, imt=imt, jmt=jmt, imt_global=g.imt_global,
                 grid=grid, state=state, forcing=forcing, params=params)
+
+
+# ---------------------------------------------------------------- Canuto mixing
+TURB_MT, TURB_MT0, TURB_NTH = 1462, 400, 75   # canuto_mod.h: mt, mt0, n_theta_r_oct
+
+
+@dataclasses.dataclass
+class TurbCase:
+    """Inputs of the Canuto producer (turbinit tables + one turb_cu call)."""
+    imt: int
+    jmt: int
+    km: int
+    grid: dict      # kmt [jmt][imt], zkp km+1, dzp km, fcort [jmt][imt]
+    tables: dict    # canuto_mod arrays and turb_param scalars (licom_turb.TABLE_*)
+    fields: dict    # turb_cu inputs ([k][j][i] / [j][i])
+    params: dict    # ncc, dfricmx, dwndmix
+
+
+def turb_tables(b1=16.6, theta_rcrp=1.1, theta_rcrn=1.7):
+    """Synthetic Canuto tables on the reference's grids.
+
+    The independent variables follow turb_ini: the Ri (= Ri_d) and
+    N_d^2/N^2 tables are linear with step dri = -ri0/mt0 = 0.01 up to index
+    mt0 and grow geometrically by rri = rib(mt0)/rib(mt0-1) beyond it
+    (canuto_ini_2002.F90:415-471, 509-533); the background tables sit on
+    theta = itheta * (pi/4)/n_theta_r_oct, itheta = -n..3n (:700).  The
+    table values are smooth positive functions, not the Canuto 2002
+    closure: the realizable region ends at a critical Ri_c(Ri_d) = 0.6 +
+    0.35 tanh(Ri_d) (irimax), beyond which slq2b is negative, as turb_ini
+    leaves it."""
+    mt, mt0, nth = TURB_MT, TURB_MT0, TURB_NTH
+    dri = 4.0 / mt0
+    idx = np.arange(-mt, mt + 1)
+    rib = idx * dri
+    rri = rib[mt + mt0] / rib[mt + mt0 - 1]
+    far = np.abs(idx) > mt0
+    rib[far] = np.sign(idx[far]) * rib[mt + mt0] * rri ** (np.abs(idx[far]) - mt0).astype(np.float64)
+    ri = rib[None, :]
+    rid = rib[:, None]
+    ri_c = 0.6 + 0.35 * np.tanh(rib)
+    irimax = (np.searchsorted(rib, ri_c, side="right") - 1 - mt).astype(np.int32)
+    lim = rib[mt + irimax][:, None]
+    slq2b = np.where(ri <= lim, 3.0 * (ri_c[:, None] - ri + 0.02) / (1.0 + 0.1 * np.abs(ri)), -1.0)
+    smb = 0.08 + 0.3 / (1.0 + np.exp(np.minimum(2.0 * ri, 700.0))) + 0.0 * rid
+    shb = smb * (1.3 + 0.2 * np.tanh(rid))
+    ssb = shb * (0.9 + 0.1 * np.tanh(0.5 * rid))
+    x = rib.copy()
+    th = np.arange(-nth, 3 * nth + 1) * ((np.pi / 4.0) / nth)
+    return dict(
+        rib=rib, irimax=irimax,
+        slq2b=np.ascontiguousarray(slq2b), smb=np.ascontiguousarray(smb),
+        shb=np.ascontiguousarray(shb), ssb=np.ascontiguousarray(ssb),
+        and2on2a1=x, amtaun2a1=1.5 + 0.5 * np.tanh(x), sma1=0.1 + 0.05 * np.tanh(x),
+        sha1=0.15 + 0.05 * np.tanh(0.5 * x), ssa1=0.12 + 0.04 * np.tanh(x),
+        back_ra_r=0.12 + 0.05 * np.cos(2.0 * th), sm_r1=0.05 + 0.02 * np.sin(th),
+        sh_r1=0.07 + 0.03 * np.cos(th), ss_r1=0.06 + 0.02 * np.cos(2.0 * th),
+        slq2_r1=2.0 + 0.5 * np.sin(2.0 * th),
+        rri=rri, rnd2on2=rri, dri=dri, deltheta_r=(np.pi / 4.0) / nth, b1=b1,
+        theta_rcrp=theta_rcrp, theta_rcrn=theta_rcrn, dand2on2=dri)
+
+
+def make_turb_case(case: Case, seed: int = 7, tables: dict | None = None) -> TurbCase:
+    """turb_cu inputs over a tracer case's grid (kmt, zkp, latitudes).
+
+    Profiles: a mixed layer of 10-120 m over a thermocline, density rising
+    with depth, N^2 mostly stable with unstable near-surface levels in a
+    third of the columns, S^2 log-uniform, and a few columns of strong
+    convection with almost no shear (the zero-shear table), an unstable
+    level below stable ones in a tenth of the columns; the equator row has
+    f = 0 exactly."""
+    rng = np.random.default_rng(seed)
+    km, jmt, imt = case.km, case.jmt, case.imt
+    kmt = case.grid["kmt"].astype(np.int32)
+    zkp = case.grid["zkp"].astype(np.float64)
+    dzp = zkp[:-1] - zkp[1:]
+    depth = -0.5 * (zkp[:-1] + zkp[1:])                       # T-level depths (m)
+    lev = np.arange(km)[:, None, None]
+    vit = (lev < kmt[None]).astype(np.float64)
+    lat = np.deg2rad(case.grid["lat_t"])
+    fcort = np.repeat((2.0 * 7.292e-5 * np.sin(lat))[:, None], imt, axis=1)
+    fcort[jmt // 2, :] = 0.0
+    s3 = (km, jmt, imt)
+    d3 = depth[:, None, None]
+    mld = rng.uniform(10.0, 120.0, (jmt, imt))
+    ts = 28.0 - 20.0 * np.abs(np.sin(lat))[:, None] + rng.normal(0.0, 0.5, (jmt, imt))
+    at = ts[None] - 8.0 * (1.0 - np.exp(-np.maximum(d3 - mld[None], 0.0) / 400.0))
+    at = at + rng.normal(0.0, 0.01, s3)
+    pden = 1022.0 + 5.0 * (1.0 - np.exp(-d3 / 800.0)) + rng.normal(0.0, 0.002, s3)
+    s2t = 10.0 ** rng.uniform(-8.0, -4.0, s3)
+    rict = 10.0 ** rng.uniform(-7.0, -4.0, s3)
+    conv = (rng.random((jmt, imt)) < 0.33)[None] & (lev < 3)
+    rict = np.where(conv, -(10.0 ** rng.uniform(-7.0, -5.0, s3)), rict)
+    # an unstable level below stable ones in a tenth of the columns
+    inv = (rng.random((jmt, imt)) < 0.1)[None] & (lev == rng.integers(3, max(km - 1, 4), (jmt, imt))[None])
+    rict = np.where(inv, -(10.0 ** rng.uniform(-7.0, -5.0, s3)), rict)
+    calm = (rng.random((jmt, imt)) < 0.03)[None] & (lev < 4)
+    s2t = np.where(calm, 1.0e-10, s2t)
+    rict = np.where(calm, -1.0e-6, rict)
+    rit = rict / (s2t + 1.0e-25)
+    ricdttms = rit * rng.uniform(-1.5, 1.5, s3)
+    bottom = np.take(np.concatenate([[0.0], -zkp[1:]]), kmt)  # column depth (m)
+    fz = np.exp(-np.maximum(bottom[None] - d3, 0.0) / 500.0) * vit
+    fields = dict(at=at, pdensity=pden, rit=rit, ricdttms=ricdttms, s2t=s2t, rict=rict,
+                  fz_tide=fz, vit=vit,
+                  buoytur=rng.normal(0.0, 1.0e-8, (jmt, imt)),
+                  buoysol=-np.abs(rng.normal(0.0, 5.0e-9, (jmt, imt))),
+                  wave_dis=10.0 ** rng.uniform(-4.0, -2.0, (jmt, imt)))
+    fields = {k: np.ascontiguousarray(v, np.float64) for k, v in fields.items()}
+    params = dict(ncc=int(round(1.0 / case.params["oncc"])), dfricmx=100.0e-4, dwndmix=10.0e-4)
+    return TurbCase(imt=imt, jmt=jmt, km=km,
+                    grid=dict(kmt=kmt, zkp=zkp, dzp=dzp, fcort=fcort),
+                    tables=tables if tables is not None else turb_tables(), fields=fields,
+                    params=params)

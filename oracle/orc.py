@@ -233,3 +233,116 @@ def content(case, oracle, st):
     for n in range(2):
         out.append(math.fsum((w * st["at"][n][:, j0:j1, i0:i1]).ravel().tolist()))
     return out
+
+
+# ---------------------------------------------------------------- Canuto mixing (turb.h)
+TURB_MT, TURB_MT0, TURB_NTH = 1462, 400, 75
+TURB_COUNTERS = ("pure", "table2d", "below", "deep", "epsy", "deardorff", "rafglt", "theta_p",
+                 "theta_n", "exit")
+_TURB_ARRAYS = ("rib", "irimax", "slq2b", "smb", "shb", "ssb", "and2on2a1", "amtaun2a1", "sma1",
+                "sha1", "ssa1", "back_ra_r", "sm_r1", "sh_r1", "ss_r1", "slq2_r1")
+_TURB_SCALARS = ("rri", "rnd2on2", "dri", "deltheta_r", "b1", "theta_rcrp", "theta_rcrn",
+                 "dand2on2")
+_TURB_IN = ("at", "pdensity", "rit", "ricdttms", "s2t", "rict", "fz_tide", "vit",
+            "buoytur", "buoysol", "fcort", "wave_dis")
+_TURB_OUT = ("akmt", "akt", "ak_tide", "amld", "richardson", "fztidal", "wp3_tidal")
+
+
+class OrcTurbTables(ctypes.Structure):
+    _fields_ = ([("rib", _DP), ("irimax", _IP)] + [(k, _DP) for k in _TURB_ARRAYS[2:]] +
+                [(k, ctypes.c_double) for k in _TURB_SCALARS])
+
+
+class OrcTurbFields(ctypes.Structure):
+    _fields_ = [(k, _DP) for k in _TURB_IN + _TURB_OUT]
+
+
+def _turb_lib():
+    L = lib()
+    if not getattr(L, "_turb_bound", False):
+        T = ctypes.POINTER(OrcTurbTables)
+        L.orc_turb_formld.argtypes = [ctypes.c_int, _DP, _DP]
+        L.orc_turb_formld.restype = ctypes.c_double
+        L.orc_turb_eplatidepend.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_turb_eplatidepend.restype = ctypes.c_double
+        L.orc_turb_interp1d.argtypes = [T, ctypes.c_double, _DP]
+        L.orc_turb_interp2d.argtypes = [T, _DP, _DP, _DP]
+        L.orc_turb_column.argtypes = [T, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP, _DP, _DP, _DP,
+                                      _DP, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_double, ctypes.c_double, _DP, _DP, _DP, _DP,
+                                      ctypes.POINTER(ctypes.c_long)]
+        L.orc_turb_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _IP, _DP, _DP, T,
+                                   ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                   ctypes.POINTER(OrcTurbFields), ctypes.POINTER(ctypes.c_long)]
+        L._turb_bound = True
+    return L
+
+
+class TurbOracle:
+    """The oracle's view of one set of Canuto tables."""
+
+    def __init__(self, tables):
+        self._keep = []
+        T = OrcTurbTables()
+        for k in _TURB_ARRAYS:
+            a = np.ascontiguousarray(tables[k], np.int32 if k == "irimax" else np.float64)
+            self._keep.append(a)
+            setattr(T, k, _i(a) if k == "irimax" else _d(a))
+        for k in _TURB_SCALARS:
+            setattr(T, k, float(tables[k]))
+        self.T = T
+        self.L = _turb_lib()
+
+    def interp1d(self, x):
+        o = np.zeros(4)
+        st = self.L.orc_turb_interp1d(ctypes.byref(self.T), float(x), _d(o))
+        return st, o
+
+    def interp2d(self, ri, rid):
+        a, b, o = ctypes.c_double(ri), ctypes.c_double(rid), np.zeros(4)
+        st = self.L.orc_turb_interp2d(ctypes.byref(self.T), ctypes.byref(a), ctypes.byref(b), _d(o))
+        return st, o, a.value, b.value
+
+    def column(self, n, nmax, z, t, rh, ri, rid, s2, an2, fricmx, wndmix, buoytur, buoysol, coriol):
+        arr = [np.ascontiguousarray(v, np.float64).copy() for v in (z, t, rh, ri, rid, s2, an2)]
+        akm, akh, aks = np.zeros(nmax), np.zeros(nmax), np.zeros(nmax)
+        amld = ctypes.c_double()
+        cnt = (ctypes.c_long * len(TURB_COUNTERS))()
+        ex = self.L.orc_turb_column(ctypes.byref(self.T), n, nmax, *[_d(a) for a in arr], fricmx,
+                                    wndmix, buoytur, buoysol, coriol, ctypes.byref(amld), _d(akm),
+                                    _d(akh), _d(aks), cnt)
+        return dict(exit=ex, amld=amld.value, akm=akm, akh=akh, aks=aks, s2=arr[5],
+                    counts=dict(zip(TURB_COUNTERS, list(cnt))))
+
+    def run(self, tc, outputs=None):
+        """turb_cu over a TurbCase; returns (outputs, branch counts).  Pass
+        ``outputs`` to start from given output arrays (akt, richardson,
+        fztidal and wp3_tidal keep what the call does not write)."""
+        km, jmt, imt = tc.km, tc.jmt, tc.imt
+        if outputs is None:
+            outputs = dict(akmt=np.zeros((km, jmt, imt)), akt=np.zeros((2, km, jmt, imt)),
+                           ak_tide=np.zeros((km, jmt, imt)), amld=np.zeros((jmt, imt)),
+                           richardson=np.zeros((km, jmt, imt)), fztidal=np.zeros((km, jmt, imt)),
+                           wp3_tidal=np.zeros((km, jmt, imt)))
+        src = dict(tc.fields)
+        src["fcort"] = tc.grid["fcort"]
+        keep = {k: np.ascontiguousarray(src[k], np.float64) for k in _TURB_IN}
+        F = OrcTurbFields(*([_d(keep[k]) for k in _TURB_IN] + [_d(outputs[k]) for k in _TURB_OUT]))
+        kmt = np.ascontiguousarray(tc.grid["kmt"], np.int32)
+        zkp = np.ascontiguousarray(tc.grid["zkp"], np.float64)
+        dzp = np.ascontiguousarray(tc.grid["dzp"], np.float64)
+        cnt = (ctypes.c_long * len(TURB_COUNTERS))()
+        self.L.orc_turb_run(imt, jmt, km, _i(kmt), _d(zkp), _d(dzp), ctypes.byref(self.T),
+                            int(tc.params["ncc"]), float(tc.params["dfricmx"]),
+                            float(tc.params["dwndmix"]), ctypes.byref(F), cnt)
+        return outputs, dict(zip(TURB_COUNTERS, list(cnt)))
+
+
+def turb_formld(z, t):
+    z = np.ascontiguousarray(z, np.float64)
+    t = np.ascontiguousarray(t, np.float64)
+    return _turb_lib().orc_turb_formld(len(z), _d(z), _d(t))
+
+
+def turb_eplatidepend(f, an):
+    return _turb_lib().orc_turb_eplatidepend(float(f), float(an))

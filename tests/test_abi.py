@@ -53,3 +53,36 @@ def test_product_does_not_touch_the_oracle():
             if f.endswith((".py", ".hip", ".cpp", ".h", "Makefile")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.lower(), f
+
+
+def test_turb_declarations_are_exported():
+    import licom_turb
+    names = declared("licom_turb.h")
+    assert names
+    L = ctypes.CDLL(licom_turb.lt.lib_path())
+    assert [n for n in names if not hasattr(L, n)] == []
+    assert sorted(licom_turb.EXPORTED) == names
+
+
+def test_turb_create_validates_before_touching_a_device():
+    import numpy as np
+    import licom_synth as ls
+    import licom_turb as ltu
+    tb = ls.turb_tables()
+    keep = []
+    T = ltu.make_tables(tb, keep)
+    kmt = np.zeros((6, 6), np.int32)
+    z = np.zeros(11)
+    h = ctypes.c_void_p()
+    L = ltu.lib()
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    rc = L.licom_turb_create(6, 6, 1, kmt.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), dp(z),
+                             dp(z), dp(z), ctypes.byref(T), 2, 1e-2, 1e-3, ctypes.byref(h))
+    assert rc == -1 and b"invalid extents" in L.licom_last_error()
+    tb["irimax"] = tb["irimax"].copy()
+    tb["irimax"][5] = 5000
+    T = ltu.make_tables(tb, keep)
+    rc = L.licom_turb_create(6, 6, 10, kmt.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), dp(z),
+                             dp(z), dp(np.zeros(36)), ctypes.byref(T), 2, 1e-2, 1e-3, ctypes.byref(h))
+    assert rc == -1 and b"irimax" in L.licom_last_error()
+    assert not h.value
