@@ -224,7 +224,23 @@ def cpu_baseline(args):
     band1 = max(4, args.cpu_band // 8)
     v_one, n_one, case1 = timed(band1, args.cpu_seconds / 2, 3, 10)
     orc.lib().orc_set_nthreads(threads)
-    return {"value": v_all, "unit": "tracer cell-updates/s", "cores": threads,
+    # the Canuto producer's restatement on a band of the same workload
+    tcase = ls.make_case("C3", ny=args.cpu_band)
+    tc = ls.make_turb_case(tcase, seed=17)
+    to = orc.TurbOracle(tc.tables)
+    to.run(tc)
+    tt = []
+    t_end = time.time() + args.cpu_seconds / 2
+    while len(tt) < 3 or (time.time() < t_end and len(tt) < 10):
+        t0 = time.perf_counter()
+        to.run(tc)
+        tt.append(time.perf_counter() - t0)
+    tt.sort()
+    turb_cpu = {"value": tcase.nx * tcase.ny * tcase.km / tt[len(tt) // 2],
+                "unit": "column cells/s", "cores": threads, "kind": "port",
+                "sample": f"oracle turb_cu restatement on a {tcase.nx}x{tcase.ny}x{tcase.km} band, "
+                          f"median of {len(tt)} runs"}
+    return {"value": v_all, "unit": "tracer cell-updates/s", "cores": threads, "turb": turb_cpu,
             "kind": "port",
             "sample": f"oracle (C, OpenMP, {threads} threads) on a {case.nx}x{case.ny}x{case.km} band "
                       f"of the 0.1 deg workload, median of {n_all} steps after 1 warm-up",
@@ -273,6 +289,77 @@ def run_icesnow(model, args, torch, lt, case, plain):
         out["plain_k_solve_ms"] = plain["kernel_ms"]["k_solve"]
     print(f"[bench] icesnow: {ms:.3f} ms alone, step {1000 * sec:.2f} ms fused", file=sys.stderr, flush=True)
     return out
+
+
+def turb_algorithmic_bytes(kmt, km):
+    """Bytes the Canuto producer must move once (each array read or written
+    once): a wet interior column with n = kmt-1 closure levels reads at,
+    pdensity and vit at n+1 levels and rit, ricdttms, s2t, rict, fz_tide at
+    n, plus kmt, fcort, buoytur, buoysol, wave_dis; it writes akmt and
+    ak_tide at km levels, akt at 2(km-1), richardson, fztidal, wp3_tidal at
+    n and amld.  Any other column reads vit and kmt and writes akmt, ak_tide
+    and amld."""
+    import numpy as np
+    wet = np.zeros(kmt.shape, bool)
+    wet[1:-1, 1:-1] = kmt[1:-1, 1:-1] > 0
+    n = (kmt[wet].astype(np.int64) - 1)
+    per_wet = 8 * (3 * (n + 1) + 5 * n + 2 * km + 2 * (km - 1) + 3 * n + 1) + 4 + 8 * 4
+    dry = int((~wet).sum())
+    return int(per_wet.sum()) + dry * (8 * (2 * km + 1) + 8 + 4)
+
+
+def run_turb(args, torch, lt, case, model):
+    """The Canuto producer (turb_cu) at the workload size: alone, and ahead
+    of the FP64 tracer step writing the model's AKT (readyc's turb_cu, then
+    tracer_, as stepon orders them).  Fields are drawn on the device."""
+    import licom_synth as ls
+    import licom_turb as ltu
+    dev = model.device
+    t0 = time.time()
+    tc = ls.make_turb_case_device(case, torch, dev, seed=17)
+    p = ltu.TurbProducer(tc, device=dev, stream=model.stream)
+    print(f"[bench] turb: inputs drawn in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    alg = turb_algorithmic_bytes(case.grid["kmt"], case.km)
+    for _ in range(max(1, args.warmup)):
+        p.run()
+    p.sync()
+    reps = max(3, args.steps)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(reps)]
+    for a, b in ev:
+        a.record(model.stream)
+        p.run()
+        b.record(model.stream)
+    p.sync()
+    ms = sorted(a.elapsed_time(b) for a, b in ev)
+    avg = sum(ms) / len(ms)
+    flags = lt.LICOM_STEP_SIDE_EFFECTS | lt.LICOM_STEP_CONVADJ
+    p.run(akt=model.state["akt"])
+    model.tracer(0, flags)
+    model.sync()
+    steps = max(2, args.steps // 2)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        p.run(akt=model.state["akt"])
+        model.tracer(0, flags)
+    model.sync()
+    torch.cuda.synchronize()
+    fused = (time.perf_counter() - t1) / steps
+    p.close()
+    del p, tc
+    torch.cuda.empty_cache()
+    cells = case.nx * case.ny * case.km
+    ach = alg / (avg * 1e-3) / 1e9
+    print(f"[bench] turb: {avg:.3f} ms alone, turb + step {1000 * fused:.2f} ms", file=sys.stderr,
+          flush=True)
+    return {"kernel": "k_turb", "avg_launch_ms": round(avg, 4), "min_launch_ms": round(ms[0], 4),
+            "launches": reps, "column_cells_per_s": cells / (avg * 1e-3),
+            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                         "algorithmic_bytes": alg},
+            "turb_plus_fp64_step_ms": round(1000 * fused, 3),
+            "tables": "synthetic Canuto tables on the reference grids (2925 x 2925 Ri x Ri_d)"}
 
 
 def run_variant(model, args, torch, lt, case, updates):
@@ -512,6 +599,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-error-study", action="store_true")
     ap.add_argument("--no-icesnow", action="store_true")
+    ap.add_argument("--no-turb", action="store_true")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--autotune-cache", default=None,
                     help="JSON file to read and write the ensemble's tuned launch shapes "
@@ -587,6 +675,9 @@ def main():
     ice = None
     if "fp64" in models and not args.no_icesnow and not args.no_kernel_timing:
         ice = run_icesnow(models["fp64"], args, torch, lt, case, runs["fp64"])
+    turb = None
+    if "fp64" in models and not args.no_turb:
+        turb = run_turb(args, torch, lt, case, models["fp64"])
     for m in models.values():
         m.close()
     del models
@@ -647,6 +738,8 @@ def main():
                         "error": mixed_err, "error_100_steps_1deg": err_study}
     if ice:
         out["icesnow"] = ice
+    if turb:
+        out["turb"] = turb
     if ens_out:
         out["ensemble"] = ens_out
     if serve_out:

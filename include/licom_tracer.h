@@ -368,7 +368,8 @@ int licom_last_mix_count(licom_ctx *ctx, int *count);
 int licom_tracer_set_flag_output(licom_ctx *ctx, void *flags);
 
 /* Number of tracer columns (one tracer of one column) whose new state held a
- * NaN or an infinity in the last step (waits for the stream).  A batch
+ * NaN or an infinity at a wet level in the last step (waits for the
+ * stream).  A batch
  * member's count is in its own context, so a member that goes non-finite is
  * flagged without touching the others (each member has its own arrays).
  * The reference aborts the run instead (energy_c.c:109-112). */

@@ -236,7 +236,6 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
     const double *C = sC, *to = sTO, *so = sSO;
     double xprev = 0.0, Tp = 0.0, Sp = 0.0;
     uint64_t mask = 0;
-    bool nonfinite = false;   // the column's new state holds a NaN or an infinity
     double qd[SEG], qa[SEG], qt[SEG], qw = 0.0;
     auto load_seg = [&](int m) {
 #pragma unroll
@@ -304,7 +303,6 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
                 double x = o;
                 if (wet) x = (k == 0) ? (se[0] * stf + sf[0]) * 1.0 : (se[r] * xprev + sf[r]) * 1.0;
                 xprev = x;
-                nonfinite |= !__builtin_isfinite(x);
                 const double vk = wet ? 1.0 : 0.0;
                 double v = (x - o) / dts * vk;
                 if (k == 0) v = v - stf * 0.5 * odzp[0];
@@ -370,9 +368,14 @@ __device__ __forceinline__ void solve_body(const KGrid &g, const KPar &p, int fl
         }
     }
 
-    // a member whose state went non-finite is flagged (its counter), not
-    // aborted: other members of a batch never read its arrays
-    if (nonfinite && act) atomicAdd(mix_count + 1, 1);
+    // A member whose state went non-finite is flagged (its counter), not
+    // aborted: other members of a batch never read its arrays.  The
+    // substitution carries a NaN or an infinity at any wet level down to the
+    // bottom wet level (x(k) = e(k) x(k-1) + f(k); 0 * inf is NaN), so the
+    // lane reads back that one value instead of testing every level, which
+    // kept a lane mask live through the unrolled sweep and spilled registers.
+    if (act && kz > 0 && !__builtin_isfinite(Bb[(size_t)(kz - 1) * n2 + q]))
+        atomicAdd(mix_count + 1, 1);
 
     // ---- convadj for statically unstable columns (both lanes of the pair)
     if constexpr (!ice) {

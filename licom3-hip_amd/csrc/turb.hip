@@ -28,8 +28,14 @@
 // interpolates between in two cache lines instead of twelve; the zero-shear
 // table likewise, and the five background tables in 64-byte entries.
 //
+// The closure is bound by FP64 divisions and elementary functions, not by
+// memory: the values of elementary functions that do not change from level
+// to level (pi/4, the logs of the table constants, eplatidepend_fun's
+// denominator) are evaluated once per lane.  Forcing a third wave per SIMD
+// (launch bounds) spills registers and is slower.
+//
 // Compiled with -ffp-contract=off and written in the reference's operation
-// order; atan, sin, cos, acosh and log come from the device math library.
+// order; atan, sincos, acosh and log come from the device math library.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -69,20 +75,30 @@ struct TurbArgs {
     licom_turb_fields f;
 };
 
+// Elementary-function values that do not change from level to level,
+// evaluated once per lane with the same device functions.
+struct Consts {
+    double quarter;    // atan(1) = pi/4
+    double lxs2, lrat2;   // log of the 2-D table's first geometric key and ratio
+    double lxs1, lrat1;   // the same for the zero-shear table
+    double eden;       // eplatidepend_fun's denominator: f30 * acosh(an0 / f30)
+};
+
 __device__ __forceinline__ double fsign(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
 __device__ __forceinline__ int isgn(int a, int b) { return b >= 0 ? abs(a) : -abs(a); }
 
 // Cell (l0, l1) of x in an expabs table: uniform step delta below xt[MT+MT0],
 // ratio rat beyond.  False when the cell does not bracket x (the reference
 // then leaves its outputs alone).  The log estimate is kept inside the table.
-__device__ bool table_cell(const double *__restrict__ xt, double x, double delta, double rat,
-                           int &l0, int &l1) {
+// lxs = log(xt[MT+MT0]) and lrat = log(rat), the same for every call.
+__device__ bool table_cell(const double *__restrict__ xt, double x, double delta, double lxs,
+                           double lrat, int &l0, int &l1) {
     const double ax = fabs(x), xs = xt[MT + MT0], xe = xt[2 * MT];
     if (ax >= xs && ax >= xe) {
         l0 = l1 = (int)fsign((double)MT, x);
     } else {
         int l = ax < xs ? (int)(x / delta) + (int)fsign(1.0, x)
-                        : (int)fsign((double)MT0 + ((log(ax) - log(xs)) / log(rat)), x) +
+                        : (int)fsign((double)MT0 + ((log(ax) - lxs) / lrat), x) +
                               (int)fsign(1.0, x);
         l = min(max(l, -MT), MT);
         if (fabs(xt[MT + l]) < ax)
@@ -97,12 +113,12 @@ __device__ bool table_cell(const double *__restrict__ xt, double x, double delta
 }
 
 // interp1d_expabs over the zero-shear table (readyc.cpp:915-975)
-__device__ bool interp_shear0(const TurbArgs &a, double x, double o[4]) {
+__device__ bool interp_shear0(const TurbArgs &a, const Consts &c, double x, double o[4]) {
     if (__builtin_isnan(x)) return false;
     const double *xt = a.x1;
     x = x > xt[2 * MT] ? xt[2 * MT] : (x < xt[0] ? xt[0] : x);
     int l0, l1;
-    if (!table_cell(xt, x, a.dand2on2, a.rnd2on2, l0, l1)) return false;
+    if (!table_cell(xt, x, a.dand2on2, c.lxs1, c.lrat1, l0, l1)) return false;
     const double span = xt[MT + l1] - xt[MT + l0], dx = x - xt[MT + l0];
     const Q4 f0 = a.tab1[MT + l0], f1 = a.tab1[MT + l1];
 #pragma unroll
@@ -116,7 +132,8 @@ __device__ bool interp_shear0(const TurbArgs &a, double x, double o[4]) {
 // interp2d_expabs over the (Ri, Ri_d) table (readyc.cpp:977-1115); ri and
 // rid are clamped in place as in the reference.  0: outputs left alone,
 // 1: interpolated, 2: beyond the realizable Ri (zeros).
-__device__ int interp_table2(const TurbArgs &a, double &ri, double &rid, double o[4]) {
+__device__ int interp_table2(const TurbArgs &a, const Consts &c, double &ri, double &rid,
+                             double o[4]) {
     const double *rb = a.rib;
     const double lo = rb[0], hi = rb[2 * MT];
     if (ri > hi) {
@@ -150,37 +167,34 @@ __device__ int interp_table2(const TurbArgs &a, double &ri, double &rid, double 
     }
     if (__builtin_isnan(ri) || __builtin_isnan(rid)) return 0;
     int d0, d1, r0, r1;
-    if (!table_cell(rb, rid, a.dri, a.rri, d0, d1)) return 0;
+    if (!table_cell(rb, rid, a.dri, c.lxs2, c.lrat2, d0, d1)) return 0;
     if (ri > fmin(rb[MT + a.irimax[MT + d0]], rb[MT + a.irimax[MT + d1]])) {
         o[0] = o[1] = o[2] = o[3] = 0.0;
         return 2;
     }
-    if (!table_cell(rb, ri, a.dri, a.rri, r0, r1)) return 0;
+    if (!table_cell(rb, ri, a.dri, c.lxs2, c.lrat2, r0, r1)) return 0;
     const double span_d = rb[MT + d1] - rb[MT + d0], span_r = rb[MT + r1] - rb[MT + r0];
     const double dd = rid - rb[MT + d0], dr = ri - rb[MT + r0];
     const size_t row0 = (size_t)(MT + d0) * W2, row1 = (size_t)(MT + d1) * W2;
-    const Q4 f00 = a.tab2[row0 + MT + r0];
-    Q4 f10 = f00, f01 = f00;
-    if (d1 != d0) {
-        f10 = a.tab2[row0 + MT + r1];
-        f01 = a.tab2[row1 + MT + r0];
-    }
-#pragma unroll
+    const double *e00 = a.tab2[row0 + MT + r0].v, *e10 = a.tab2[row0 + MT + r1].v,
+                 *e01 = a.tab2[row1 + MT + r0].v;
+#pragma unroll 1
     for (int m = 0; m < 4; ++m) {
         double s_d = 0.0, s_r = 0.0;
+        const double f00 = e00[m];
         if (d1 != d0) {
-            s_d = (f01.v[m] - f00.v[m]) / span_d;
-            s_r = (f10.v[m] - f00.v[m]) / span_r;
+            s_d = (e01[m] - f00) / span_d;
+            s_r = (e10[m] - f00) / span_r;
         }
-        o[m] = f00.v[m] + s_r * dr + s_d * dd;
+        o[m] = f00 + s_r * dr + s_d * dd;
     }
     return 1;
 }
 
-__device__ double eplat(double f, double an) {
+__device__ double eplat_den() {
     const double an0 = 5.24e-3;
-    const double f30 = 4.0 * atan(1.0) / 43082.0;
-    return f * acosh(an / f) / (f30 * acosh(an0 / f30));
+    const double f30 = 4.0 * atan(1.0) / 43082.0;   // f at 30 degrees
+    return f30 * acosh(an0 / f30);
 }
 
 __global__ __launch_bounds__(TPB) void k_turb(TurbArgs a) {
@@ -260,7 +274,14 @@ __global__ __launch_bounds__(TPB) void k_turb(TurbArgs a) {
 
     // ---- 3. the closure, level by level from the surface
     const double b1 = a.b1, dth = a.dth, coriol = a.fcort[q];
-    const double quarter = atan(1.0);
+    Consts cst;
+    cst.quarter = atan(1.0);
+    cst.lxs2 = log(a.rib[MT + MT0]);
+    cst.lrat2 = log(a.rri);
+    cst.lxs1 = log(a.x1[MT + MT0]);
+    cst.lrat1 = log(a.rnd2on2);
+    cst.eden = eplat_den();
+    const double quarter = cst.quarter;
     const double buoytot = F.vit[q] * F.buoytur[q] * 1.0e4 + F.vit[q] * F.buoysol[q] * 1.0e4;
     const double al0 = 0.17 * amld;
     bool alive = true, below = false;
@@ -288,7 +309,7 @@ __global__ __launch_bounds__(TPB) void k_turb(TurbArgs a) {
                 const bool pure = an2 < a.rib[0] * s2;
                 double t4[4];
                 if (pure) {
-                    if (interp_shear0(a, and2on2, t4)) {
+                    if (interp_shear0(a, cst, and2on2, t4)) {
                         amtaun2 = t4[0];
                         sm = t4[1];
                         sh = t4[2];
@@ -296,7 +317,7 @@ __global__ __launch_bounds__(TPB) void k_turb(TurbArgs a) {
                     }
                     slq2 = (-amtaun2) / ((b1 * b1) * ri + 1.0e-25);
                 } else {
-                    if (interp_table2(a, ri1, rid1, t4)) {
+                    if (interp_table2(a, cst, ri1, rid1, t4)) {
                         slq2 = t4[0];
                         sm = t4[1];
                         sh = t4[2];
@@ -326,7 +347,8 @@ __global__ __launch_bounds__(TPB) void k_turb(TurbArgs a) {
                 if (an2 < 0.0) {
                     epson2 = 0.288;
                 } else {
-                    const double lat = fmax(nofsmall ? 0.0 : eplat(fabs(coriol), an), 7.0e-2);
+                    const double fa = fabs(coriol);
+                    const double lat = fmax(nofsmall ? 0.0 : fa * acosh(an / fa) / cst.eden, 7.0e-2);
                     epson2 = 0.288 * lat;
                 }
                 if (ri > 0.0) {
@@ -366,11 +388,13 @@ __global__ __launch_bounds__(TPB) void k_turb(TurbArgs a) {
                         alive = false;
                         break;
                     }
-                    const double brit = cos(theta) * back_ra, bric = sin(theta) * back_ra;
+                    double st, ct;
+                    sincos(theta, &st, &ct);
+                    const double brit = ct * back_ra, bric = st * back_ra;
                     back_ri1 = brit + bric;
                     double back_rid1 = brit - bric;
                     if (glt) {
-                        if (interp_table2(a, back_ri1, back_rid1, t4)) {
+                        if (interp_table2(a, cst, back_ri1, back_rid1, t4)) {
                             slq2_bk = t4[0];
                             sm_bk = t4[1];
                             sh_bk = t4[2];

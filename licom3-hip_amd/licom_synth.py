@@ -299,3 +299,66 @@ def make_turb_case(case: Case, seed: int = 7, tables: dict | None = None) -> Tur
                     grid=dict(kmt=kmt, zkp=zkp, dzp=dzp, fcort=fcort),
                     tables=tables if tables is not None else turb_tables(), fields=fields,
                     params=params)
+
+
+def make_turb_case_device(case: Case, torch, device, seed: int = 7,
+                          tables: dict | None = None) -> TurbCase:
+    """make_turb_case's profiles drawn on the device with torch's generator
+    (for grids too large to draw on the host); the fields are float64
+    device tensors, the grid and tables stay numpy."""
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    km, jmt, imt = case.km, case.jmt, case.imt
+    f64 = dict(dtype=torch.float64, device=device)
+    kmt_np = case.grid["kmt"].astype(np.int32)
+    zkp = case.grid["zkp"].astype(np.float64)
+    dzp = zkp[:-1] - zkp[1:]
+    kmt = torch.from_numpy(kmt_np).to(device)
+    depth = torch.from_numpy(-0.5 * (zkp[:-1] + zkp[1:])).to(device)[:, None, None]
+    lev = torch.arange(km, device=device)[:, None, None]
+    lat = np.deg2rad(case.grid["lat_t"])
+    fcort = np.repeat((2.0 * 7.292e-5 * np.sin(lat))[:, None], imt, axis=1)
+    fcort[jmt // 2, :] = 0.0
+    sinlat = torch.from_numpy(np.abs(np.sin(lat))).to(device)[:, None]
+
+    def u(lo, hi, shape):
+        return torch.rand(shape, generator=g, **f64) * (hi - lo) + lo
+
+    def nrm(sd, shape):
+        return torch.randn(shape, generator=g, **f64) * sd
+
+    s3, s2 = (km, jmt, imt), (jmt, imt)
+    fields = {}
+    fields["vit"] = (lev < kmt[None]).to(torch.float64)
+    mld = u(10.0, 120.0, s2)
+    ts = 28.0 - 20.0 * sinlat + nrm(0.5, s2)
+    fields["at"] = (ts[None] - 8.0 * (1.0 - torch.exp(-torch.clamp(depth - mld[None], min=0.0) / 400.0))
+                    + nrm(0.01, s3))
+    del mld, ts
+    fields["pdensity"] = 1022.0 + 5.0 * (1.0 - torch.exp(-depth / 800.0)) + nrm(0.002, s3)
+    s2t = 10.0 ** u(-8.0, -4.0, s3)
+    rict = 10.0 ** u(-7.0, -4.0, s3)
+    conv = (torch.rand(s2, generator=g, device=device) < 0.33)[None] & (lev < 3)
+    rict = torch.where(conv, -(10.0 ** u(-7.0, -5.0, s3)), rict)
+    del conv
+    lk = torch.randint(3, max(km - 1, 4), s2, generator=g, device=device)
+    inv = (torch.rand(s2, generator=g, device=device) < 0.1)[None] & (lev == lk[None])
+    rict = torch.where(inv, -(10.0 ** u(-7.0, -5.0, s3)), rict)
+    del inv, lk
+    calm = (torch.rand(s2, generator=g, device=device) < 0.03)[None] & (lev < 4)
+    s2t = torch.where(calm, torch.full_like(s2t, 1.0e-10), s2t)
+    rict = torch.where(calm, torch.full_like(rict, -1.0e-6), rict)
+    del calm
+    fields["rit"] = rict / (s2t + 1.0e-25)
+    fields["ricdttms"] = fields["rit"] * u(-1.5, 1.5, s3)
+    fields["s2t"], fields["rict"] = s2t, rict
+    bottom = torch.from_numpy(np.take(np.concatenate([[0.0], -zkp[1:]]), kmt_np)).to(device)
+    fields["fz_tide"] = torch.exp(-torch.clamp(bottom[None] - depth, min=0.0) / 500.0) * fields["vit"]
+    fields["buoytur"] = nrm(1.0e-8, s2)
+    fields["buoysol"] = -nrm(5.0e-9, s2).abs()
+    fields["wave_dis"] = 10.0 ** u(-4.0, -2.0, s2)
+    fields = {k: v.contiguous() for k, v in fields.items()}
+    params = dict(ncc=int(round(1.0 / case.params["oncc"])), dfricmx=100.0e-4, dwndmix=10.0e-4)
+    return TurbCase(imt=imt, jmt=jmt, km=km, grid=dict(kmt=kmt_np, zkp=zkp, dzp=dzp, fcort=fcort),
+                    tables=tables if tables is not None else turb_tables(), fields=fields,
+                    params=params)

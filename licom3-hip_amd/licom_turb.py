@@ -110,8 +110,14 @@ class TurbProducer:
         self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
         lt.check(L.licom_turb_set_stream(self.handle, _VP(self.stream.cuda_stream)),
                  "licom_turb_set_stream")
-        self.inputs = {k: torch.from_numpy(np.ascontiguousarray(tc.fields[k])).to(self.device)
-                       for k in INPUTS}
+        self.inputs = {}
+        for k in INPUTS:
+            v = tc.fields[k]
+            if isinstance(v, torch.Tensor):
+                assert v.dtype == torch.float64 and v.is_contiguous() and v.device == self.device
+                self.inputs[k] = v
+            else:
+                self.inputs[k] = torch.from_numpy(np.ascontiguousarray(v)).to(self.device)
         km, jmt, imt = self.km, self.jmt, self.imt
         z = lambda *s: torch.zeros(s, dtype=torch.float64, device=self.device)  # noqa: E731
         self.outputs = dict(akmt=z(km, jmt, imt), akt=z(2, km, jmt, imt), ak_tide=z(km, jmt, imt),
