@@ -356,7 +356,8 @@ def run_turb(args, torch, lt, case, model):
     return {"kernel": "k_turb", "avg_launch_ms": round(avg, 4), "min_launch_ms": round(ms[0], 4),
             "launches": reps, "column_cells_per_s": cells / (avg * 1e-3),
             "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                         "traffic": traffic_from_profiles("k_turb", args.workload),
                          "algorithmic_bytes": alg},
             "turb_plus_fp64_step_ms": round(1000 * fused, 3),
             "tables": "synthetic Canuto tables on the reference grids (2925 x 2925 Ri x Ri_d)"}
