@@ -561,7 +561,7 @@ def run_serving(args, torch, lt, le, offline_value, iso, load):
     print(f"[bench] serving: {len(reqs)} requests at {rate:.2f}/s (load {load})", file=sys.stderr,
           flush=True)
     loop = le.RequestLoop(hbm_cap_bytes=args.hbm_cap_gb * 1e9, chunk_steps=args.chunk_steps,
-                          policy=args.serve_policy)
+                          policy=args.serve_policy, lanes=args.serve_lanes)
     stats = loop.run(reqs)
     iso_t = lambda r: r.n_steps * iso.get((r.case.nx, r.case.ny, r.case.km), np.nan)  # noqa: E731
     served = [r for r in reqs if r.finished is not None]
@@ -611,9 +611,13 @@ def main():
     ap.add_argument("--loads", default="0.25,0.8",
                     help="request arrival rates as fractions of the offline batched capacity")
     ap.add_argument("--chunk-steps", type=int, default=4)
-    ap.add_argument("--serve-policy", default="srpt,classes",
-                    help="request loop scheduling, comma-separated: least remaining work first "
-                         "over one batch (srpt), size classes advancing in lock step (classes)")
+    ap.add_argument("--serve-policy", default="solo,srpt,classes",
+                    help="request loop scheduling, comma-separated: every request alone on one "
+                         "of --serve-lanes streams, least remaining work first (solo), least "
+                         "remaining work first over one batch (srpt), size classes advancing in "
+                         "lock step (classes)")
+    ap.add_argument("--serve-lanes", type=int, default=4,
+                    help="streams of the solo policy")
     ap.add_argument("--hbm-cap-gb", type=float, default=200.0)
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the HBM traffic summary from two rocprofv3 --pmc runs")

@@ -111,6 +111,13 @@ class Ensemble:
             lt.check(L.licom_batch_set_persistent(h, 1), "licom_batch_set_persistent")
         self.captured = 0
 
+    def set_stream(self, stream):
+        """Moves the batch's launches to another stream; the caller orders it
+        after the work already queued on the old one."""
+        lt.check(_lib().licom_batch_set_stream(self.handle, ctypes.c_void_p(stream.cuda_stream)),
+                 "licom_batch_set_stream")
+        self.stream = stream
+
     def _order(self):
         if self.member_stream is not None:
             self.stream.wait_stream(self.member_stream)
@@ -228,7 +235,7 @@ class Autotuner:
 
 class Request:
     __slots__ = ("rid", "arrival", "case", "n_steps", "done_steps", "model", "admitted",
-                 "finished", "bytes", "error")
+                 "finished", "bytes", "error", "ens", "lane", "queued")
 
     def __init__(self, rid, arrival, case, n_steps):
         self.rid, self.arrival, self.case, self.n_steps = rid, arrival, case, int(n_steps)
@@ -238,6 +245,9 @@ class Request:
         self.finished = None
         self.bytes = member_bytes(case.nx, case.ny, case.km)
         self.error = None     # reason the request was rejected, if it was
+        self.ens = None       # solo policy: the request's own batch of one
+        self.lane = None      # solo policy: the lane it is bound to once started
+        self.queued = 0       # solo policy: steps launched and not yet retired
 
 
 class _Lane:
@@ -284,14 +294,27 @@ class RequestLoop:
     exceeds the device's free memory while nothing else is resident -- is
     rejected: it is not run, its `error` says why, and it is listed in
     `rejected` (and counted in the statistics) instead of blocking the
-    requests behind it."""
+    requests behind it.
+
+    policy="solo": no lock step at all.  Every request runs alone (a batch of
+    one, the launches of an isolated member) on one of `lanes` streams; a lane
+    that can take a chunk runs the request with the least remaining work
+    (remaining steps x cells per step) among those bound to it and those not
+    started yet, so a short request that arrives while long ones run starts
+    at the next chunk boundary of a lane and is not slowed to a batch's step
+    time.  A request stays on the lane it started on (its state is only
+    touched by that stream); chunks are queued two deep per lane."""
 
     def __init__(self, hbm_cap_bytes=200e9, chunk_steps=8, tuner=None, keep_results=False,
                  class_cells=(5.0e5, 2.0e6), one_at_a_time=(), one_at_a_time_upto=None,
                  free_margin_bytes=2.0e9, free_bytes="device", policy="classes",
-                 fill_cells=4.0e6, admit_threads=2):
-        if policy not in ("classes", "srpt"):
-            raise ValueError(f"policy must be 'classes' or 'srpt', not {policy!r}")
+                 fill_cells=4.0e6, admit_threads=2, lanes=4):
+        if policy not in ("classes", "srpt", "solo"):
+            raise ValueError(f"policy must be 'classes', 'srpt' or 'solo', not {policy!r}")
+        if lanes < 1:
+            raise ValueError("lanes must be at least 1")
+        # "solo": streams the requests run on, one request per chunk
+        self.lanes = int(lanes)
         # "srpt": one batch of the resident requests with the least remaining
         # work (remaining steps x cells per step), as many as fill_cells cells
         # per step (about what fills the GPU), re-chosen at every chunk
@@ -429,6 +452,66 @@ class RequestLoop:
         ln.running = q[0][1] if q else []
         return launched, upd, rb
 
+    @staticmethod
+    def _solo_pick(active, li):
+        """The request lane `li` runs next: least remaining work (steps not yet
+        launched x cells per step) among the requests bound to the lane and
+        those not started, earliest arrival on ties; None if there is none."""
+        best, key = None, None
+        for r in active:
+            if r.lane not in (None, li):
+                continue
+            left = r.n_steps - r.done_steps - r.queued
+            if left <= 0:
+                continue
+            k = (left * (r.case.nx * r.case.ny * r.case.km), r.arrival)
+            if key is None or k < key:
+                best, key = r, k
+        return best
+
+    def _solo_turn(self, lanes, active, done, grave, clock, t0, torch):
+        """One scheduling turn of the solo policy: retire the lanes' finished
+        chunks, then fill every lane's queue (two chunks deep), idle lanes
+        first.  Returns (launched, cell-updates retired)."""
+        upd, launched = 0, False
+        for ln in lanes:
+            q = ln.__dict__.setdefault("queue", [])
+            while q and q[0][0].query():
+                _, r, steps = q.pop(0)
+                r.queued -= steps
+                r.done_steps += steps
+                upd += steps * r.case.nx * r.case.ny * r.case.km * 2
+                if r.done_steps >= r.n_steps:
+                    r.finished = clock() - t0
+                    done.append(r)
+                    self._check_finite(r, ln.stream)
+                    if self.keep_results:
+                        self.results[r.rid] = r.model.host_state()["at"]
+                    grave.append((r.ens, 0.0))
+                    grave.append((r.model, r.bytes))
+                    r.ens = r.model = None
+        active[:] = [r for r in active if r.finished is None]
+        for depth in (0, 1):
+            for li, ln in enumerate(lanes):
+                if len(ln.queue) != depth:
+                    continue
+                r = self._solo_pick(active, li)
+                if r is None:
+                    continue
+                if r.lane is None:
+                    r.lane = li
+                    r.ens.set_stream(ln.stream)
+                steps = min(self.chunk, r.n_steps - r.done_steps - r.queued)
+                r.ens.step(steps)
+                ev = torch.cuda.Event()
+                ev.record(ln.stream)
+                ln.queue.append((ev, r, steps))
+                r.queued += steps
+                launched = True
+        for ln in lanes:
+            ln.busy = bool(ln.queue)
+        return launched, upd
+
     def run(self, requests, clock=time.perf_counter, sleep=time.sleep):
         import queue
         import threading
@@ -451,13 +534,15 @@ class RequestLoop:
         dev = None
         grave, grave_bytes = [], 0.0
         todo, ready, failures = queue.Queue(), queue.Queue(), []
+        tune_lock = threading.Lock()
+        solo_active = []      # solo policy: admitted requests not finished
         inflight = 0
         worker = None
 
         def start_lanes():
             nonlocal dev, worker
             dev = torch.cuda.current_device()
-            nclass = 1 if self.policy == "srpt" else len(self.class_cells) + 1
+            nclass = {"srpt": 1, "solo": self.lanes}.get(self.policy, len(self.class_cells) + 1)
             for k in range(nclass):
                 ln = _Lane(torch.cuda.Stream(device=dev))
                 ln.single = k in self.one_at_a_time
@@ -476,6 +561,10 @@ class RequestLoop:
                 try:
                     with torch.cuda.stream(side):
                         r.model = lt.TracerModel(r.case, diagnostics=True, precision="mixed", stream=side)
+                        if self.policy == "solo":
+                            with tune_lock:
+                                tuning = [self.tuner.tune(r.case)] if self.tuner else None
+                            r.ens = Ensemble([r.model], tuning=tuning, stream=side, wait_members=False)
                     side.synchronize()     # state resident before a class stream reads it
                 except Exception as e:     # handed to the loop, which raises it
                     failures.append(e)
@@ -519,10 +608,17 @@ class RequestLoop:
                     raise failures[0]
                 inflight -= 1
                 r.admitted = clock() - t0
+                if self.policy == "solo":
+                    solo_active.append(r)
+                    continue
                 ln = lanes[0 if self.policy == "srpt" else self._class(r)]
                 ln.active.append(r)
                 ln.changed = True
             launched = False
+            if self.policy == "solo" and lanes:
+                launched, upd = self._solo_turn(lanes, solo_active, done, grave, clock, t0, torch)
+                updates += upd
+                grave_bytes = sum(b for _, b in grave)
             if self.policy == "srpt" and lanes:
                 ln = lanes[0]
                 launched, upd, rb = self._srpt_turn(ln, done, grave, clock, t0, torch)
@@ -580,12 +676,12 @@ class RequestLoop:
                     ln.event.record(ln.stream)
                     ln.busy = True
                     launched = True
-            peak_active = max(peak_active, sum(len(ln.active) for ln in lanes))
+            peak_active = max(peak_active, sum(len(ln.active) for ln in lanes) + len(solo_active))
             idle = not any(ln.busy for ln in lanes)
             if grave and (idle or grave_bytes > 0.25 * self.cap):
                 torch.cuda.synchronize()
                 flush()
-            if idle and not any(ln.active for ln in lanes) and inflight == 0:
+            if idle and not any(ln.active for ln in lanes) and not solo_active and inflight == 0:
                 if i >= len(pending):
                     break
                 wait = pending[i].arrival - (clock() - t0)

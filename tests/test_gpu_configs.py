@@ -235,7 +235,7 @@ def test_ensemble_of_64_graph_replay_equals_members_alone(tmp_path, need_gpu):
 
 
 # --------------------------------------------------------- request loop
-@pytest.mark.parametrize("policy", ["classes", "srpt"])
+@pytest.mark.parametrize("policy", ["classes", "srpt", "solo"])
 def test_request_loop_poisson_requests_same_alone_and_in_the_mix(policy, need_gpu):
     """Requests of the serving workload (domains of the ensemble
     distribution, 24..240 steps, Poisson arrivals) served under a cap that

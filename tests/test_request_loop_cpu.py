@@ -74,3 +74,33 @@ def test_srpt_pick_orders_by_remaining_work_and_fills():
     assert [r.rid for r in loop._srpt_pick([d])] == [3]
     with pytest.raises(ValueError):
         le.RequestLoop(policy="fifo")
+
+
+def test_solo_pick_least_remaining_work_bound_or_unstarted():
+    """The solo policy's lane takes, among the requests bound to it and those
+    not started, the one with the least remaining work (steps not yet launched
+    x cells per step); requests bound to another lane are never taken."""
+    class C:
+        def __init__(self, nx, ny, km):
+            self.nx, self.ny, self.km = nx, ny, km
+
+    class R:
+        def __init__(self, rid, shape, steps, done=0, queued=0, lane=None, arrival=0.0):
+            self.rid, self.case, self.n_steps = rid, C(*shape), steps
+            self.done_steps, self.queued, self.lane, self.arrival = done, queued, lane, arrival
+
+    small, big = (96, 64, 30), (360, 218, 55)
+    a = R(0, big, 240, done=200, lane=1)         # 40 x 4.3M, on lane 1
+    b = R(1, small, 240, done=100, lane=0)       # 140 x 184k, on lane 0
+    c = R(2, small, 24, arrival=0.5)             # 24 x 184k, not started
+    d = R(3, small, 24, arrival=0.4)             # same work, arrived first
+    e = R(4, small, 30, done=20, queued=10, lane=0)   # everything launched
+    pick = le.RequestLoop._solo_pick
+    assert pick([a, b, c, d, e], 0).rid == 3
+    assert pick([a, b, c, e], 0).rid == 2
+    assert pick([a, b, e], 0).rid == 1
+    assert pick([a, b, e], 1).rid == 0
+    assert pick([a, e], 0) is None
+    assert pick([a, b], 2) is None
+    with pytest.raises(ValueError):
+        le.RequestLoop(policy="solo", lanes=0)
