@@ -363,6 +363,85 @@ def run_turb(args, torch, lt, case, model):
             "tables": "synthetic Canuto tables on the reference grids (2925 x 2925 Ri x Ri_d)"}
 
 
+def readyt_algorithmic_bytes(case):
+    """Bytes readyt_ must move once per call, every cell of the block
+    (halos included, as the reference computes them): reads vit, viv, at and
+    atb of both tracers, u, v, utf, vtf (10 x 8 B); writes utl, vtl, utf,
+    vtf, dlu, dlv, pdensity, gg, pp, ppa, ppb, ppc, alpha, beta, akt of
+    both tracers (16 x 8 B) and ric, rict, ricdttms, ricdt, rit on km-1
+    planes; the 2-D fields are under 1%."""
+    n2 = case.imt * case.jmt
+    return n2 * case.km * 8 * (10 + 16) + n2 * (case.km - 1) * 8 * 5
+
+
+def run_readyt(args, torch, lt, case, model):
+    """readyt_ at the workload size on the FP64 model's at / atb, the other
+    fields built on the device; its CPU baseline is the oracle on a band."""
+    import licom_readyt as lr
+    import licom_synth as ls
+    dev = model.device
+    km, jmt, imt = case.km, case.jmt, case.imt
+    g = ls.readyt_grid(case)
+    f64 = dict(dtype=torch.float64, device=dev)
+    lev = torch.arange(km, device=dev)[:, None, None]
+    kmt = torch.from_numpy(case.grid["kmt"]).to(dev)
+    kmu = torch.from_numpy(g["kmu"]).to(dev)
+    st = model.state
+    f = dict(vit=(lev < kmt[None]).to(torch.float64), viv=(lev < kmu[None]).to(torch.float64),
+             at=st["at"], atb=st["atb"], h0=torch.zeros((jmt, imt), **f64),
+             u=st["utf"] * 1.01, v=st["vtf"] * 0.99,
+             psa=torch.full((jmt, imt), 100.0, **f64), swv=torch.full((jmt, imt), 1.0e-5, **f64),
+             nswv=torch.full((jmt, imt), -1.0e-5, **f64),
+             h0l=torch.zeros((jmt, imt), **f64), h0f=torch.zeros((jmt, imt), **f64),
+             utl=st["utl"].clone(), vtl=st["vtl"].clone(), utf=st["utf"].clone(), vtf=st["vtf"].clone())
+    r = lr.Readyt(imt, jmt, km, g, f, device=dev, stream=model.stream)
+    for _ in range(max(1, args.warmup)):
+        r.readyt(1)
+    r.sync()
+    reps = max(3, args.steps)
+    tot, parts = [], []
+    for _ in range(reps):
+        r.readyt(1)
+        ms = r.last_ms()
+        tot.append(ms[0])
+        parts.append(ms[1:])
+    r.close()
+    del r, f
+    torch.cuda.empty_cache()
+    avg = sum(tot) / len(tot)
+    part = [sum(p[i] for p in parts) / len(parts) for i in range(3)]
+    alg = readyt_algorithmic_bytes(case)
+    ach = alg / (avg * 1e-3) / 1e9
+    print(f"[bench] readyt: {avg:.3f} ms per call ({ach:.0f} GB/s algorithmic)", file=sys.stderr,
+          flush=True)
+    out = {"ms_per_call": round(avg, 4), "calls": reps,
+           "kernel_ms": {"k_readyt_col": round(part[0], 4), "k_readyt_pg": round(part[1], 4),
+                         "k_readyt_sfc": round(part[2], 4)},
+           "cells_per_s": imt * jmt * km / (avg * 1e-3),
+           "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": traffic_from_profiles("k_readyt_col", args.workload),
+                        "algorithmic_bytes": alg}}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import orc
+        band = ls.make_case(args.workload, ny=min(64, case.ny))
+        gb = ls.readyt_grid(band)
+        fb = ls.readyt_fields(band, gb)
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            orc.readyt(band.imt, band.jmt, band.km, gb, fb, ist=1)
+            times.append(time.perf_counter() - t0)
+        cells = band.imt * band.jmt * band.km
+        out["cpu_baseline"] = {"value": cells / sorted(times)[1], "unit": "cells/s",
+                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                               "kind": "port",
+                               "sample": f"oracle readyt_ restatement on a {band.imt}x{band.jmt}x{band.km} "
+                                         "block, median of 3 calls"}
+    return out
+
+
 def run_variant(model, args, torch, lt, case, updates):
     """Warm-up, then K timed steps of one precision form; per-kernel HIP-event
     times on the library's stream over the timed steps."""
@@ -572,7 +651,8 @@ def run_serving(args, torch, lt, le, offline_value, iso, load):
     worst = sorted(served, key=lambda r: -slow_of[r.rid])[:4]
     stats["slowest"] = [{"shape": f"{r.case.nx}x{r.case.ny}x{r.case.km}", "steps": r.n_steps,
                          "isolated_s": round(iso_t(r), 5), "admission_s": round(r.admitted - r.arrival, 5),
-                         "run_s": round(r.finished - r.admitted, 5), "slowdown": round(slow_of[r.rid], 2)}
+                         "run_s": round(r.finished - r.admitted, 5), "slowdown": round(slow_of[r.rid], 2),
+                         "admission_phases_s": {k: round(v, 5) for k, v in r.timing.items()}}
                         for r in worst]
     last_arrival = max(r.arrival for r in reqs)
     stats.update({"policy": args.serve_policy, "arrival_rate_per_s": rate, "load_fraction": load,
@@ -601,6 +681,7 @@ def main():
     ap.add_argument("--no-error-study", action="store_true")
     ap.add_argument("--no-icesnow", action="store_true")
     ap.add_argument("--no-turb", action="store_true")
+    ap.add_argument("--no-readyt", action="store_true")
     ap.add_argument("--members", type=int, default=64)
     ap.add_argument("--autotune-cache", default=None,
                     help="JSON file to read and write the ensemble's tuned launch shapes "
@@ -677,12 +758,18 @@ def main():
                      "rms_dS_psu": rms[1], "wet_fraction_dT_above_1e-4": frac[0],
                      "wet_fraction_dS_above_1e-4": frac[1],
                      "steps": args.warmup + args.steps, "against": "FP64 drop-in"}
+    if "mixed" in models:
+        models.pop("mixed").close()
+        torch.cuda.empty_cache()
     ice = None
     if "fp64" in models and not args.no_icesnow and not args.no_kernel_timing:
         ice = run_icesnow(models["fp64"], args, torch, lt, case, runs["fp64"])
     turb = None
     if "fp64" in models and not args.no_turb:
         turb = run_turb(args, torch, lt, case, models["fp64"])
+    readyt = None
+    if "fp64" in models and not args.no_readyt:
+        readyt = run_readyt(args, torch, lt, case, models["fp64"])
     for m in models.values():
         m.close()
     del models
@@ -745,6 +832,8 @@ def main():
         out["icesnow"] = ice
     if turb:
         out["turb"] = turb
+    if readyt:
+        out["readyt"] = readyt
     if ens_out:
         out["ensemble"] = ens_out
     if serve_out:

@@ -235,7 +235,7 @@ class Autotuner:
 
 class Request:
     __slots__ = ("rid", "arrival", "case", "n_steps", "done_steps", "model", "admitted",
-                 "finished", "bytes", "error", "ens", "lane", "queued")
+                 "finished", "bytes", "error", "ens", "lane", "queued", "timing")
 
     def __init__(self, rid, arrival, case, n_steps):
         self.rid, self.arrival, self.case, self.n_steps = rid, arrival, case, int(n_steps)
@@ -248,6 +248,8 @@ class Request:
         self.ens = None       # solo policy: the request's own batch of one
         self.lane = None      # solo policy: the lane it is bound to once started
         self.queued = 0       # solo policy: steps launched and not yet retired
+        self.timing = {}      # admission phases (s after arrival): handed to a worker,
+                              # picked up, context + upload done, batch made, resident
 
 
 class _Lane:
@@ -559,13 +561,17 @@ class RequestLoop:
                 if r is None:
                     return
                 try:
+                    r.timing["picked"] = clock() - t0 - r.arrival
                     with torch.cuda.stream(side):
                         r.model = lt.TracerModel(r.case, diagnostics=True, precision="mixed", stream=side)
+                        r.timing["context"] = clock() - t0 - r.arrival
                         if self.policy == "solo":
                             with tune_lock:
                                 tuning = [self.tuner.tune(r.case)] if self.tuner else None
                             r.ens = Ensemble([r.model], tuning=tuning, stream=side, wait_members=False)
+                            r.timing["batch"] = clock() - t0 - r.arrival
                     side.synchronize()     # state resident before a class stream reads it
+                    r.timing["resident"] = clock() - t0 - r.arrival
                 except Exception as e:     # handed to the loop, which raises it
                     failures.append(e)
                     ready.put(None)
@@ -595,6 +601,7 @@ class RequestLoop:
                     start_lanes()
                 i += 1
                 used += r.bytes
+                r.timing["handed"] = clock() - t0 - r.arrival
                 todo.put(r)
                 inflight += 1
             while True:

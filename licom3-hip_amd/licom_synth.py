@@ -362,3 +362,77 @@ def make_turb_case_device(case: Case, torch, device, seed: int = 7,
     return TurbCase(imt=imt, jmt=jmt, km=km, grid=dict(kmt=kmt_np, zkp=zkp, dzp=dzp, fcort=fcort),
                     tables=tables if tables is not None else turb_tables(), fields=fields,
                     params=params)
+
+
+# ---------------------------------------------------------------- readyt_
+def readyt_grid(case: Case, seed: int = 11) -> dict:
+    """licom_readyt_grid of a tracer case's block: kmu (the minimum kmt of
+    the four T cells around a U point), T->U weights, 1/dxu, 1/dyu, the
+    inverse column depths at U and T points, a smooth synthetic hbx/hby, dzp,
+    odzt and the T-level depths from zkp, the EOS table (to, so, C) and a
+    synthetic density offset po rising 4.5 kg/m^3 per 1000 m."""
+    rng = np.random.default_rng(seed)
+    km, jmt, imt = case.km, case.jmt, case.imt
+    g = case.grid
+    kmt = g["kmt"].astype(np.int32)
+    kmu = np.zeros_like(kmt)
+    kmu[:-1, 1:] = np.minimum(np.minimum(kmt[:-1, :-1], kmt[:-1, 1:]),
+                              np.minimum(kmt[1:, :-1], kmt[1:, 1:]))
+    zkp = g["zkp"].astype(np.float64)
+    dzp = zkp[:-1] - zkp[1:]
+    zkt = 0.5 * (zkp[:-1] + zkp[1:])
+    odzt = np.empty(km)
+    odzt[0] = 2.0 / dzp[0]
+    odzt[1:] = 1.0 / (zkt[:-1] - zkt[1:])
+    cum = np.concatenate([[0.0], np.cumsum(dzp)])
+    hu, ht = cum[kmu], cum[kmt]
+    inv = lambda h: np.where(h > 0.0, 1.0 / np.where(h > 0.0, h, 1.0), 0.0)  # noqa: E731
+    recip = lambda a: np.where(a > 0.0, 1.0 / np.where(a > 0.0, a, 1.0), 0.0)  # noqa: E731
+
+    def smooth(amp):
+        a = rng.normal(0.0, 1.0, (jmt, imt))
+        for _ in range(3):
+            a = 0.25 * (np.roll(a, 1, 1) + np.roll(a, -1, 1) + np.roll(a, 1, 0) + np.roll(a, -1, 0))
+        return amp * a
+
+    return dict(kmu=kmu, au0=g["au0"].copy(), aus=g["aus"].copy(), auw=g["auw"].copy(),
+                ausw=g["ausw"].copy(), dxur=recip(g["dxu"]), dyur=recip(g["dyu"]),
+                ohbu=inv(hu), ohbt=inv(ht), hbx=smooth(1.0e-3), hby=smooth(1.0e-3),
+                dzp=dzp, odzt=odzt, zkt=zkt, to=g["to"].copy(), so=g["so"].copy(),
+                po=4.5e-3 * (-zkt), eosc=g["eosc"].copy(),
+                od0=float(case.params["od0"]), od0cp=float(case.params["od0cp"]))
+
+
+def readyt_fields(case: Case, grid: dict, seed: int = 12, sentinel: float = 7.0) -> dict:
+    """licom_readyt_fields over a tracer case: the wet masks from kmt / kmu,
+    the case's at / atb and time levels, new velocities and surface height
+    (the current ones perturbed by 1%), a smooth surface pressure, surface
+    heat fluxes; outputs start at `sentinel` so that a test sees what a call
+    leaves alone."""
+    rng = np.random.default_rng(seed)
+    km, jmt, imt = case.km, case.jmt, case.imt
+    lev = np.arange(km)[:, None, None]
+    vit = (lev < case.grid["kmt"][None]).astype(np.float64)
+    viv = (lev < grid["kmu"][None]).astype(np.float64)
+    st = case.state
+    s2 = (jmt, imt)
+    f = dict(vit=vit, viv=viv, at=st["at"].copy(), atb=st["atb"].copy(),
+             h0=rng.normal(0.0, 0.3, s2),
+             u=st["utf"] * (1.0 + rng.normal(0.0, 0.01, st["utf"].shape)),
+             v=st["vtf"] * (1.0 + rng.normal(0.0, 0.01, st["vtf"].shape)),
+             psa=1.0e3 * np.cos(np.linspace(0.0, 6.0, imt))[None, :] * np.sin(np.linspace(0.0, 3.0, jmt))[:, None]
+             + rng.normal(0.0, 5.0, s2),
+             swv=rng.uniform(0.0, 3.0e-5, s2), nswv=rng.normal(-2.0e-5, 1.0e-5, s2),
+             h0l=st["h0l"].copy(), h0f=st["h0f"].copy() + rng.normal(0.0, 0.1, s2),
+             utl=st["utl"].copy(), vtl=st["vtl"].copy(), utf=st["utf"].copy(), vtf=st["vtf"].copy())
+    shapes = dict(dlu=(km, jmt, imt), dlv=(km, jmt, imt), pdensity=(km, jmt, imt), gg=(km, jmt, imt),
+                  pp=(km, jmt, imt), ppa=(km, jmt, imt), ppb=(km, jmt, imt), ppc=(km, jmt, imt),
+                  alpha=(km, jmt, imt), beta=(km, jmt, imt), ric=(km - 1, jmt, imt),
+                  rict=(km - 1, jmt, imt), ricdttms=(km - 1, jmt, imt), ricdt=(km - 1, jmt, imt),
+                  rit=(km - 1, jmt, imt), akt=(2, km, jmt, imt))
+    for k in ("buoytur", "buoysol", "pxb", "pyb", "wgp", "work", "whx", "why", "work1", "work2",
+              "pax", "pay"):
+        shapes[k] = s2
+    for k, sh in shapes.items():
+        f[k] = np.full(sh, sentinel)
+    return {k: np.ascontiguousarray(v, np.float64) for k, v in f.items()}

@@ -346,3 +346,77 @@ def turb_formld(z, t):
 
 def turb_eplatidepend(f, an):
     return _turb_lib().orc_turb_eplatidepend(float(f), float(an))
+
+
+# ------------------------------------------------------------------ readyt_
+READYT_GRID_2D = ("au0", "aus", "auw", "ausw", "dxur", "dyur", "ohbu", "ohbt", "hbx", "hby")
+READYT_GRID_1D = ("dzp", "odzt", "zkt", "to", "so", "po")
+READYT_FIELDS = ("vit", "viv", "at", "atb", "h0", "u", "v", "psa", "swv", "nswv",
+                 "h0l", "h0f", "utl", "vtl", "utf", "vtf",
+                 "dlu", "dlv", "pdensity", "gg", "pp", "ppa", "ppb", "ppc", "alpha", "beta",
+                 "ric", "rict", "ricdttms", "ricdt", "rit", "akt", "buoytur", "buoysol", "pxb", "pyb",
+                 "wgp", "work", "whx", "why", "work1", "work2", "pax", "pay", "ax", "ay", "az")
+
+
+class OrcReadytGrid(ctypes.Structure):
+    _fields_ = ([("imt", ctypes.c_int), ("jmt", ctypes.c_int), ("km", ctypes.c_int), ("kmu", _IP)] +
+                [(k, _DP) for k in READYT_GRID_2D + READYT_GRID_1D] +
+                [("eosc", _DP), ("od0", ctypes.c_double), ("od0cp", ctypes.c_double)])
+
+
+class OrcReadytFields(ctypes.Structure):
+    _fields_ = [(k, _DP) for k in READYT_FIELDS]
+
+
+def _readyt_lib():
+    L = lib()
+    if not getattr(L, "_readyt_bound", False):
+        L.orc_readyt.argtypes = [ctypes.POINTER(OrcReadytGrid), ctypes.POINTER(OrcReadytFields),
+                                 ctypes.c_int]
+        L.orc_thermal.argtypes = [ctypes.c_double] * 5 + [_DP, _DP]
+        L._readyt_bound = True
+    return L
+
+
+def readyt(imt, jmt, km, grid, fields, ist=1, threads=None):
+    """readyt_ (readyt.cpp:511-598) on numpy arrays: returns a dict with a
+    copy of every field after the call (inputs unchanged, in/out and output
+    arrays as the reference leaves them; ax/ay/az only if given)."""
+    L = _readyt_lib()
+    need = [k for k in READYT_FIELDS[:-3] if fields.get(k) is None]
+    if need:
+        raise ValueError(f"readyt: missing fields {need}")
+    keep = []
+    G = OrcReadytGrid()
+    G.imt, G.jmt, G.km = int(imt), int(jmt), int(km)
+    kmu = np.ascontiguousarray(grid["kmu"], np.int32)
+    keep.append(kmu)
+    G.kmu = _i(kmu)
+    for k in READYT_GRID_2D + READYT_GRID_1D + ("eosc",):
+        a = np.ascontiguousarray(grid[k], np.float64)
+        keep.append(a)
+        setattr(G, k, _d(a))
+    G.od0, G.od0cp = float(grid["od0"]), float(grid["od0cp"])
+    out = {k: (None if fields.get(k) is None else np.array(fields[k], np.float64, order="C"))
+           for k in READYT_FIELDS}
+    F = OrcReadytFields(*[_d(out[k]) for k in READYT_FIELDS])
+    old = None
+    if threads is not None:
+        old = L.orc_nthreads()
+        L.orc_set_nthreads(int(threads))
+    try:
+        L.orc_readyt(ctypes.byref(G), ctypes.byref(F), int(ist))
+    finally:
+        if old is not None:
+            L.orc_set_nthreads(old)
+    return {k: v for k, v in out.items() if v is not None}
+
+
+def thermal(tt, ss, ppa, vit, od0):
+    """thermal_cu's alpha, beta at one cell (readyt.cpp:272-294)."""
+    readyt_lib = _readyt_lib()
+    a, b = ctypes.c_double(), ctypes.c_double()
+    readyt_lib.orc_thermal(float(tt), float(ss), float(ppa), float(vit), float(od0),
+                           ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+

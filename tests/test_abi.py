@@ -86,3 +86,48 @@ def test_turb_create_validates_before_touching_a_device():
                              dp(z), dp(np.zeros(36)), ctypes.byref(T), 2, 1e-2, 1e-3, ctypes.byref(h))
     assert rc == -1 and b"irimax" in L.licom_last_error()
     assert not h.value
+
+
+def test_readyt_declarations_are_exported():
+    import licom_readyt
+    names = declared("licom_readyt.h")
+    assert names
+    L = ctypes.CDLL(licom_readyt.lt.lib_path())
+    assert [n for n in names if not hasattr(L, n)] == []
+    assert sorted(licom_readyt.EXPORTED) == names
+
+
+def test_readyt_create_validates_before_touching_a_device():
+    import numpy as np
+    import licom_readyt as lr
+    import licom_synth as ls
+    c = ls.make_case("C1", nx=16, ny=8, km=6, seed=2)
+    g = ls.readyt_grid(c)
+    L = lr.lib()
+    h = ctypes.c_void_p()
+    keep = []
+
+    def grid(km, **over):
+        G = lr.ReadytGrid()
+        G.imt, G.jmt, G.km = c.imt, c.jmt, km
+        kmu = np.ascontiguousarray(over.get("kmu", g["kmu"]), np.int32)
+        keep.append(kmu)
+        G.kmu = kmu.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        for k in lr.GRID_2D + lr.GRID_1D + ("eosc",):
+            a = np.ascontiguousarray(g[k], np.float64)
+            keep.append(a)
+            setattr(G, k, a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        G.od0, G.od0cp = g["od0"], g["od0cp"]
+        return G
+
+    assert L.licom_readyt_create(ctypes.byref(grid(1)), ctypes.byref(h)) == -1
+    assert b"invalid extents" in L.licom_last_error()
+    bad = g["kmu"].copy()
+    bad[2, 2] = 99
+    assert L.licom_readyt_create(ctypes.byref(grid(c.km, kmu=bad)), ctypes.byref(h)) == -1
+    assert b"kmu out of range" in L.licom_last_error()
+    G = grid(c.km)
+    G.ohbt = None
+    assert L.licom_readyt_create(ctypes.byref(G), ctypes.byref(h)) == -1
+    assert b"NULL" in L.licom_last_error()
+    assert not h.value
