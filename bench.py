@@ -700,6 +700,9 @@ def main():
     ap.add_argument("--serve-lanes", type=int, default=4,
                     help="streams of the solo policy")
     ap.add_argument("--hbm-cap-gb", type=float, default=200.0)
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="GPU_MAX_HW_QUEUES for this process (0: leave the environment's): the "
+                         "request loop's lanes and admission streams each want a queue of their own")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the HBM traffic summary from two rocprofv3 --pmc runs")
     ap.add_argument("--summarize-trace", metavar="TRACE_DIR",
@@ -721,6 +724,11 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.hw_queues:
+        if not 1 <= args.hw_queues <= 32:
+            print("bench.py: --hw-queues must be within 1..32", file=sys.stderr)
+            sys.exit(2)
+        os.environ["GPU_MAX_HW_QUEUES"] = str(args.hw_queues)
     import torch
     import licom_synth as ls
     import licom_tracer as lt

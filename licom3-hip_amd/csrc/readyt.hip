@@ -125,26 +125,59 @@ __global__ __launch_bounds__(256) void k_readyt_col(RGrid g, licom_readyt_fields
         f.h0f[q] = f.h0[q];
     }
     const double *C = S.c;
+    // Each level's inputs are loaded through plane buffers one level ahead,
+    // before the current level's stores: on gfx9 one counter (vmcnt) covers
+    // loads and stores in issue order, so a load issued after the stores
+    // would make the next level wait for all of them.  Masked neighbours (no
+    // U point) read 0 from an out-of-range offset, without a branch.
+    const size_t pbytes = n2 * sizeof(double);
+    const uint32_t oq = (uint32_t)q * 8u;
+    const uint32_t os = au_ok ? (uint32_t)(q + imt) * 8u : kBufOff;
+    const uint32_t ow = au_ok ? (uint32_t)(q - 1) * 8u : kBufOff;
+    const uint32_t osw = au_ok ? (uint32_t)(q + imt - 1) * 8u : kBufOff;
+    struct LvIn {
+        double uf, vf, un, vn, T, S, vt, vv, bT, bS, bTs, bTw, bTsw, bSs, bSw, bSsw;
+    };
+    auto load_level = [&](int k, LvIn &L) {
+        const size_t o = (size_t)k * n2;
+        L.uf = buf_load(plane_rsrc(f.utf + o, pbytes), oq, 0.0);
+        L.vf = buf_load(plane_rsrc(f.vtf + o, pbytes), oq, 0.0);
+        L.un = buf_load(plane_rsrc(f.u + o, pbytes), oq, 0.0);
+        L.vn = buf_load(plane_rsrc(f.v + o, pbytes), oq, 0.0);
+        L.T = buf_load(plane_rsrc(atT + o, pbytes), oq, 0.0);
+        L.S = buf_load(plane_rsrc(atS + o, pbytes), oq, 0.0);
+        L.vt = buf_load(plane_rsrc(f.vit + o, pbytes), oq, 0.0);
+        L.vv = buf_load(plane_rsrc(f.viv + o, pbytes), oq, 0.0);
+        const auto rT = plane_rsrc(abT + o, pbytes), rS = plane_rsrc(abS + o, pbytes);
+        L.bT = buf_load(rT, oq, 0.0);
+        L.bS = buf_load(rS, oq, 0.0);
+        L.bTs = buf_load(rT, os, 0.0);
+        L.bTw = buf_load(rT, ow, 0.0);
+        L.bTsw = buf_load(rT, osw, 0.0);
+        L.bSs = buf_load(rS, os, 0.0);
+        L.bSw = buf_load(rS, ow, 0.0);
+        L.bSsw = buf_load(rS, osw, 0.0);
+    };
     double Tp = 0.0, Sp = 0.0, bTp = 0.0, bSp = 0.0, duTp = 0.0, duSp = 0.0;
     double ppp = 0.0, ppap = 0.0, ggp = 0.0;
     double s0 = 0.0, s1 = 0.0;
+    LvIn cur, nxt;
+    load_level(0, cur);
     for (int k = 0; k < km; ++k) {
         const size_t c = (size_t)k * n2 + q;
+        if (k + 1 < km) load_level(k + 1, nxt);
         // readyt_cu_1: time levels
-        {
-            const double uf = f.utf[c], vf = f.vtf[c], un = f.u[c], vn = f.v[c];
-            f.utl[c] = uf;
-            f.vtl[c] = vf;
-            f.utf[c] = un;
-            f.vtf[c] = vn;
-        }
-        const double T = atT[c], Sl = atS[c], vt = f.vit[c], vv = f.viv[c];
-        const double bT = abT[c], bS = abS[c];
+        f.utl[c] = cur.uf;
+        f.vtl[c] = cur.vf;
+        f.utf[c] = cur.un;
+        f.vtf[c] = cur.vn;
+        const double T = cur.T, Sl = cur.S, vt = cur.vt, vv = cur.vv;
+        const double bT = cur.bT, bS = cur.bS;
         // T and S of atb on the U grid (readyt_cu_1's dlu / dlv, read by dens_cu)
         double duT = 0.0, duS = 0.0;
         if (au_ok) {
-            duT = a0 * bT + as * abT[c + imt] + aw * abT[c - 1] + asw * abT[c + imt - 1];
-            duS = a0 * bS + as * abS[c + imt] + aw * abS[c - 1] + asw * abS[c + imt - 1];
+            duT = a0 * bT + as * cur.bTs + aw * cur.bTw + asw * cur.bTsw;
+            duS = a0 * bS + as * cur.bSs + aw * cur.bSw + asw * cur.bSsw;
         }
         // readyt_cu_2: potential density and its weight in the pressure integral
         double pd = 0.0;
@@ -226,6 +259,7 @@ __global__ __launch_bounds__(256) void k_readyt_col(RGrid g, licom_readyt_fields
         ppp = pp;
         ppap = ppa;
         ggp = gg0;
+        cur = nxt;
     }
     f.dlu[q] = s0;
     f.dlu[n2 + q] = s1;
