@@ -700,9 +700,11 @@ def main():
     ap.add_argument("--serve-lanes", type=int, default=4,
                     help="streams of the solo policy")
     ap.add_argument("--hbm-cap-gb", type=float, default=200.0)
-    ap.add_argument("--hw-queues", type=int, default=0,
+    ap.add_argument("--hw-queues", type=int, default=16,
                     help="GPU_MAX_HW_QUEUES for this process (0: leave the environment's): the "
-                         "request loop's lanes and admission streams each want a queue of their own")
+                         "request loop's lanes and admission streams each want a queue of their own "
+                         "(with HIP's default of 4 an admission's copies wait behind other streams' "
+                         "kernels)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="write the HBM traffic summary from two rocprofv3 --pmc runs")
     ap.add_argument("--summarize-trace", metavar="TRACE_DIR",

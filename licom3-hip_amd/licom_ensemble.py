@@ -305,7 +305,13 @@ class RequestLoop:
     started yet, so a short request that arrives while long ones run starts
     at the next chunk boundary of a lane and is not slowed to a batch's step
     time.  A request stays on the lane it started on (its state is only
-    touched by that stream); chunks are queued two deep per lane."""
+    touched by that stream); chunks are queued two deep per lane.
+
+    Streams share the process's hardware queues (HIP's default is four,
+    GPU_MAX_HW_QUEUES): with more streams than queues an admission's copies
+    and fills wait in order behind other streams' kernels (measured: 40-270
+    ms instead of 2-20 ms per admission), so a serving process should set
+    GPU_MAX_HW_QUEUES (up to 32) before HIP starts; bench.py uses 16."""
 
     def __init__(self, hbm_cap_bytes=200e9, chunk_steps=8, tuner=None, keep_results=False,
                  class_cells=(5.0e5, 2.0e6), one_at_a_time=(), one_at_a_time_upto=None,
