@@ -436,3 +436,30 @@ def readyt_fields(case: Case, grid: dict, seed: int = 12, sentinel: float = 7.0)
     for k, sh in shapes.items():
         f[k] = np.full(sh, sentinel)
     return {k: np.ascontiguousarray(v, np.float64) for k, v in f.items()}
+
+
+# ------------------------------------------------- readyc_ (mixing part)
+def readyc_inputs(case: Case, rgrid: dict, rfields: dict, seed: int = 13,
+                  sentinel: float = 7.0) -> tuple:
+    """licom_readyc_grid and licom_readyc_fields over a tracer case and its
+    readyt_ block: U->T weights near 1/4 (perturbed, so that the weighting is
+    exercised), the T->U weights and odzt of the readyt_ grid, velocities with
+    vertical shear, rict / ric of the readyt_ fields (their sentinel where
+    readyt_ has not run), outputs starting at `sentinel`."""
+    rng = np.random.default_rng(seed)
+    km, jmt, imt = case.km, case.jmt, case.imt
+    s2 = (jmt, imt)
+    w = lambda: 0.25 * (1.0 + rng.uniform(-0.2, 0.2, s2))  # noqa: E731
+    grid = dict(at0=w(), atn=w(), ate=w(), atne=w(), au0=rgrid["au0"], aus=rgrid["aus"],
+                auw=rgrid["auw"], ausw=rgrid["ausw"], odzt=rgrid["odzt"])
+    st = case.state
+    shear = 1.0 + 0.3 * rng.normal(0.0, 1.0, (km, 1, 1))
+    up = st["utf"] * shear + rng.normal(0.0, 0.01, st["utf"].shape) * rfields["viv"]
+    vp = st["vtf"] * shear[::-1] + rng.normal(0.0, 0.01, st["vtf"].shape) * rfields["viv"]
+    f = dict(vit=rfields["vit"], viv=rfields["viv"], up=up * rfields["viv"], vp=vp * rfields["viv"],
+             h0=rng.normal(0.0, 0.3, s2), rict=rfields["rict"], ric=rfields["ric"],
+             h0bl=np.full(s2, sentinel), h0bf=rng.normal(0.0, 0.3, s2))
+    for k in ("wp12", "wp13", "riv1", "riv2", "s2t", "s2u", "riu"):
+        f[k] = np.full((km, jmt, imt), sentinel)
+    f["rit"] = np.full((km - 1, jmt, imt), sentinel)
+    return grid, {k: np.ascontiguousarray(v, np.float64) for k, v in f.items()}

@@ -420,3 +420,64 @@ def thermal(tt, ss, ppa, vit, od0):
                            ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
 
+
+
+# ------------------------------------------------------- readyc_ (mixing part)
+READYC_GRID = ("at0", "atn", "ate", "atne", "au0", "aus", "auw", "ausw")
+READYC_FIELDS = ("vit", "viv", "up", "vp", "h0", "rict", "ric", "h0bl", "h0bf",
+                 "wp12", "wp13", "riv1", "riv2", "s2t", "rit", "s2u", "riu")
+
+
+class OrcReadycGrid(ctypes.Structure):
+    _fields_ = ([("imt", ctypes.c_int), ("jmt", ctypes.c_int), ("km", ctypes.c_int)] +
+                [(k, _DP) for k in READYC_GRID] + [("odzt", _DP)])
+
+
+class OrcReadycFields(ctypes.Structure):
+    _fields_ = [(k, _DP) for k in READYC_FIELDS]
+
+
+def _readyc_lib():
+    L = lib()
+    if not getattr(L, "_readyc_bound", False):
+        L.orc_readyc_shear.argtypes = [ctypes.POINTER(OrcReadycGrid), ctypes.POINTER(OrcReadycFields)]
+        L.orc_readyc_akmu.argtypes = [ctypes.POINTER(OrcReadycGrid), _DP, _DP, _DP]
+        L._readyc_bound = True
+    return L
+
+
+def _readyc_grid(imt, jmt, km, grid, keep):
+    G = OrcReadycGrid()
+    G.imt, G.jmt, G.km = int(imt), int(jmt), int(km)
+    for k in READYC_GRID + ("odzt",):
+        a = np.ascontiguousarray(grid[k], np.float64)
+        keep.append(a)
+        setattr(G, k, _d(a))
+    return G
+
+
+def readyc_shear(imt, jmt, km, grid, fields):
+    """readyc1 + readyc1_2 with the clears before them (readyc.cpp:44-170,
+    1636-1643) on numpy arrays: a dict with a copy of every field after the
+    call."""
+    L = _readyc_lib()
+    keep = []
+    G = _readyc_grid(imt, jmt, km, grid, keep)
+    missing = [k for k in READYC_FIELDS if fields.get(k) is None]
+    if missing:
+        raise ValueError(f"readyc_shear: missing fields {missing}")
+    out = {k: np.array(fields[k], np.float64, order="C") for k in READYC_FIELDS}
+    L.orc_readyc_shear(ctypes.byref(G), ctypes.byref(OrcReadycFields(*[_d(out[k]) for k in READYC_FIELDS])))
+    return out
+
+
+def readyc_akmu(imt, jmt, km, grid, akmt, viv):
+    """readyc2 with the clear of akmu (readyc.cpp:223-246, 1679)."""
+    L = _readyc_lib()
+    keep = []
+    G = _readyc_grid(imt, jmt, km, grid, keep)
+    a = np.ascontiguousarray(akmt, np.float64)
+    v = np.ascontiguousarray(viv, np.float64)
+    out = np.full(a.shape, np.nan)
+    L.orc_readyc_akmu(ctypes.byref(G), _d(a), _d(v), _d(out))
+    return out

@@ -131,3 +131,26 @@ def test_readyt_create_validates_before_touching_a_device():
     assert L.licom_readyt_create(ctypes.byref(G), ctypes.byref(h)) == -1
     assert b"NULL" in L.licom_last_error()
     assert not h.value
+
+
+def test_readyc_declarations_are_exported():
+    import licom_readyc
+    names = declared("licom_readyc.h")
+    assert names
+    L = ctypes.CDLL(licom_readyc.lt.lib_path())
+    assert [n for n in names if not hasattr(L, n)] == []
+    assert sorted(licom_readyc.EXPORTED) == names
+
+
+def test_readyc_create_validates_before_touching_a_device():
+    import licom_readyc as lrc
+    L = lrc.lib()
+    G = lrc.ReadycGrid()
+    G.imt, G.jmt, G.km = 10, 10, 1
+    h = ctypes.c_void_p()
+    assert L.licom_readyc_create(ctypes.byref(G), ctypes.byref(h)) == -1
+    assert b"invalid extents" in L.licom_last_error()
+    G.km = 4
+    assert L.licom_readyc_create(ctypes.byref(G), ctypes.byref(h)) == -1
+    assert b"NULL" in L.licom_last_error()
+    assert not h.value
