@@ -374,6 +374,40 @@ def readyt_algorithmic_bytes(case):
     return n2 * case.km * 8 * (10 + 16) + n2 * (case.km - 1) * 8 * 5
 
 
+def run_readyc(args, torch, case, rgrid, rt_fields, st, model):
+    """readyc_'s shear pass (readyc1 + readyc1_2) on the readyt_ outputs and
+    the FP64 model's velocities: ms per call and its HBM roofline.
+    Algorithmic bytes: reads vit, viv, up, vp (km planes) and rict, ric (km-1
+    planes); writes wp12, wp13, riv1, riv2, s2t, s2u, riu (km planes) and rit
+    (km-1 planes), 8 bytes each."""
+    import licom_readyc as lrc
+    import licom_synth as ls
+    km, jmt, imt = case.km, case.jmt, case.imt
+    cg = ls.readyc_grid(case, rgrid)
+    f64 = dict(dtype=torch.float64, device=model.device)
+    fields = dict(rt_fields, up=st["utf"], vp=st["vtf"], h0=torch.zeros((jmt, imt), **f64),
+                  h0bl=torch.zeros((jmt, imt), **f64), h0bf=torch.zeros((jmt, imt), **f64))
+    rc = lrc.Readyc(imt, jmt, km, cg, fields, device=model.device, stream=model.stream)
+    rc.shear()
+    rc.sync()
+    ms = []
+    for _ in range(max(3, args.steps)):
+        rc.shear()
+        ms.append(rc.last_ms())
+    rc.close()
+    del rc, fields
+    torch.cuda.empty_cache()
+    avg = sum(ms) / len(ms)
+    n2 = imt * jmt
+    alg = n2 * km * 8 * (4 + 7) + n2 * (km - 1) * 8 * 3
+    ach = alg / (avg * 1e-3) / 1e9
+    print(f"[bench] readyc shear: {avg:.3f} ms per call ({ach:.0f} GB/s algorithmic)", file=sys.stderr,
+          flush=True)
+    return {"kernel": "k_readyc_shear", "ms_per_call": round(avg, 4), "calls": len(ms),
+            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes": alg}}
+
+
 def run_readyt(args, torch, lt, case, model):
     """readyt_ at the workload size on the FP64 model's at / atb, the other
     fields built on the device; its CPU baseline is the oracle on a band."""
@@ -405,12 +439,14 @@ def run_readyt(args, torch, lt, case, model):
         ms = r.last_ms()
         tot.append(ms[0])
         parts.append(ms[1:])
+    keep = {k: r.fields[k] for k in ("vit", "viv", "rict", "ric")}
     r.close()
     del r, f
     torch.cuda.empty_cache()
     avg = sum(tot) / len(tot)
     part = [sum(p[i] for p in parts) / len(parts) for i in range(3)]
     alg = readyt_algorithmic_bytes(case)
+    shear = run_readyc(args, torch, case, g, keep, st, model)
     ach = alg / (avg * 1e-3) / 1e9
     print(f"[bench] readyt: {avg:.3f} ms per call ({ach:.0f} GB/s algorithmic)", file=sys.stderr,
           flush=True)
@@ -421,7 +457,8 @@ def run_readyt(args, torch, lt, case, model):
            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                         "traffic": traffic_from_profiles("k_readyt_col", args.workload),
-                        "algorithmic_bytes": alg}}
+                        "algorithmic_bytes": alg},
+           "readyc_shear": shear}
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import orc

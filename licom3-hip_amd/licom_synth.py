@@ -439,19 +439,26 @@ def readyt_fields(case: Case, grid: dict, seed: int = 12, sentinel: float = 7.0)
 
 
 # ------------------------------------------------- readyc_ (mixing part)
+def readyc_grid(case: Case, rgrid: dict, seed: int = 13) -> dict:
+    """licom_readyc_grid of a tracer case's block: U->T weights near 1/4
+    (perturbed, so that the weighting is exercised), the T->U weights and
+    odzt of its readyt_ grid."""
+    rng = np.random.default_rng(seed)
+    s2 = (case.jmt, case.imt)
+    w = lambda: 0.25 * (1.0 + rng.uniform(-0.2, 0.2, s2))  # noqa: E731
+    return dict(at0=w(), atn=w(), ate=w(), atne=w(), au0=rgrid["au0"], aus=rgrid["aus"],
+                auw=rgrid["auw"], ausw=rgrid["ausw"], odzt=rgrid["odzt"])
+
+
 def readyc_inputs(case: Case, rgrid: dict, rfields: dict, seed: int = 13,
                   sentinel: float = 7.0) -> tuple:
-    """licom_readyc_grid and licom_readyc_fields over a tracer case and its
-    readyt_ block: U->T weights near 1/4 (perturbed, so that the weighting is
-    exercised), the T->U weights and odzt of the readyt_ grid, velocities with
-    vertical shear, rict / ric of the readyt_ fields (their sentinel where
-    readyt_ has not run), outputs starting at `sentinel`."""
-    rng = np.random.default_rng(seed)
+    """licom_readyc_grid (readyc_grid) and licom_readyc_fields over a tracer
+    case and its readyt_ block: velocities with vertical shear, rict / ric
+    and the masks of the readyt_ fields, outputs starting at `sentinel`."""
+    grid = readyc_grid(case, rgrid, seed)
+    rng = np.random.default_rng(seed + 1)
     km, jmt, imt = case.km, case.jmt, case.imt
     s2 = (jmt, imt)
-    w = lambda: 0.25 * (1.0 + rng.uniform(-0.2, 0.2, s2))  # noqa: E731
-    grid = dict(at0=w(), atn=w(), ate=w(), atne=w(), au0=rgrid["au0"], aus=rgrid["aus"],
-                auw=rgrid["auw"], ausw=rgrid["ausw"], odzt=rgrid["odzt"])
     st = case.state
     shear = 1.0 + 0.3 * rng.normal(0.0, 1.0, (km, 1, 1))
     up = st["utf"] * shear + rng.normal(0.0, 0.01, st["utf"].shape) * rfields["viv"]
