@@ -1,7 +1,6 @@
-// readyc.hip -- the vertical-mixing preparation of readyc_ on gfx950
-// (include/licom_readyc.h): velocity shear and Richardson numbers for the
-// Canuto producer (readyc1, readyc1_2), and AKMT averaged to U points after
-// it (readyc2).
+// readyc.hip -- readyc_ on gfx950 (include/licom_readyc.h): velocity shear
+// and Richardson numbers for the Canuto producer (readyc1, readyc1_2), AKMT
+// averaged to U points after it (readyc2), then the momentum tendencies.
 //
 // The reference clears six 3-D arrays and runs two 3-D launches, the first
 // writing up / vp averaged to T points (wp12, wp13) that the second reads
@@ -155,6 +154,508 @@ __global__ __launch_bounds__(256) void k_readyc_akmu(CGrid g, const double *__re
     akmu[e] = v;
 }
 
+
+// ---------------------------------------------------------------------------
+// Momentum tendencies after readyc2.  The reference runs up to twelve 3-D
+// launches over whole arrays, with eight 3-D scratch arrays between them.
+// Here:
+//   k_rmom_upwell   one lane per T column: the scaled velocities of the four
+//                   U cells around it, their divergence level by level (kept
+//                   in registers), its column integral, then the recursion of
+//                   ws (upwell_cu_0 + upwell_cu);
+//   k_rmom_divcurl  (del4) one lane per T column: divergence and vorticity of
+//                   (up, vp) per level (hdel4_div, hdel4_zcurl) into scratch;
+//   k_rmom_lap1     (del4) one lane per U column: the viscosity factor from
+//                   the gradients of both (hdel4_grad twice) and the first
+//                   Laplacian scaled by it (hdiffu_del4_0) into scratch;
+//   k_rmom_main     one lane per U column marching k once: w and the face
+//                   velocities at U points (readyc3) formed on the fly,
+//                   advection, the horizontal viscosity (hdiffu_del2_cu or
+//                   hdiffu_del4_1), the depth sums and the surface and bottom
+//                   stress (readyc4) and the explicit vertical viscosity
+//                   (readyc4_2), every output cell written once.
+// Each level's inputs are loaded a level ahead through plane buffers
+// (out-of-range offsets return zero for the lanes that do not use them).
+// Operation order follows the reference statements (-ffp-contract=off).
+struct MGrid {
+    int ib, ie, jb, je, del4;
+    double am, c0f, od0, cag, sag;
+    const int *kmt, *kmu;
+    const double *ohbu, *ohbt, *hts, *htw, *hue, *hun, *tarea_r, *uarea_r, *dxu, *dyu, *dxur, *dyur,
+        *snlat, *amf, *duc, *dum, *dun, *dus, *due, *duw, *dmc, *dmn, *dms, *dme, *dmw, *dzp, *odzp;
+    double *div, *curl, *d2u, *d2v;   // del4 scratch, [km][j][i]
+};
+
+__device__ __forceinline__ uint32_t boff(bool ok, size_t idx) { return ok ? (uint32_t)idx * 8u : kBufOff; }
+
+__global__ __launch_bounds__(256) void k_rmom_upwell(CGrid g, MGrid m, licom_readyc_mfields f) {
+    __shared__ double sDZ[LICOM_MAX_KM];
+    {
+        const int t = threadIdx.x + 64 * threadIdx.y;
+        for (int r = t; r < g.km; r += 256) sDZ[r] = m.dzp[r];
+        __syncthreads();
+    }
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int imt = g.imt, jmt = g.jmt, km = g.km;
+    // upwell_cu writes ws in 1 <= j < jmt-1, 1 <= i < imt-1 only
+    if (i < 1 || i > imt - 2 || j < 1 || j > jmt - 2) return;
+    const size_t n2 = g.n2, q = (size_t)j * imt + i;
+    // upwell_cu_0 (readyc.cpp:609-621): 1 + (h0 at the U point) / H there
+    auto scale = [&](size_t p) {
+        const double w = g.au0[p] * f.h0[p] + g.aus[p] * f.h0[p + imt] + g.auw[p] * f.h0[p - 1] +
+                         g.ausw[p] * f.h0[p + imt - 1];
+        return 1.0e0 + w * m.ohbu[p];
+    };
+    // the U cells (j, i+1), (j-1, i+1), (j, i), (j-1, i)
+    const size_t qa = q + 1, qb = q - imt + 1, qd = q - imt;
+    const double sa = scale(qa), sb = scale(qb), sc = scale(q), sd = scale(qd);
+    const double htwe = m.htw[q + 1], htwc = m.htw[q], htsc = m.hts[q], htsn = m.hts[q - imt];
+    const double tr = m.tarea_r[q], ohbt = m.ohbt[q], h0 = f.h0[q];
+    const int kmt = m.kmt[q];
+    const size_t pb = n2 * sizeof(double);
+    const uint32_t oa = (uint32_t)qa * 8u, ob = (uint32_t)qb * 8u, oc = (uint32_t)q * 8u,
+                   od = (uint32_t)qd * 8u;
+    struct Lv {
+        double ua, ub, uc, ud, va, vb, vc, vd, vt;
+    };
+    auto load = [&](int k, Lv &L) {
+        const size_t o = (size_t)k * n2;
+        const auto ru = plane_rsrc(f.u + o, pb), rv = plane_rsrc(f.v + o, pb);
+        L.ua = buf_load(ru, oa, 0.0);
+        L.ub = buf_load(ru, ob, 0.0);
+        L.uc = buf_load(ru, oc, 0.0);
+        L.ud = buf_load(ru, od, 0.0);
+        L.va = buf_load(rv, oa, 0.0);
+        L.vb = buf_load(rv, ob, 0.0);
+        L.vc = buf_load(rv, oc, 0.0);
+        L.vd = buf_load(rv, od, 0.0);
+        L.vt = buf_load(plane_rsrc(f.vit + o, pb), oc, 0.0);
+    };
+    // upwell_cu (readyc.cpp:636-671): divergence per level, its integral
+    double wk[LICOM_MAX_KM];
+    double work = 0.0e0;
+    uint64_t vone = 0;      // levels whose vit is exactly 1.0
+    bool v01 = true;        // every vit of the column is +0.0 or 1.0
+    Lv cur, nxt;
+    load(0, cur);
+#pragma unroll
+    for (int k = 0; k < LICOM_MAX_KM; ++k) {
+        if (k < km) {
+            if (k + 1 < km) load(k + 1, nxt);
+            double d = 0.0e0;
+            if (k <= kmt - 1) {
+                const double uka = sa * cur.ua, ukb = sb * cur.ub, ukc = sc * cur.uc, ukd = sd * cur.ud;
+                const double vka = sa * cur.va, vkb = sb * cur.vb, vkc = sc * cur.vc, vkd = sd * cur.vd;
+                d = 0.5 * ((uka + ukb) * htwe - (ukc + ukd) * htwc + (vka + vkc) * htsc - (vkb + vkd) * htsn) *
+                    tr;
+            }
+            wk[k] = d;
+            work -= sDZ[k] * d * cur.vt;
+            const uint64_t bits = __builtin_bit_cast(uint64_t, cur.vt);
+            if (bits == 0x3FF0000000000000ull)
+                vone |= 1ull << k;
+            else if (bits != 0)
+                v01 = false;
+            cur = nxt;
+        }
+    }
+    const double wo = work * ohbt;
+    const double r = 1.0e0 / (1.0e0 + h0 * ohbt);
+    double prev = f.ws[q];
+#pragma unroll
+    for (int k = 1; k < LICOM_MAX_KM; ++k) {
+        if (k < km) {
+            const size_t c = (size_t)k * n2 + q;
+            const double vt = v01 ? (((vone >> k) & 1ull) ? 1.0 : 0.0) : f.vit[c];
+            const double w = vt * (prev + sDZ[k - 1] * (wo + wk[k - 1]));
+            f.ws[c] = w * r;
+            prev = w;
+        }
+    }
+}
+
+// hdel4_div (readyc.cpp:372-402) and hdel4_zcurl (:343-370) of (up, vp)
+__global__ __launch_bounds__(256) void k_rmom_divcurl(CGrid g, MGrid m, licom_readyc_mfields f) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int imt = g.imt, jmt = g.jmt, km = g.km;
+    if (i >= imt || j >= jmt) return;
+    const size_t n2 = g.n2, q = (size_t)j * imt + i;
+    const bool dv = i < imt - 1 && j >= 1, cu = i >= 1 && j >= 1;
+    double htwe = 0, htwc = 0, htsc = 0, htsn = 0, tr = 0, dyc = 0, dyn = 0, dyw = 0, dynw = 0, dxc = 0,
+           dxn = 0, dxw = 0, dxnw = 0;
+    if (dv) {
+        htwe = m.htw[q + 1];
+        htwc = m.htw[q];
+        htsc = m.hts[q];
+        htsn = m.hts[q - imt];
+        tr = m.tarea_r[q];
+    }
+    if (cu) {
+        dyc = m.dyu[q];
+        dyn = m.dyu[q - imt];
+        dyw = m.dyu[q - 1];
+        dynw = m.dyu[q - imt - 1];
+        dxc = m.dxu[q];
+        dxn = m.dxu[q - imt];
+        dxw = m.dxu[q - 1];
+        dxnw = m.dxu[q - imt - 1];
+    }
+    const int kmt = m.kmt[q];
+    const size_t pb = n2 * sizeof(double);
+    const uint32_t oc = (uint32_t)q * 8u, oe = boff(dv, q + 1), one = boff(dv, q - imt + 1),
+                   on = boff(j >= 1, q - imt), ow = boff(cu, q - 1), onw = boff(cu, q - imt - 1);
+    struct Lv {
+        double u, ue, une, un, uw, unw, v, ve, vne, vn, vw, vnw;
+    };
+    auto load = [&](int k, Lv &L) {
+        const size_t o = (size_t)k * n2;
+        const auto ru = plane_rsrc(f.up + o, pb), rv = plane_rsrc(f.vp + o, pb);
+        L.u = buf_load(ru, oc, 0.0);
+        L.ue = buf_load(ru, oe, 0.0);
+        L.une = buf_load(ru, one, 0.0);
+        L.un = buf_load(ru, on, 0.0);
+        L.uw = buf_load(ru, ow, 0.0);
+        L.unw = buf_load(ru, onw, 0.0);
+        L.v = buf_load(rv, oc, 0.0);
+        L.ve = buf_load(rv, oe, 0.0);
+        L.vne = buf_load(rv, one, 0.0);
+        L.vn = buf_load(rv, on, 0.0);
+        L.vw = buf_load(rv, ow, 0.0);
+        L.vnw = buf_load(rv, onw, 0.0);
+    };
+    Lv cur, nxt;
+    load(0, cur);
+    for (int k = 0; k < km; ++k) {
+        if (k + 1 < km) load(k + 1, nxt);
+        double d = 0.0e0, z = 0.0e0;
+        if (dv && k <= kmt - 1)
+            d = 0.5 *
+                ((cur.ue + cur.une) * htwe - (cur.u + cur.un) * htwc + (cur.ve + cur.v) * htsc -
+                 (cur.vne + cur.vn) * htsn) *
+                tr;
+        if (cu && k <= kmt - 1)
+            z = 0.5 * ((cur.v * dyc + cur.vn * dyn - cur.vw * dyw - cur.vnw * dynw - cur.u * dxc - cur.uw * dxw +
+                        cur.un * dxn + cur.unw * dxnw));
+        const size_t c = (size_t)k * n2 + q;
+        m.div[c] = d;
+        m.curl[c] = z;
+        cur = nxt;
+    }
+}
+
+// the 5-point operator of hdiffu_*: the du coefficients on a, dm on b
+struct Lap {
+    double c, n, s, e, w, mc, mn, ms, me, mw;
+};
+__device__ __forceinline__ double lap_du(const Lap &L, double a, double an, double as, double ae, double aw) {
+    return L.c * a + L.n * an + L.s * as + L.e * ae + L.w * aw;
+}
+__device__ __forceinline__ double lap_dm(const Lap &L, double b, double bn, double bs, double be, double bw) {
+    return L.mc * b + L.mn * bn + L.ms * bs + L.me * be + L.mw * bw;
+}
+__device__ __forceinline__ Lap load_lap(const MGrid &m, size_t q) {
+    Lap L;
+    L.c = m.duc[q] + m.dum[q];   // (duc + dum), summed first as in the reference
+    L.n = m.dun[q];
+    L.s = m.dus[q];
+    L.e = m.due[q];
+    L.w = m.duw[q];
+    L.mc = m.dmc[q];
+    L.mn = m.dmn[q];
+    L.ms = m.dms[q];
+    L.me = m.dme[q];
+    L.mw = m.dmw[q];
+    return L;
+}
+
+// hdel4_grad twice (readyc.cpp:405-441) and hdiffu_del4_0 (:444-507)
+__global__ __launch_bounds__(256) void k_rmom_lap1(CGrid g, MGrid m, licom_readyc_mfields f) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int imt = g.imt, jmt = g.jmt, km = g.km;
+    if (i >= imt || j >= jmt) return;
+    const size_t n2 = g.n2, q = (size_t)j * imt + i;
+    // the reference compares 0-based j with POP's 1-based bounds as given
+    const bool reg = j >= m.jb - 2 && j <= m.je && i >= m.ib - 2 && i <= m.ie;
+    const double amf = m.amf[q];
+    const int kmu = m.kmu[q];
+    Lap L{};
+    double gxf = 0, gyf = 0, dxdy = 0, amd = 1.0;
+    if (reg) {
+        L = load_lap(m, q);
+        gxf = m.dxur[q] * 0.5;
+        gyf = m.dyur[q] * 0.5;
+        const double t_area = sqrt(1.0e0 / m.uarea_r[q]);
+        dxdy = t_area * t_area * t_area * t_area * t_area * 45.0e0;
+        amd = fabs(m.am * amf);
+    }
+    const size_t pb = n2 * sizeof(double);
+    const uint32_t oc = (uint32_t)q * 8u, os = boff(reg, q + imt), ow = boff(reg, q - 1),
+                   osw = boff(reg, q + imt - 1), on = boff(reg, q - imt), oe = boff(reg, q + 1);
+    struct Lv {
+        double dc, ds, dw, dsw, zc, zs, zw, zsw, u, un, us, ue, uw, v, vn, vs, ve, vw;
+    };
+    auto load = [&](int k, Lv &V) {
+        const size_t o = (size_t)k * n2;
+        const auto rd = plane_rsrc(m.div + o, pb), rz = plane_rsrc(m.curl + o, pb),
+                   ru = plane_rsrc(f.up + o, pb), rv = plane_rsrc(f.vp + o, pb);
+        V.dc = buf_load(rd, reg ? oc : kBufOff, 0.0);
+        V.ds = buf_load(rd, os, 0.0);
+        V.dw = buf_load(rd, ow, 0.0);
+        V.dsw = buf_load(rd, osw, 0.0);
+        V.zc = buf_load(rz, reg ? oc : kBufOff, 0.0);
+        V.zs = buf_load(rz, os, 0.0);
+        V.zw = buf_load(rz, ow, 0.0);
+        V.zsw = buf_load(rz, osw, 0.0);
+        V.u = buf_load(ru, reg ? oc : kBufOff, 0.0);
+        V.un = buf_load(ru, on, 0.0);
+        V.us = buf_load(ru, os, 0.0);
+        V.ue = buf_load(ru, oe, 0.0);
+        V.uw = buf_load(ru, ow, 0.0);
+        V.v = buf_load(rv, reg ? oc : kBufOff, 0.0);
+        V.vn = buf_load(rv, on, 0.0);
+        V.vs = buf_load(rv, os, 0.0);
+        V.ve = buf_load(rv, oe, 0.0);
+        V.vw = buf_load(rv, ow, 0.0);
+    };
+    Lv cur, nxt;
+    load(0, cur);
+    for (int k = 0; k < km; ++k) {
+        if (k + 1 < km) load(k + 1, nxt);
+        double am_f = 1.0e0, hu = 0.0, hv = 0.0;
+        if (reg && k <= kmu - 1) {
+            const double gx = gxf * (cur.ds - cur.dw - cur.dsw + cur.dc);
+            const double gy = gyf * (cur.ds - cur.dw + cur.dsw - cur.dc);
+            double af = 0.0 + (gx * gx + gy * gy);
+            const double gx2 = gxf * (cur.zs - cur.zw - cur.zsw + cur.zc);
+            const double gy2 = gyf * (cur.zs - cur.zw + cur.zsw - cur.zc);
+            af += gx2 * gx2 + gy2 * gy2;
+            am_f = sqrt(af) * dxdy / amd;
+            am_f = fmin(40.0e0, am_f);
+            am_f = fmax(1.0e0, am_f);
+            hu = lap_du(L, cur.u, cur.un, cur.us, cur.ue, cur.uw) + lap_dm(L, cur.v, cur.vn, cur.vs, cur.ve, cur.vw);
+            hv = lap_du(L, cur.v, cur.vn, cur.vs, cur.ve, cur.vw) - lap_dm(L, cur.u, cur.un, cur.us, cur.ue, cur.uw);
+        }
+        const size_t c = (size_t)k * n2 + q;
+        m.d2u[c] = am_f * amf * hu;
+        m.d2v[c] = am_f * amf * hv;
+        cur = nxt;
+    }
+}
+
+template <bool DEL4>
+__global__ __launch_bounds__(256, 2) void k_rmom_main(CGrid g, MGrid m, licom_readyc_mfields f) {
+    __shared__ double sDZ[LICOM_MAX_KM], sOZP[LICOM_MAX_KM], sOZT[LICOM_MAX_KM];
+    {
+        const int t = threadIdx.x + 64 * threadIdx.y;
+        for (int r = t; r < g.km; r += 256) {
+            sDZ[r] = m.dzp[r];
+            sOZP[r] = m.odzp[r];
+            sOZT[r] = g.odzt[r];
+        }
+        __syncthreads();
+    }
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int imt = g.imt, jmt = g.jmt, km = g.km;
+    if (i >= imt || j >= jmt) return;
+    const size_t n2 = g.n2, q = (size_t)j * imt + i;
+    const bool in = j >= 1 && j < jmt - 1 && i >= 1 && i < imt - 1;
+    // hdiffu_del2_cu / hdiffu_del4_1: 0-based j against POP's 1-based bounds
+    const bool hreg = j >= m.jb - 1 && j < m.je && i >= m.ib - 1 && i < m.ie;
+    const double ohbu = m.ohbu[q], c0f = m.c0f, cag = m.cag, sag = m.sag, am = m.am;
+    const int kmu = m.kmu[q];
+    double a0 = 0, as = 0, aw = 0, asw = 0, huew = 0, huec = 0, huns = 0, hunc = 0, ua = 0, sn = 0, su = 0,
+           sv = 0;
+    Lap L{};
+    if (in) {
+        a0 = g.au0[q];
+        as = g.aus[q];
+        aw = g.auw[q];
+        asw = g.ausw[q];
+        huew = m.hue[q - 1];
+        huec = m.hue[q];
+        huns = m.hun[q + imt];
+        hunc = m.hun[q];
+        ua = m.uarea_r[q];
+        sn = m.snlat[q];
+        su = f.su[q];
+        sv = f.sv[q];
+    }
+    if (hreg) L = load_lap(m, q);
+    const size_t pb = n2 * sizeof(double);
+    const uint32_t oc = (uint32_t)q * 8u, ow = boff(in, q - 1), oe = boff(in, q + 1), on = boff(in, q - imt),
+                   os = boff(in, q + imt), osw = boff(in, q + imt - 1), ocin = in ? oc : kBufOff;
+    const uint32_t hw = boff(hreg, q - 1), he = boff(hreg, q + 1), hn = boff(hreg, q - imt),
+                   hs = boff(hreg, q + imt), hc = hreg ? oc : kBufOff;
+    const double *ha = DEL4 ? m.d2u : f.up, *hb = DEL4 ? m.d2v : f.vp;
+    struct Lv {
+        double u, uw, ue, un, us, v, vw, ve, vn, vs;   // u, v around the U point
+        double wc, ws, ww, wsw;                         // ws of the four T points around it
+        double a, an, as, ae, aw, b, bn, bs, be, bw;     // the viscosity operand (up, vp or d2u, d2v)
+        double up, vp, vv, ak;
+    };
+    auto load = [&](int k, Lv &V) {
+        const size_t o = (size_t)k * n2;
+        const auto ru = plane_rsrc(f.u + o, pb), rv = plane_rsrc(f.v + o, pb), rw = plane_rsrc(f.ws + o, pb),
+                   ra = plane_rsrc(ha + o, pb), rb = plane_rsrc(hb + o, pb);
+        V.u = buf_load(ru, ocin, 0.0);
+        V.uw = buf_load(ru, ow, 0.0);
+        V.ue = buf_load(ru, oe, 0.0);
+        V.un = buf_load(ru, on, 0.0);
+        V.us = buf_load(ru, os, 0.0);
+        V.v = buf_load(rv, ocin, 0.0);
+        V.vw = buf_load(rv, ow, 0.0);
+        V.ve = buf_load(rv, oe, 0.0);
+        V.vn = buf_load(rv, on, 0.0);
+        V.vs = buf_load(rv, os, 0.0);
+        V.wc = buf_load(rw, ocin, 0.0);
+        V.ws = buf_load(rw, os, 0.0);
+        V.ww = buf_load(rw, ow, 0.0);
+        V.wsw = buf_load(rw, osw, 0.0);
+        V.an = buf_load(ra, hn, 0.0);
+        V.as = buf_load(ra, hs, 0.0);
+        V.ae = buf_load(ra, he, 0.0);
+        V.aw = buf_load(ra, hw, 0.0);
+        V.bn = buf_load(rb, hn, 0.0);
+        V.bs = buf_load(rb, hs, 0.0);
+        V.be = buf_load(rb, he, 0.0);
+        V.bw = buf_load(rb, hw, 0.0);
+        if (DEL4) {
+            V.a = buf_load(ra, hc, 0.0);
+            V.b = buf_load(rb, hc, 0.0);
+            V.up = buf_load(plane_rsrc(f.up + o, pb), oc, 0.0);
+            V.vp = buf_load(plane_rsrc(f.vp + o, pb), oc, 0.0);
+        } else {
+            V.up = buf_load(ra, oc, 0.0);
+            V.vp = buf_load(rb, oc, 0.0);
+            V.a = V.up;
+            V.b = V.vp;
+        }
+        V.vv = buf_load(plane_rsrc(f.viv + o, pb), oc, 0.0);
+        V.ak = buf_load(plane_rsrc(f.akmu + o, pb), ocin, 0.0);
+    };
+    constexpr double p25 = 0.25, p5 = 0.5, aid = 1 - 0.5;   // (1 - aidif)
+    Lv cur, nxt{};
+    load(0, cur);
+    // w at the U point (readyc3), level 0
+    double wu = a0 * (cur.wc) + as * (cur.ws) + aw * (cur.ww) + asw * (cur.wsw);
+    double uprv = 0, vprv = 0, upprv = 0, vpprv = 0, akprv = 0, wkprv = 0;
+    double dlub = 0.0, dlvb = 0.0, wkb = 0.0, ub = 0.0, vb = 0.0;
+    for (int k = 0; k < km; ++k) {
+        const bool more = k + 1 < km;
+        if (more) load(k + 1, nxt);
+        const size_t c = (size_t)k * n2 + q;
+        const double wkc = c0f * sqrt(cur.up * cur.up + cur.vp * cur.vp);   // hdiffu_*'s wka
+        double dlu = 0.0, dlv = 0.0;
+        double wun = 0.0;
+        if (in) {
+            // readyc3 (readyc.cpp:258-269) and advection_momentum_cu (:694-716)
+            if (more) wun = a0 * (nxt.wc) + as * (nxt.ws) + aw * (nxt.ww) + asw * (nxt.wsw);
+            const double uwi = (cur.uw + cur.u) * p25 * huew, uwe = (cur.u + cur.ue) * p25 * huec;
+            const double vsj = (cur.v + cur.vs) * p25 * huns, vsn = (cur.vn + cur.v) * p25 * hunc;
+            dlu = (-uwi * (cur.u - cur.uw) - uwe * (cur.ue - cur.u) - vsj * (cur.us - cur.u) -
+                   vsn * (cur.u - cur.un)) *
+                  ua;
+            dlv = (-uwi * (cur.v - cur.vw) - uwe * (cur.ve - cur.v) - vsj * (cur.vs - cur.v) -
+                   vsn * (cur.v - cur.vn)) *
+                  ua;
+            double z1 = 0.0e0, z2 = 0.0e0, z3 = 0.0e0, z4 = 0.0e0;
+            if (k > 0) {
+                z1 = wu * (uprv - cur.u);
+                z3 = wu * (vprv - cur.v);
+            }
+            if (k < km - 1) {
+                z2 = wun * (cur.u - nxt.u);
+                z4 = wun * (cur.v - nxt.v);
+            }
+            dlu += -p5 * sOZP[k] * (z1 + z2);
+            dlv += -p5 * sOZP[k] * (z3 + z4);
+            // hdiffu_del2_cu (:285-335) / hdiffu_del4_1 (:521-559)
+            double hu = 0.0, hv = 0.0;
+            if (hreg) {
+                if (DEL4) {
+                    hu = am * (lap_du(L, cur.a, cur.an, cur.as, cur.ae, cur.aw) +
+                               lap_dm(L, cur.b, cur.bn, cur.bs, cur.be, cur.bw));
+                    hv = am * (lap_du(L, cur.b, cur.bn, cur.bs, cur.be, cur.bw) -
+                               lap_dm(L, cur.a, cur.an, cur.as, cur.ae, cur.aw));
+                } else {
+                    hu = am *
+                         (lap_du(L, cur.a, cur.an, cur.as, cur.ae, cur.aw) +
+                          lap_dm(L, cur.b, cur.bn, cur.bs, cur.be, cur.bw)) *
+                         cur.vv;
+                    hv = am *
+                         (lap_du(L, cur.b, cur.bn, cur.bs, cur.be, cur.bw) -
+                          lap_dm(L, cur.a, cur.an, cur.as, cur.ae, cur.aw)) *
+                         cur.vv;
+                }
+            }
+            if (k > kmu - 1) hu = hv = 0.0;
+            dlv = dlv + hv;
+            dlu = dlu + hu;
+        }
+        // readyc4's depth sums (:730-743) see the tendencies before readyc4_2
+        dlub += sDZ[k] * ohbu * dlu * cur.vv;
+        dlvb += sDZ[k] * ohbu * dlv * cur.vv;
+        if (k == kmu - 1) {
+            wkb = wkc;
+            ub = cur.up;
+            vb = cur.vp;
+        }
+        if (in) {
+            // readyc4_2 (:803-874)
+            double du1, du2, dv1, dv2;
+            if (k == 0) {
+                dv1 = sv * m.od0 * aid;
+                du1 = su * m.od0 * aid;
+            } else {
+                dv1 = akprv * aid * (vpprv - cur.vp) * sOZT[k] * cur.vv +
+                      (1.0e0 - cur.vv) * wkprv * aid * (-sn * upprv * sag + vpprv * cag);
+                du1 = akprv * aid * (upprv - cur.up) * sOZT[k] * cur.vv +
+                      (1.0e0 - cur.vv) * wkprv * aid * (upprv * cag + sn * vpprv * sag);
+            }
+            if (k == km - 1) {
+                dv2 = wkc * (-sn * cur.up * sag + cur.vp * cag) * aid;
+                du2 = wkc * (cur.up * cag + sn * cur.vp * sag) * aid;
+            } else {
+                dv2 = cur.ak * aid * (cur.vp - nxt.vp) * sOZT[k + 1] * nxt.vv +
+                      (1.0e0 - nxt.vv) * wkc * aid * (-sn * cur.up * sag + cur.vp * cag);
+                du2 = cur.ak * aid * (cur.up - nxt.up) * sOZT[k + 1] * nxt.vv +
+                      (1.0e0 - nxt.vv) * wkc * aid * (cur.up * cag + sn * cur.vp * sag);
+            }
+            dlv = dlv + sOZP[k] * (dv1 - dv2);
+            dlu = dlu + sOZP[k] * (du1 - du2);
+        }
+        f.dlu[c] = dlu;
+        f.dlv[c] = dlv;
+        f.wka[c] = wkc;
+        uprv = cur.u;
+        vprv = cur.v;
+        upprv = cur.up;
+        vpprv = cur.vp;
+        akprv = cur.ak;
+        wkprv = wkc;
+        wu = wun;
+        cur = nxt;
+    }
+    // readyc4 (:745-785): surface stress, bottom drag at the deepest wet level
+    if (in) {
+        double sbcx = 0.0e0, sbcy = 0.0e0, bbcx = 0.0e0, bbcy = 0.0e0;
+        if (kmu - 1 >= 0) {
+            sbcx = su * m.od0;
+            sbcy = sv * m.od0;
+            bbcx = wkb * (ub * cag + sn * vb * sag);
+            bbcy = wkb * (-sn * ub * sag + vb * cag);
+        }
+        f.sbcx[q] = sbcx;
+        f.sbcy[q] = sbcy;
+        f.bbcx[q] = bbcx;
+        f.bbcy[q] = bbcy;
+        dlub = dlub + (sbcx - bbcx) * ohbu;
+        dlvb = dlvb + (sbcy - bbcy) * ohbu;
+    }
+    f.dlub[q] = dlub;
+    f.dlvb[q] = dlvb;
+}
+
 }  // namespace
 }  // namespace licom
 
@@ -166,6 +667,9 @@ struct licom_readyc {
     void *mem = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool ran = false;
+    MGrid mg{};             // momentum grid (licom_readyc_momentum_grid)
+    void *mmem = nullptr, *mscratch = nullptr;
+    bool mom = false;
 };
 
 extern "C" {
@@ -273,6 +777,130 @@ int licom_readyc_akmu(licom_readyc *r, const double *akmt, const double *viv, do
     return hip_status(hipGetLastError(), "k_readyc_akmu launch");
 }
 
+int licom_readyc_momentum_grid(licom_readyc *r, const licom_readyc_mgrid *mg) {
+    if (!r || !mg) {
+        set_error("licom_readyc_momentum_grid: NULL argument");
+        return LICOM_EINVAL;
+    }
+    const licom_readyc_mgrid &G = *mg;
+    const CGrid &g = r->g;
+    const int imt = g.imt, jmt = g.jmt, km = g.km;
+    // the reference's stencils stay inside the block for these bounds
+    const int lo = G.del4 ? 3 : 2, hij = G.del4 ? jmt - 2 : jmt - 1, hii = G.del4 ? imt - 2 : imt - 1;
+    if (G.jb < lo || G.je > hij || G.ib < lo || G.ie > hii) {
+        set_error("licom_readyc_momentum_grid: block bounds ib=%d ie=%d jb=%d je=%d outside %d..%d x %d..%d "
+                  "(%s)", G.ib, G.ie, G.jb, G.je, lo, hii, lo, hij, G.del4 ? "del4" : "del2");
+        return LICOM_EINVAL;
+    }
+    const double *src[] = {G.ohbu, G.ohbt, G.hts, G.htw, G.hue, G.hun, G.tarea_r, G.uarea_r, G.dxu,
+                           G.dyu,  G.dxur, G.dyur, G.snlat, G.amf, G.duc, G.dum, G.dun, G.dus,
+                           G.due,  G.duw, G.dmc, G.dmn, G.dms, G.dme, G.dmw};
+    constexpr int n2d = sizeof(src) / sizeof(src[0]);
+    for (const double *p : src)
+        if (!p) {
+            set_error("licom_readyc_momentum_grid: a grid array is NULL");
+            return LICOM_EINVAL;
+        }
+    if (!G.kmt || !G.kmu || !G.dzp || !G.odzp) {
+        set_error("licom_readyc_momentum_grid: a grid array is NULL");
+        return LICOM_EINVAL;
+    }
+    const size_t n2 = g.n2;
+    for (size_t p = 0; p < n2; ++p)
+        if (G.kmt[p] < 0 || G.kmt[p] > km || G.kmu[p] < 0 || G.kmu[p] > km) {
+            set_error("licom_readyc_momentum_grid: kmt or kmu outside 0..%d at cell %zu", km, p);
+            return LICOM_EINVAL;
+        }
+    if (r->mmem) (void)hipFree(r->mmem);
+    if (r->mscratch) (void)hipFree(r->mscratch);
+    r->mmem = r->mscratch = nullptr;
+    r->mom = false;
+    // doubles: the 2-D arrays, dzp, odzp; then the two int arrays
+    const size_t nd = n2d * n2 + 2 * (size_t)km, bytes = nd * sizeof(double) + 2 * n2 * sizeof(int);
+    std::vector<double> h(nd);
+    for (int a = 0; a < n2d; ++a) std::copy(src[a], src[a] + n2, h.begin() + a * n2);
+    std::copy(G.dzp, G.dzp + km, h.begin() + n2d * n2);
+    std::copy(G.odzp, G.odzp + km, h.begin() + n2d * n2 + km);
+    int rc = hip_status(hipMalloc(&r->mmem, bytes), "hipMalloc");
+    char *base = static_cast<char *>(r->mmem);
+    if (!rc) rc = hip_status(hipMemcpy(base, h.data(), nd * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy");
+    int *ik = reinterpret_cast<int *>(base + nd * sizeof(double));
+    if (!rc) rc = hip_status(hipMemcpy(ik, G.kmt, n2 * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = hip_status(hipMemcpy(ik + n2, G.kmu, n2 * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc && G.del4) rc = hip_status(hipMalloc(&r->mscratch, 4 * g.n3 * sizeof(double)), "hipMalloc");
+    if (rc) {
+        if (r->mmem) (void)hipFree(r->mmem);
+        if (r->mscratch) (void)hipFree(r->mscratch);
+        r->mmem = r->mscratch = nullptr;
+        return rc;
+    }
+    const double *d = reinterpret_cast<const double *>(base);
+    MGrid &m = r->mg;
+    m.ib = G.ib;
+    m.ie = G.ie;
+    m.jb = G.jb;
+    m.je = G.je;
+    m.del4 = G.del4 ? 1 : 0;
+    m.am = G.am;
+    m.c0f = G.c0f;
+    m.od0 = G.od0;
+    m.cag = G.cag;
+    m.sag = G.sag;
+    const double **dst[] = {&m.ohbu, &m.ohbt, &m.hts, &m.htw, &m.hue, &m.hun, &m.tarea_r, &m.uarea_r, &m.dxu,
+                            &m.dyu,  &m.dxur, &m.dyur, &m.snlat, &m.amf, &m.duc, &m.dum, &m.dun, &m.dus,
+                            &m.due,  &m.duw, &m.dmc, &m.dmn, &m.dms, &m.dme, &m.dmw};
+    for (int a = 0; a < n2d; ++a) *dst[a] = d + a * n2;
+    m.dzp = d + n2d * n2;
+    m.odzp = d + n2d * n2 + km;
+    m.kmt = ik;
+    m.kmu = ik + n2;
+    double *sc = static_cast<double *>(r->mscratch);
+    m.div = sc;
+    m.curl = sc ? sc + g.n3 : nullptr;
+    m.d2u = sc ? sc + 2 * g.n3 : nullptr;
+    m.d2v = sc ? sc + 3 * g.n3 : nullptr;
+    r->mom = true;
+    return LICOM_OK;
+}
+
+int licom_readyc_momentum(licom_readyc *r, const licom_readyc_mfields *f) {
+    if (!r || !f) {
+        set_error("licom_readyc_momentum: NULL argument");
+        return LICOM_EINVAL;
+    }
+    if (!r->mom) {
+        set_error("licom_readyc_momentum: no momentum grid (call licom_readyc_momentum_grid first)");
+        return LICOM_EINVAL;
+    }
+    const void *need[] = {f->u,   f->v,   f->up,   f->vp,   f->h0,   f->su,   f->sv,   f->vit,  f->viv,
+                          f->akmu, f->ws, f->dlu, f->dlv, f->wka, f->dlub, f->dlvb, f->sbcx, f->sbcy,
+                          f->bbcx, f->bbcy};
+    for (const void *p : need)
+        if (!p) {
+            set_error("licom_readyc_momentum: a field pointer is NULL");
+            return LICOM_EINVAL;
+        }
+    const CGrid &g = r->g;
+    const MGrid &m = r->mg;
+    const dim3 blk(64, 4), grd((g.imt + 63) / 64, (g.jmt + 3) / 4);
+    if (int rc = hip_status(hipEventRecord(r->ev0, r->stream), "hipEventRecord")) return rc;
+    k_rmom_upwell<<<grd, blk, 0, r->stream>>>(g, m, *f);
+    if (int rc = hip_status(hipGetLastError(), "k_rmom_upwell launch")) return rc;
+    if (m.del4) {
+        k_rmom_divcurl<<<grd, blk, 0, r->stream>>>(g, m, *f);
+        if (int rc = hip_status(hipGetLastError(), "k_rmom_divcurl launch")) return rc;
+        k_rmom_lap1<<<grd, blk, 0, r->stream>>>(g, m, *f);
+        if (int rc = hip_status(hipGetLastError(), "k_rmom_lap1 launch")) return rc;
+        k_rmom_main<true><<<grd, blk, 0, r->stream>>>(g, m, *f);
+    } else {
+        k_rmom_main<false><<<grd, blk, 0, r->stream>>>(g, m, *f);
+    }
+    if (int rc = hip_status(hipGetLastError(), "k_rmom_main launch")) return rc;
+    if (int rc = hip_status(hipEventRecord(r->ev1, r->stream), "hipEventRecord")) return rc;
+    r->ran = true;
+    return LICOM_OK;
+}
+
 int licom_readyc_sync(licom_readyc *r) {
     if (!r) {
         set_error("licom_readyc_sync: NULL context");
@@ -299,6 +927,8 @@ void licom_readyc_destroy(licom_readyc *r) {
     if (!r) return;
     if (r->ran) (void)hipStreamSynchronize(r->stream);
     if (r->mem) (void)hipFree(r->mem);
+    if (r->mmem) (void)hipFree(r->mmem);
+    if (r->mscratch) (void)hipFree(r->mscratch);
     if (r->ev0) (void)hipEventDestroy(r->ev0);
     if (r->ev1) (void)hipEventDestroy(r->ev1);
     delete r;

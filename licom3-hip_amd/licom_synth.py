@@ -470,3 +470,64 @@ def readyc_inputs(case: Case, rgrid: dict, rfields: dict, seed: int = 13,
         f[k] = np.full((km, jmt, imt), sentinel)
     f["rit"] = np.full((km - 1, jmt, imt), sentinel)
     return grid, {k: np.ascontiguousarray(v, np.float64) for k, v in f.items()}
+
+
+# ---------------------------------------------- readyc_ (momentum part)
+def readyc_mgrid(case: Case, rgrid: dict, del4: bool = False, seed: int = 15) -> dict:
+    """licom_readyc_mgrid of a tracer case's block: POP block bounds of a
+    two-cell halo (ib = jb = 3, ie = imt-2, je = jmt-2, 1-based), the case's
+    kmt and metrics with the readyt_ grid's kmu, ohbu, ohbt, dzp, a
+    Cartesian-like momentum Laplacian (du* about 1/dx^2 around a negative
+    centre) with small metric and rotation terms (dum, dm*), a del4 factor
+    amf spread over three decades so that the am_f limits 1 and 40 and the
+    range between are all met, hemisphere signs of both kinds, and a
+    turning angle of 10 degrees."""
+    rng = np.random.default_rng(seed)
+    km, jmt, imt = case.km, case.jmt, case.imt
+    g = case.grid
+    s2 = (jmt, imt)
+    dxu, dyu = g["dxu"], g["dyu"]
+    uarea = dxu * dyu
+    pert = lambda: 1.0 + rng.uniform(-0.1, 0.1, s2)  # noqa: E731
+    due, duw = pert() / (dxu * dxu), pert() / (dxu * dxu)
+    dun, dus = pert() / (dyu * dyu), pert() / (dyu * dyu)
+    duc = -(due + duw + dun + dus)
+    small = lambda s: s * rng.normal(0.0, 1.0, s2) / (dxu * dyu)  # noqa: E731
+    dum = small(1.0e-3)
+    dmc, dmn, dms, dme, dmw = (small(1.0e-2) for _ in range(5))
+    dzp = rgrid["dzp"]
+    theta = np.deg2rad(10.0)
+    jj = np.arange(jmt)[:, None] * np.ones((1, imt))
+    return dict(ib=3, ie=imt - 2, jb=3, je=jmt - 2, del4=int(bool(del4)),
+                am=-1.0e14 if del4 else 2.0e4, c0f=2.6e-3, od0=float(case.params["od0"]),
+                cag=float(np.cos(theta)), sag=float(np.sin(theta)),
+                kmt=g["kmt"].astype(np.int32), kmu=rgrid["kmu"].astype(np.int32),
+                ohbu=rgrid["ohbu"], ohbt=rgrid["ohbt"], hts=g["hts"], htw=g["htw"], hue=g["hue"],
+                hun=g["hun"], tarea_r=1.0 / g["tarea"], uarea_r=1.0 / uarea, dxu=dxu, dyu=dyu,
+                dxur=rgrid["dxur"], dyur=rgrid["dyur"],
+                snlat=np.where(jj < jmt // 3, -1.0, 1.0), amf=10.0 ** rng.uniform(-2.0, 1.0, s2),
+                duc=duc, dum=dum, dun=dun, dus=dus, due=due, duw=duw,
+                dmc=dmc, dmn=dmn, dms=dms, dme=dme, dmw=dmw, dzp=dzp, odzp=1.0 / dzp)
+
+
+def readyc_minputs(case: Case, rfields: dict, cfields: dict, akmu: np.ndarray, seed: int = 16,
+                   sentinel: float = 7.0) -> dict:
+    """licom_readyc_mfields over a tracer case, its readyt_ fields and its
+    readyc_ mixing inputs: current velocities (the previous ones of the
+    mixing pass perturbed by 5%), wind stress, AKMU, ws with a random
+    surface level, outputs starting at `sentinel`."""
+    rng = np.random.default_rng(seed)
+    km, jmt, imt = case.km, case.jmt, case.imt
+    s2, s3 = (jmt, imt), (km, jmt, imt)
+    viv = rfields["viv"]
+    up, vp = cfields["up"], cfields["vp"]
+    f = dict(u=up * (1.0 + rng.normal(0.0, 0.05, s3)) * viv,
+             v=vp * (1.0 + rng.normal(0.0, 0.05, s3)) * viv,
+             up=up, vp=vp, h0=cfields["h0"], su=rng.normal(0.0, 0.1, s2), sv=rng.normal(0.0, 0.1, s2),
+             vit=rfields["vit"], viv=viv, akmu=akmu, ws=np.full(s3, sentinel))
+    f["ws"][0] = rng.normal(0.0, 1.0e-6, s2)
+    for k in ("dlu", "dlv", "wka"):
+        f[k] = np.full(s3, sentinel)
+    for k in ("dlub", "dlvb", "sbcx", "sbcy", "bbcx", "bbcy"):
+        f[k] = np.full(s2, sentinel)
+    return {k: np.ascontiguousarray(v, np.float64) for k, v in f.items()}

@@ -481,3 +481,82 @@ def readyc_akmu(imt, jmt, km, grid, akmt, viv):
     out = np.full(a.shape, np.nan)
     L.orc_readyc_akmu(ctypes.byref(G), _d(a), _d(v), _d(out))
     return out
+
+
+# ------------------------------------------------- readyc_ (momentum part)
+READYC_MGRID_SCALARS = ("am", "c0f", "od0", "cag", "sag")
+READYC_MGRID_2D = ("ohbu", "ohbt", "hts", "htw", "hue", "hun", "tarea_r", "uarea_r", "dxu", "dyu",
+                   "dxur", "dyur", "snlat", "amf", "duc", "dum", "dun", "dus", "due", "duw", "dmc",
+                   "dmn", "dms", "dme", "dmw")
+READYC_MFIELDS_IN = ("u", "v", "up", "vp", "h0", "su", "sv", "vit", "viv", "akmu")
+READYC_MFIELDS_OUT = ("ws", "dlu", "dlv", "wka", "dlub", "dlvb", "sbcx", "sbcy", "bbcx", "bbcy")
+
+
+class OrcReadycMgrid(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_int) for k in ("ib", "ie", "jb", "je", "del4")] +
+                [(k, ctypes.c_double) for k in READYC_MGRID_SCALARS] +
+                [("kmt", _IP), ("kmu", _IP)] + [(k, _DP) for k in READYC_MGRID_2D] +
+                [("dzp", _DP), ("odzp", _DP)])
+
+
+class OrcReadycMfields(ctypes.Structure):
+    _fields_ = [(k, _DP) for k in READYC_MFIELDS_IN + READYC_MFIELDS_OUT]
+
+
+READYC_MOMENTUM_PIECES = ("upwell", "advect", "hdiff", "stress", "vdiff")
+
+
+def _readyc_mlib():
+    L = _readyc_lib()
+    if not getattr(L, "_readyc_mbound", False):
+        for name in ["orc_readyc_momentum"] + ["orc_rm_" + p for p in READYC_MOMENTUM_PIECES]:
+            getattr(L, name).argtypes = [ctypes.POINTER(OrcReadycGrid), ctypes.POINTER(OrcReadycMgrid),
+                                         ctypes.POINTER(OrcReadycMfields)]
+        L._readyc_mbound = True
+    return L
+
+
+def readyc_momentum(imt, jmt, km, grid, mgrid, fields, pieces=None):
+    """The momentum tendencies of readyc_ after readyc2 (readyc.cpp:1700-1793)
+    on numpy arrays: ``grid`` as for readyc_shear, ``mgrid`` the momentum
+    grid (block bounds ib, ie, jb, je 1-based, del4, the scalars, kmt, kmu,
+    the 2-D metrics, dzp, odzp), ``fields`` the inputs and the initial
+    outputs (ws is read at level 0).  ``pieces`` runs only the named
+    reference kernels in launch order (upwell, advect, hdiff, stress, vdiff)
+    on the fields given.  Returns a dict with a copy of every output."""
+    L = _readyc_mlib()
+    keep = []
+    G = _readyc_grid(imt, jmt, km, grid, keep)
+    M = OrcReadycMgrid()
+    for k in ("ib", "ie", "jb", "je", "del4"):
+        setattr(M, k, int(mgrid[k]))
+    for k in READYC_MGRID_SCALARS:
+        setattr(M, k, float(mgrid[k]))
+    for k in ("kmt", "kmu"):
+        a = np.ascontiguousarray(mgrid[k], np.int32)
+        keep.append(a)
+        setattr(M, k, _i(a))
+    for k in READYC_MGRID_2D + ("dzp", "odzp"):
+        a = np.ascontiguousarray(mgrid[k], np.float64)
+        keep.append(a)
+        setattr(M, k, _d(a))
+    n2, n3 = (jmt, imt), (km, jmt, imt)
+    shapes = {k: n2 for k in ("h0", "su", "sv", "dlub", "dlvb", "sbcx", "sbcy", "bbcx", "bbcy")}
+    ins = {}
+    for k in READYC_MFIELDS_IN:
+        a = np.ascontiguousarray(fields[k], np.float64)
+        if a.shape != shapes.get(k, n3):
+            raise ValueError(f"readyc_momentum: {k} has shape {a.shape}")
+        keep.append(a)
+        ins[k] = a
+    out = {}
+    for k in READYC_MFIELDS_OUT:
+        v = fields.get(k)
+        out[k] = (np.zeros(shapes.get(k, n3)) if v is None else np.array(v, np.float64, order="C"))
+    F = OrcReadycMfields(*([_d(ins[k]) for k in READYC_MFIELDS_IN] + [_d(out[k]) for k in READYC_MFIELDS_OUT]))
+    if pieces is None:
+        L.orc_readyc_momentum(ctypes.byref(G), ctypes.byref(M), ctypes.byref(F))
+    else:
+        for p in pieces:
+            getattr(L, "orc_rm_" + p)(ctypes.byref(G), ctypes.byref(M), ctypes.byref(F))
+    return out
