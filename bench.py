@@ -376,7 +376,8 @@ def readyt_algorithmic_bytes(case):
 
 def run_readyc(args, torch, case, rgrid, rt_fields, st, model):
     """readyc_'s shear pass (readyc1 + readyc1_2) on the readyt_ outputs and
-    the FP64 model's velocities: ms per call and its HBM roofline.
+    the FP64 model's velocities, then its momentum tendencies with each
+    viscosity (del4, del2): ms per call and their HBM rooflines.
     Algorithmic bytes: reads vit, viv, up, vp (km planes) and rict, ric (km-1
     planes); writes wp12, wp13, riv1, riv2, s2t, s2u, riu (km planes) and rit
     (km-1 planes), 8 bytes each."""
@@ -394,18 +395,56 @@ def run_readyc(args, torch, case, rgrid, rt_fields, st, model):
     for _ in range(max(3, args.steps)):
         rc.shear()
         ms.append(rc.last_ms())
-    rc.close()
-    del rc, fields
-    torch.cuda.empty_cache()
     avg = sum(ms) / len(ms)
     n2 = imt * jmt
     alg = n2 * km * 8 * (4 + 7) + n2 * (km - 1) * 8 * 3
     ach = alg / (avg * 1e-3) / 1e9
     print(f"[bench] readyc shear: {avg:.3f} ms per call ({ach:.0f} GB/s algorithmic)", file=sys.stderr,
           flush=True)
-    return {"kernel": "k_readyc_shear", "ms_per_call": round(avg, 4), "calls": len(ms),
-            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes": alg}}
+    out = {"kernel": "k_readyc_shear", "ms_per_call": round(avg, 4), "calls": len(ms),
+           "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes": alg}}
+    # the momentum tendencies with the biharmonic viscosity, on the same
+    # context: velocities of the FP64 model, a random AKMU, ws from level 0
+    mom = {}
+    for del4 in (1, 0):
+        rc.momentum_grid(ls.readyc_mgrid(case, rgrid, del4=del4))
+        gen = torch.Generator(device=model.device).manual_seed(3)
+        mf = rc.momentum_fields(dict(u=st["utf"] * 1.01, v=st["vtf"] * 0.99, up=st["utf"], vp=st["vtf"],
+                                     vit=rt_fields["vit"], viv=rt_fields["viv"],
+                                     akmu=torch.rand((km, jmt, imt), generator=gen, **f64) * 1.0e-2,
+                                     su=torch.full((jmt, imt), 0.1, **f64),
+                                     sv=torch.full((jmt, imt), -0.05, **f64)))
+        rc.momentum(mf)
+        rc.sync()
+        mms = []
+        for _ in range(max(3, args.steps)):
+            rc.momentum(mf)
+            mms.append(rc.last_ms())
+        del mf
+        mavg = sum(mms) / len(mms)
+        malg = readyc_momentum_algorithmic_bytes(case)
+        mach = malg / (mavg * 1e-3) / 1e9
+        name = "del4" if del4 else "del2"
+        print(f"[bench] readyc momentum ({name}): {mavg:.3f} ms per call ({mach:.0f} GB/s algorithmic)",
+              file=sys.stderr, flush=True)
+        mom[name] = {"ms_per_call": round(mavg, 4), "calls": len(mms),
+                     "roofline": {"bound": "hbm", "achieved": round(mach, 1), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(mach / HBM_PEAK_GBS, 4),
+                                  "algorithmic_bytes": malg}}
+    out["momentum"] = mom
+    rc.close()
+    del rc, fields
+    torch.cuda.empty_cache()
+    return out
+
+
+def readyc_momentum_algorithmic_bytes(case):
+    """licom_readyc_momentum's compulsory HBM bytes: reads u, v, up, vp,
+    vit, viv, akmu and ws level 0; writes ws (levels 1..km-1), dlu, dlv,
+    wka (8 bytes each); the 2-D inputs and outputs."""
+    n2, km = case.imt * case.jmt, case.km
+    return n2 * km * 8 * (7 + 4) + n2 * 8 * (3 + 6)
 
 
 def run_readyt(args, torch, lt, case, model):

@@ -93,8 +93,8 @@ int licom_readyc_akmu(licom_readyc *r, const double *akmt, const double *viv, do
  *                      :1784) and readyc4_2 (:793-877, :1789).
  * The scratch arrays of the reference (uk, vk, u_wface, v_sface, hduk2,
  * hdvk2, am_factor, work) are not materialised; with del4 the context holds
- * four 3-D scratch arrays of the block (divergence, vorticity and the two
- * first Laplacians). */
+ * two 3-D scratch arrays of the block (the first Laplacians of up and vp,
+ * scaled by the viscosity factor). */
 typedef struct licom_readyc_mgrid {
     int ib, ie, jb, je;                      /* the block's first and last physical
                                                 points, 1-based as in POP's blocks

@@ -163,11 +163,11 @@ __global__ __launch_bounds__(256) void k_readyc_akmu(CGrid g, const double *__re
 //                   U cells around it, their divergence level by level (kept
 //                   in registers), its column integral, then the recursion of
 //                   ws (upwell_cu_0 + upwell_cu);
-//   k_rmom_divcurl  (del4) one lane per T column: divergence and vorticity of
-//                   (up, vp) per level (hdel4_div, hdel4_zcurl) into scratch;
-//   k_rmom_lap1     (del4) one lane per U column: the viscosity factor from
-//                   the gradients of both (hdel4_grad twice) and the first
-//                   Laplacian scaled by it (hdiffu_del4_0) into scratch;
+//   k_rmom_lap1     (del4) one lane per U column: divergence and vorticity of
+//                   (up, vp) at the four T points around it (hdel4_div,
+//                   hdel4_zcurl), the viscosity factor from their gradients
+//                   (hdel4_grad twice) and the first Laplacian scaled by it
+//                   (hdiffu_del4_0) into scratch;
 //   k_rmom_main     one lane per U column marching k once: w and the face
 //                   velocities at U points (readyc3) formed on the fly,
 //                   advection, the horizontal viscosity (hdiffu_del2_cu or
@@ -183,7 +183,7 @@ struct MGrid {
     const int *kmt, *kmu;
     const double *ohbu, *ohbt, *hts, *htw, *hue, *hun, *tarea_r, *uarea_r, *dxu, *dyu, *dxur, *dyur,
         *snlat, *amf, *duc, *dum, *dun, *dus, *due, *duw, *dmc, *dmn, *dms, *dme, *dmw, *dzp, *odzp;
-    double *div, *curl, *d2u, *d2v;   // del4 scratch, [km][j][i]
+    double *d2u, *d2v;   // del4 scratch, [km][j][i]
 };
 
 __device__ __forceinline__ uint32_t boff(bool ok, size_t idx) { return ok ? (uint32_t)idx * 8u : kBufOff; }
@@ -274,75 +274,6 @@ __global__ __launch_bounds__(256) void k_rmom_upwell(CGrid g, MGrid m, licom_rea
     }
 }
 
-// hdel4_div (readyc.cpp:372-402) and hdel4_zcurl (:343-370) of (up, vp)
-__global__ __launch_bounds__(256) void k_rmom_divcurl(CGrid g, MGrid m, licom_readyc_mfields f) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
-    const int imt = g.imt, jmt = g.jmt, km = g.km;
-    if (i >= imt || j >= jmt) return;
-    const size_t n2 = g.n2, q = (size_t)j * imt + i;
-    const bool dv = i < imt - 1 && j >= 1, cu = i >= 1 && j >= 1;
-    double htwe = 0, htwc = 0, htsc = 0, htsn = 0, tr = 0, dyc = 0, dyn = 0, dyw = 0, dynw = 0, dxc = 0,
-           dxn = 0, dxw = 0, dxnw = 0;
-    if (dv) {
-        htwe = m.htw[q + 1];
-        htwc = m.htw[q];
-        htsc = m.hts[q];
-        htsn = m.hts[q - imt];
-        tr = m.tarea_r[q];
-    }
-    if (cu) {
-        dyc = m.dyu[q];
-        dyn = m.dyu[q - imt];
-        dyw = m.dyu[q - 1];
-        dynw = m.dyu[q - imt - 1];
-        dxc = m.dxu[q];
-        dxn = m.dxu[q - imt];
-        dxw = m.dxu[q - 1];
-        dxnw = m.dxu[q - imt - 1];
-    }
-    const int kmt = m.kmt[q];
-    const size_t pb = n2 * sizeof(double);
-    const uint32_t oc = (uint32_t)q * 8u, oe = boff(dv, q + 1), one = boff(dv, q - imt + 1),
-                   on = boff(j >= 1, q - imt), ow = boff(cu, q - 1), onw = boff(cu, q - imt - 1);
-    struct Lv {
-        double u, ue, une, un, uw, unw, v, ve, vne, vn, vw, vnw;
-    };
-    auto load = [&](int k, Lv &L) {
-        const size_t o = (size_t)k * n2;
-        const auto ru = plane_rsrc(f.up + o, pb), rv = plane_rsrc(f.vp + o, pb);
-        L.u = buf_load(ru, oc, 0.0);
-        L.ue = buf_load(ru, oe, 0.0);
-        L.une = buf_load(ru, one, 0.0);
-        L.un = buf_load(ru, on, 0.0);
-        L.uw = buf_load(ru, ow, 0.0);
-        L.unw = buf_load(ru, onw, 0.0);
-        L.v = buf_load(rv, oc, 0.0);
-        L.ve = buf_load(rv, oe, 0.0);
-        L.vne = buf_load(rv, one, 0.0);
-        L.vn = buf_load(rv, on, 0.0);
-        L.vw = buf_load(rv, ow, 0.0);
-        L.vnw = buf_load(rv, onw, 0.0);
-    };
-    Lv cur, nxt;
-    load(0, cur);
-    for (int k = 0; k < km; ++k) {
-        if (k + 1 < km) load(k + 1, nxt);
-        double d = 0.0e0, z = 0.0e0;
-        if (dv && k <= kmt - 1)
-            d = 0.5 *
-                ((cur.ue + cur.une) * htwe - (cur.u + cur.un) * htwc + (cur.ve + cur.v) * htsc -
-                 (cur.vne + cur.vn) * htsn) *
-                tr;
-        if (cu && k <= kmt - 1)
-            z = 0.5 * ((cur.v * dyc + cur.vn * dyn - cur.vw * dyw - cur.vnw * dynw - cur.u * dxc - cur.uw * dxw +
-                        cur.un * dxn + cur.unw * dxnw));
-        const size_t c = (size_t)k * n2 + q;
-        m.div[c] = d;
-        m.curl[c] = z;
-        cur = nxt;
-    }
-}
-
 // the 5-point operator of hdiffu_*: the du coefficients on a, dm on b
 struct Lap {
     double c, n, s, e, w, mc, mn, ms, me, mw;
@@ -368,8 +299,10 @@ __device__ __forceinline__ Lap load_lap(const MGrid &m, size_t q) {
     return L;
 }
 
-// hdel4_grad twice (readyc.cpp:405-441) and hdiffu_del4_0 (:444-507)
-__global__ __launch_bounds__(256) void k_rmom_lap1(CGrid g, MGrid m, licom_readyc_mfields f) {
+// hdel4_div and hdel4_zcurl at the four T points around the U point,
+// hdel4_grad of both (readyc.cpp:343-441) and hdiffu_del4_0 (:444-507).  The
+// lane reads up and vp over rows j-1..j+1 and columns i-2..i+1.
+__global__ __launch_bounds__(256, 2) void k_rmom_lap1(CGrid g, MGrid m, licom_readyc_mfields f) {
     const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
     if (i >= imt || j >= jmt) return;
@@ -378,9 +311,36 @@ __global__ __launch_bounds__(256) void k_rmom_lap1(CGrid g, MGrid m, licom_ready
     const bool reg = j >= m.jb - 2 && j <= m.je && i >= m.ib - 2 && i <= m.ie;
     const double amf = m.amf[q];
     const int kmu = m.kmu[q];
+    // T points t: 0 (j, i), 1 (j+1, i), 2 (j, i-1), 3 (j+1, i-1)
+    constexpr int DR[4] = {0, 1, 0, 1}, DC[4] = {0, 0, -1, -1};
+    double htwe[4] = {}, htwc[4] = {}, htsc[4] = {}, htsn[4] = {}, tr[4] = {}, dyc[4] = {}, dyn[4] = {},
+           dyw[4] = {}, dynw[4] = {}, dxc[4] = {}, dxn[4] = {}, dxw[4] = {}, dxnw[4] = {};
+    int kt[4] = {};
+    bool cu[4] = {};
     Lap L{};
     double gxf = 0, gyf = 0, dxdy = 0, amd = 1.0;
     if (reg) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const size_t p = q + DR[t] * imt + DC[t];
+            htwe[t] = m.htw[p + 1];
+            htwc[t] = m.htw[p];
+            htsc[t] = m.hts[p];
+            htsn[t] = m.hts[p - imt];
+            tr[t] = m.tarea_r[p];
+            kt[t] = m.kmt[p];
+            cu[t] = i + DC[t] >= 1;   // hdel4_zcurl needs i >= 1
+            if (cu[t]) {
+                dyc[t] = m.dyu[p];
+                dyn[t] = m.dyu[p - imt];
+                dyw[t] = m.dyu[p - 1];
+                dynw[t] = m.dyu[p - imt - 1];
+                dxc[t] = m.dxu[p];
+                dxn[t] = m.dxu[p - imt];
+                dxw[t] = m.dxu[p - 1];
+                dxnw[t] = m.dxu[p - imt - 1];
+            }
+        }
         L = load_lap(m, q);
         gxf = m.dxur[q] * 0.5;
         gyf = m.dyur[q] * 0.5;
@@ -389,33 +349,25 @@ __global__ __launch_bounds__(256) void k_rmom_lap1(CGrid g, MGrid m, licom_ready
         amd = fabs(m.am * amf);
     }
     const size_t pb = n2 * sizeof(double);
-    const uint32_t oc = (uint32_t)q * 8u, os = boff(reg, q + imt), ow = boff(reg, q - 1),
-                   osw = boff(reg, q + imt - 1), on = boff(reg, q - imt), oe = boff(reg, q + 1);
+    uint32_t off[3][4];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            off[r][c] = boff(reg && (c > 0 || i >= 2), q + (size_t)((r - 1) * imt + (c - 2)));
     struct Lv {
-        double dc, ds, dw, dsw, zc, zs, zw, zsw, u, un, us, ue, uw, v, vn, vs, ve, vw;
+        double u[3][4], v[3][4];   // rows j-1..j+1, columns i-2..i+1
     };
     auto load = [&](int k, Lv &V) {
         const size_t o = (size_t)k * n2;
-        const auto rd = plane_rsrc(m.div + o, pb), rz = plane_rsrc(m.curl + o, pb),
-                   ru = plane_rsrc(f.up + o, pb), rv = plane_rsrc(f.vp + o, pb);
-        V.dc = buf_load(rd, reg ? oc : kBufOff, 0.0);
-        V.ds = buf_load(rd, os, 0.0);
-        V.dw = buf_load(rd, ow, 0.0);
-        V.dsw = buf_load(rd, osw, 0.0);
-        V.zc = buf_load(rz, reg ? oc : kBufOff, 0.0);
-        V.zs = buf_load(rz, os, 0.0);
-        V.zw = buf_load(rz, ow, 0.0);
-        V.zsw = buf_load(rz, osw, 0.0);
-        V.u = buf_load(ru, reg ? oc : kBufOff, 0.0);
-        V.un = buf_load(ru, on, 0.0);
-        V.us = buf_load(ru, os, 0.0);
-        V.ue = buf_load(ru, oe, 0.0);
-        V.uw = buf_load(ru, ow, 0.0);
-        V.v = buf_load(rv, reg ? oc : kBufOff, 0.0);
-        V.vn = buf_load(rv, on, 0.0);
-        V.vs = buf_load(rv, os, 0.0);
-        V.ve = buf_load(rv, oe, 0.0);
-        V.vw = buf_load(rv, ow, 0.0);
+        const auto ru = plane_rsrc(f.up + o, pb), rv = plane_rsrc(f.vp + o, pb);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                V.u[r][c] = buf_load(ru, off[r][c], 0.0);
+                V.v[r][c] = buf_load(rv, off[r][c], 0.0);
+            }
     };
     Lv cur, nxt;
     load(0, cur);
@@ -423,17 +375,38 @@ __global__ __launch_bounds__(256) void k_rmom_lap1(CGrid g, MGrid m, licom_ready
         if (k + 1 < km) load(k + 1, nxt);
         double am_f = 1.0e0, hu = 0.0, hv = 0.0;
         if (reg && k <= kmu - 1) {
-            const double gx = gxf * (cur.ds - cur.dw - cur.dsw + cur.dc);
-            const double gy = gyf * (cur.ds - cur.dw + cur.dsw - cur.dc);
+            double dv[4], cz[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int r0 = 1 + DR[t], c0 = 2 + DC[t];
+                const auto &U = cur.u, &V = cur.v;
+                dv[t] = 0.0e0;
+                cz[t] = 0.0e0;
+                if (k <= kt[t] - 1) {
+                    dv[t] = 0.5 *
+                            ((U[r0][c0 + 1] + U[r0 - 1][c0 + 1]) * htwe[t] - (U[r0][c0] + U[r0 - 1][c0]) * htwc[t] +
+                             (V[r0][c0 + 1] + V[r0][c0]) * htsc[t] - (V[r0 - 1][c0 + 1] + V[r0 - 1][c0]) * htsn[t]) *
+                            tr[t];
+                    if (cu[t])
+                        cz[t] = 0.5 * ((V[r0][c0] * dyc[t] + V[r0 - 1][c0] * dyn[t] - V[r0][c0 - 1] * dyw[t] -
+                                        V[r0 - 1][c0 - 1] * dynw[t] - U[r0][c0] * dxc[t] - U[r0][c0 - 1] * dxw[t] +
+                                        U[r0 - 1][c0] * dxn[t] + U[r0 - 1][c0 - 1] * dxnw[t]));
+                }
+            }
+            const double gx = gxf * (dv[1] - dv[2] - dv[3] + dv[0]);
+            const double gy = gyf * (dv[1] - dv[2] + dv[3] - dv[0]);
             double af = 0.0 + (gx * gx + gy * gy);
-            const double gx2 = gxf * (cur.zs - cur.zw - cur.zsw + cur.zc);
-            const double gy2 = gyf * (cur.zs - cur.zw + cur.zsw - cur.zc);
+            const double gx2 = gxf * (cz[1] - cz[2] - cz[3] + cz[0]);
+            const double gy2 = gyf * (cz[1] - cz[2] + cz[3] - cz[0]);
             af += gx2 * gx2 + gy2 * gy2;
             am_f = sqrt(af) * dxdy / amd;
             am_f = fmin(40.0e0, am_f);
             am_f = fmax(1.0e0, am_f);
-            hu = lap_du(L, cur.u, cur.un, cur.us, cur.ue, cur.uw) + lap_dm(L, cur.v, cur.vn, cur.vs, cur.ve, cur.vw);
-            hv = lap_du(L, cur.v, cur.vn, cur.vs, cur.ve, cur.vw) - lap_dm(L, cur.u, cur.un, cur.us, cur.ue, cur.uw);
+            const auto &U = cur.u, &V = cur.v;
+            hu = lap_du(L, U[1][2], U[0][2], U[2][2], U[1][3], U[1][1]) +
+                 lap_dm(L, V[1][2], V[0][2], V[2][2], V[1][3], V[1][1]);
+            hv = lap_du(L, V[1][2], V[0][2], V[2][2], V[1][3], V[1][1]) -
+                 lap_dm(L, U[1][2], U[0][2], U[2][2], U[1][3], U[1][1]);
         }
         const size_t c = (size_t)k * n2 + q;
         m.d2u[c] = am_f * amf * hu;
@@ -490,13 +463,11 @@ __global__ __launch_bounds__(256, 2) void k_rmom_main(CGrid g, MGrid m, licom_re
     struct Lv {
         double u, uw, ue, un, us, v, vw, ve, vn, vs;   // u, v around the U point
         double wc, ws, ww, wsw;                         // ws of the four T points around it
-        double a, an, as, ae, aw, b, bn, bs, be, bw;     // the viscosity operand (up, vp or d2u, d2v)
         double up, vp, vv, ak;
     };
     auto load = [&](int k, Lv &V) {
         const size_t o = (size_t)k * n2;
-        const auto ru = plane_rsrc(f.u + o, pb), rv = plane_rsrc(f.v + o, pb), rw = plane_rsrc(f.ws + o, pb),
-                   ra = plane_rsrc(ha + o, pb), rb = plane_rsrc(hb + o, pb);
+        const auto ru = plane_rsrc(f.u + o, pb), rv = plane_rsrc(f.v + o, pb), rw = plane_rsrc(f.ws + o, pb);
         V.u = buf_load(ru, ocin, 0.0);
         V.uw = buf_load(ru, ow, 0.0);
         V.ue = buf_load(ru, oe, 0.0);
@@ -511,27 +482,34 @@ __global__ __launch_bounds__(256, 2) void k_rmom_main(CGrid g, MGrid m, licom_re
         V.ws = buf_load(rw, os, 0.0);
         V.ww = buf_load(rw, ow, 0.0);
         V.wsw = buf_load(rw, osw, 0.0);
-        V.an = buf_load(ra, hn, 0.0);
-        V.as = buf_load(ra, hs, 0.0);
-        V.ae = buf_load(ra, he, 0.0);
-        V.aw = buf_load(ra, hw, 0.0);
-        V.bn = buf_load(rb, hn, 0.0);
-        V.bs = buf_load(rb, hs, 0.0);
-        V.be = buf_load(rb, he, 0.0);
-        V.bw = buf_load(rb, hw, 0.0);
-        if (DEL4) {
-            V.a = buf_load(ra, hc, 0.0);
-            V.b = buf_load(rb, hc, 0.0);
-            V.up = buf_load(plane_rsrc(f.up + o, pb), oc, 0.0);
-            V.vp = buf_load(plane_rsrc(f.vp + o, pb), oc, 0.0);
-        } else {
-            V.up = buf_load(ra, oc, 0.0);
-            V.vp = buf_load(rb, oc, 0.0);
-            V.a = V.up;
-            V.b = V.vp;
-        }
+        V.up = buf_load(plane_rsrc(f.up + o, pb), oc, 0.0);
+        V.vp = buf_load(plane_rsrc(f.vp + o, pb), oc, 0.0);
         V.vv = buf_load(plane_rsrc(f.viv + o, pb), oc, 0.0);
         V.ak = buf_load(plane_rsrc(f.akmu + o, pb), ocin, 0.0);
+    };
+    // the viscosity operand (up, vp or d2u, d2v) around the U point, loaded
+    // for the current level only
+    struct Hv {
+        double a, an, as, ae, aw, b, bn, bs, be, bw;
+    };
+    auto load_h = [&](int k, const Lv &C, Hv &H) {
+        const size_t o = (size_t)k * n2;
+        const auto ra = plane_rsrc(ha + o, pb), rb = plane_rsrc(hb + o, pb);
+        H.an = buf_load(ra, hn, 0.0);
+        H.as = buf_load(ra, hs, 0.0);
+        H.ae = buf_load(ra, he, 0.0);
+        H.aw = buf_load(ra, hw, 0.0);
+        H.bn = buf_load(rb, hn, 0.0);
+        H.bs = buf_load(rb, hs, 0.0);
+        H.be = buf_load(rb, he, 0.0);
+        H.bw = buf_load(rb, hw, 0.0);
+        if (DEL4) {
+            H.a = buf_load(ra, hc, 0.0);
+            H.b = buf_load(rb, hc, 0.0);
+        } else {
+            H.a = C.up;
+            H.b = C.vp;
+        }
     };
     constexpr double p25 = 0.25, p5 = 0.5, aid = 1 - 0.5;   // (1 - aidif)
     Lv cur, nxt{};
@@ -542,6 +520,8 @@ __global__ __launch_bounds__(256, 2) void k_rmom_main(CGrid g, MGrid m, licom_re
     double dlub = 0.0, dlvb = 0.0, wkb = 0.0, ub = 0.0, vb = 0.0;
     for (int k = 0; k < km; ++k) {
         const bool more = k + 1 < km;
+        Hv hh;
+        load_h(k, cur, hh);
         if (more) load(k + 1, nxt);
         const size_t c = (size_t)k * n2 + q;
         const double wkc = c0f * sqrt(cur.up * cur.up + cur.vp * cur.vp);   // hdiffu_*'s wka
@@ -572,19 +552,14 @@ __global__ __launch_bounds__(256, 2) void k_rmom_main(CGrid g, MGrid m, licom_re
             // hdiffu_del2_cu (:285-335) / hdiffu_del4_1 (:521-559)
             double hu = 0.0, hv = 0.0;
             if (hreg) {
+                const Hv &H = hh;
                 if (DEL4) {
-                    hu = am * (lap_du(L, cur.a, cur.an, cur.as, cur.ae, cur.aw) +
-                               lap_dm(L, cur.b, cur.bn, cur.bs, cur.be, cur.bw));
-                    hv = am * (lap_du(L, cur.b, cur.bn, cur.bs, cur.be, cur.bw) -
-                               lap_dm(L, cur.a, cur.an, cur.as, cur.ae, cur.aw));
+                    hu = am * (lap_du(L, H.a, H.an, H.as, H.ae, H.aw) + lap_dm(L, H.b, H.bn, H.bs, H.be, H.bw));
+                    hv = am * (lap_du(L, H.b, H.bn, H.bs, H.be, H.bw) - lap_dm(L, H.a, H.an, H.as, H.ae, H.aw));
                 } else {
-                    hu = am *
-                         (lap_du(L, cur.a, cur.an, cur.as, cur.ae, cur.aw) +
-                          lap_dm(L, cur.b, cur.bn, cur.bs, cur.be, cur.bw)) *
+                    hu = am * (lap_du(L, H.a, H.an, H.as, H.ae, H.aw) + lap_dm(L, H.b, H.bn, H.bs, H.be, H.bw)) *
                          cur.vv;
-                    hv = am *
-                         (lap_du(L, cur.b, cur.bn, cur.bs, cur.be, cur.bw) -
-                          lap_dm(L, cur.a, cur.an, cur.as, cur.ae, cur.aw)) *
+                    hv = am * (lap_du(L, H.b, H.bn, H.bs, H.be, H.bw) - lap_dm(L, H.a, H.an, H.as, H.ae, H.aw)) *
                          cur.vv;
                 }
             }
@@ -827,7 +802,7 @@ int licom_readyc_momentum_grid(licom_readyc *r, const licom_readyc_mgrid *mg) {
     int *ik = reinterpret_cast<int *>(base + nd * sizeof(double));
     if (!rc) rc = hip_status(hipMemcpy(ik, G.kmt, n2 * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
     if (!rc) rc = hip_status(hipMemcpy(ik + n2, G.kmu, n2 * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
-    if (!rc && G.del4) rc = hip_status(hipMalloc(&r->mscratch, 4 * g.n3 * sizeof(double)), "hipMalloc");
+    if (!rc && G.del4) rc = hip_status(hipMalloc(&r->mscratch, 2 * g.n3 * sizeof(double)), "hipMalloc");
     if (rc) {
         if (r->mmem) (void)hipFree(r->mmem);
         if (r->mscratch) (void)hipFree(r->mscratch);
@@ -855,10 +830,8 @@ int licom_readyc_momentum_grid(licom_readyc *r, const licom_readyc_mgrid *mg) {
     m.kmt = ik;
     m.kmu = ik + n2;
     double *sc = static_cast<double *>(r->mscratch);
-    m.div = sc;
-    m.curl = sc ? sc + g.n3 : nullptr;
-    m.d2u = sc ? sc + 2 * g.n3 : nullptr;
-    m.d2v = sc ? sc + 3 * g.n3 : nullptr;
+    m.d2u = sc;
+    m.d2v = sc ? sc + g.n3 : nullptr;
     r->mom = true;
     return LICOM_OK;
 }
@@ -887,8 +860,6 @@ int licom_readyc_momentum(licom_readyc *r, const licom_readyc_mfields *f) {
     k_rmom_upwell<<<grd, blk, 0, r->stream>>>(g, m, *f);
     if (int rc = hip_status(hipGetLastError(), "k_rmom_upwell launch")) return rc;
     if (m.del4) {
-        k_rmom_divcurl<<<grd, blk, 0, r->stream>>>(g, m, *f);
-        if (int rc = hip_status(hipGetLastError(), "k_rmom_divcurl launch")) return rc;
         k_rmom_lap1<<<grd, blk, 0, r->stream>>>(g, m, *f);
         if (int rc = hip_status(hipGetLastError(), "k_rmom_lap1 launch")) return rc;
         k_rmom_main<true><<<grd, blk, 0, r->stream>>>(g, m, *f);
