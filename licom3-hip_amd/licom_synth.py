@@ -531,3 +531,52 @@ def readyc_minputs(case: Case, rfields: dict, cfields: dict, akmu: np.ndarray, s
     for k in ("dlub", "dlvb", "sbcx", "sbcy", "bbcx", "bbcy"):
         f[k] = np.full(s2, sentinel)
     return {k: np.ascontiguousarray(v, np.float64) for k, v in f.items()}
+
+
+# ------------------------------------------------------------------ bclinc_
+def bclinc_grid(case: Case, rgrid: dict, dtc: float = 1200.0, seed: int = 17) -> dict:
+    """licom_bclinc_grid of a tracer case's block: the readyt_ grid's kmu,
+    ohbu, ohbt, dzp, odzt and 1/dxu, 1/dyu, T-level depths zkt (negative
+    down), a Coriolis parameter from the case's latitudes with both signs,
+    the Coriolis factors of the model's formula (grid.F90's EPEA..EPLB with
+    eps = f dtc / 2 and f dtc), LICOM's filter afc1 = 0.43, nbb = 3."""
+    rng = np.random.default_rng(seed)
+    jmt, imt = case.jmt, case.imt
+    lat = np.deg2rad(case.grid["lat_t"])[:, None] * np.ones((1, imt))
+    lat = np.where(np.arange(jmt)[:, None] < jmt // 3, -lat, lat)
+    fcor = 2.0 * 7.292e-5 * np.sin(lat) * (1.0 + rng.uniform(-0.01, 0.01, (jmt, imt)))
+    eps_e, eps_l = 0.5 * fcor * dtc, fcor * dtc
+    dzp = rgrid["dzp"]
+    afc1 = 0.43
+    theta = np.deg2rad(10.0)
+    return dict(od0=float(case.params["od0"]), c0f=2.6e-3, cag=float(np.cos(theta)),
+                sag=float(np.sin(theta)), onbb=1.0 / 4.0, dtc=dtc, dtc2=2.0 * dtc, afc1=afc1,
+                afc2=1.0 - 2.0 * afc1, kmu=rgrid["kmu"].astype(np.int32), snlat=np.sign(lat) + (lat == 0),
+                ohbt=rgrid["ohbt"], ohbu=rgrid["ohbu"], fcor=fcor, dxur=rgrid["dxur"], dyur=rgrid["dyur"],
+                epea=1.0 / (1.0 + eps_e * eps_e), epeb=eps_e / (1.0 + eps_e * eps_e),
+                epla=1.0 / (1.0 + eps_l * eps_l), eplb=eps_l / (1.0 + eps_l * eps_l),
+                dzp=dzp, odzt=rgrid["odzt"], odzp=1.0 / dzp, zkt=rgrid["zkt"])
+
+
+def bclinc_fields(case: Case, rfields: dict, seed: int = 18) -> dict:
+    """licom_bclinc_fields over a tracer case and its readyt_ fields: two
+    time levels of velocities from the case's (perturbed), buoyancy gg and
+    surface pressure of a smooth synthetic state, AKMU, wind stress,
+    barotropic velocities, readyc_-like tendencies, time sums."""
+    rng = np.random.default_rng(seed)
+    km, jmt, imt = case.km, case.jmt, case.imt
+    s2, s3 = (jmt, imt), (km, jmt, imt)
+    viv, vit = rfields["viv"], rfields["vit"]
+    st = case.state
+    u = st["utf"] * viv
+    v = st["vtf"] * viv
+    f = dict(vit=vit, viv=viv, gg=rng.normal(-1.0e-2, 2.0e-3, s3) * vit,
+             akmu=rng.uniform(1.0e-4, 5.0e-2, s3) * viv,
+             su=rng.normal(0.0, 0.1, s2), sv=rng.normal(0.0, 0.1, s2),
+             psa=1.0e3 + rng.normal(0.0, 5.0, s2), h0bl=rng.normal(0.0, 0.3, s2),
+             ub=rng.normal(0.0, 0.05, s2), vb=rng.normal(0.0, 0.05, s2), h0bf=rng.normal(0.0, 1.2, s2),
+             u=u, v=v, up=u * (1.0 + rng.normal(0.0, 0.02, s3)), vp=v * (1.0 + rng.normal(0.0, 0.02, s3)),
+             dlu=rng.normal(0.0, 1.0e-6, s3) * viv, dlv=rng.normal(0.0, 1.0e-6, s3) * viv,
+             wka=rng.normal(0.0, 1.0, s3), utf=rng.normal(0.0, 0.1, s3), vtf=rng.normal(0.0, 0.1, s3),
+             sbcx=np.full(s2, 7.0), sbcy=np.full(s2, 7.0), bbcx=np.full(s2, 7.0), bbcy=np.full(s2, 7.0))
+    return {k: np.ascontiguousarray(val, np.float64) for k, val in f.items()}

@@ -560,3 +560,60 @@ def readyc_momentum(imt, jmt, km, grid, mgrid, fields, pieces=None):
         for p in pieces:
             getattr(L, "orc_rm_" + p)(ctypes.byref(G), ctypes.byref(M), ctypes.byref(F))
     return out
+
+
+# ------------------------------------------------------------------ bclinc_
+BCLINC_SCALARS = ("od0", "c0f", "cag", "sag", "onbb", "dtc", "dtc2", "afc1", "afc2")
+BCLINC_GRID_2D = ("snlat", "ohbt", "ohbu", "fcor", "dxur", "dyur", "epea", "epeb", "epla", "eplb")
+BCLINC_GRID_1D = ("dzp", "odzt", "odzp", "zkt")
+BCLINC_IN = ("vit", "viv", "gg", "akmu", "su", "sv", "psa", "h0bl", "ub", "vb")
+BCLINC_INOUT = ("h0bf", "u", "v", "up", "vp", "dlu", "dlv", "wka", "utf", "vtf", "sbcx", "sbcy", "bbcx", "bbcy")
+BCLINC_2D = ("su", "sv", "psa", "h0bl", "ub", "vb", "h0bf", "sbcx", "sbcy", "bbcx", "bbcy")
+
+
+class OrcBclincGrid(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_int) for k in ("imt", "jmt", "km")] +
+                [(k, ctypes.c_double) for k in BCLINC_SCALARS] + [("kmu", _IP)] +
+                [(k, _DP) for k in BCLINC_GRID_2D + BCLINC_GRID_1D])
+
+
+class OrcBclincFields(ctypes.Structure):
+    _fields_ = [(k, _DP) for k in BCLINC_IN + BCLINC_INOUT]
+
+
+def bclinc_stage(imt, jmt, km, grid, fields, isc, stages=(0, 1, 2)):
+    """bclinc_ (bclinc.cpp:728-827) on numpy arrays, the given stages in
+    order with no halo exchange between them (stage 0: bclinc12, bclinc11,
+    then bclinc2 + bclinc3 or, at isc < 1, bclinc6 + bclinc71 + bclinc72;
+    stage 1: bclinc4 or bclinc81 + bclinc82; stage 2: bclinc5).  Returns a
+    dict with a copy of every in/out field after the last stage."""
+    L = lib()
+    if not getattr(L, "_bclinc_bound", False):
+        L.orc_bclinc_stage.argtypes = [ctypes.POINTER(OrcBclincGrid), ctypes.POINTER(OrcBclincFields),
+                                       ctypes.c_int, ctypes.c_int]
+        L._bclinc_bound = True
+    keep = []
+    G = OrcBclincGrid()
+    G.imt, G.jmt, G.km = int(imt), int(jmt), int(km)
+    for k in BCLINC_SCALARS:
+        setattr(G, k, float(grid[k]))
+    kmu = np.ascontiguousarray(grid["kmu"], np.int32)
+    keep.append(kmu)
+    G.kmu = _i(kmu)
+    for k in BCLINC_GRID_2D + BCLINC_GRID_1D:
+        a = np.ascontiguousarray(grid[k], np.float64)
+        keep.append(a)
+        setattr(G, k, _d(a))
+    s2, s3 = (jmt, imt), (km, jmt, imt)
+    ins = {}
+    for k in BCLINC_IN:
+        a = np.ascontiguousarray(fields[k], np.float64)
+        if a.shape != (s2 if k in BCLINC_2D else s3):
+            raise ValueError(f"bclinc_stage: {k} has shape {a.shape}")
+        keep.append(a)
+        ins[k] = a
+    out = {k: np.array(fields[k], np.float64, order="C") for k in BCLINC_INOUT}
+    F = OrcBclincFields(*([_d(ins[k]) for k in BCLINC_IN] + [_d(out[k]) for k in BCLINC_INOUT]))
+    for s in stages:
+        L.orc_bclinc_stage(ctypes.byref(G), ctypes.byref(F), int(isc), int(s))
+    return out

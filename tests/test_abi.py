@@ -154,3 +154,36 @@ def test_readyc_create_validates_before_touching_a_device():
     assert L.licom_readyc_create(ctypes.byref(G), ctypes.byref(h)) == -1
     assert b"NULL" in L.licom_last_error()
     assert not h.value
+
+
+def test_bclinc_declarations_are_exported():
+    import licom_bclinc
+    names = declared("licom_bclinc.h")
+    assert names
+    L = ctypes.CDLL(licom_bclinc.lt.lib_path())
+    assert [n for n in names if not hasattr(L, n)] == []
+    assert sorted(licom_bclinc.EXPORTED) == names
+
+
+def test_bclinc_create_and_stage_validate_before_touching_a_device():
+    import numpy as np
+    import licom_bclinc as lb
+    import licom_synth as ls
+    L = lb.lib()
+    c = ls.make_case("C1", nx=16, ny=8, km=6, seed=2)
+    g = ls.bclinc_grid(c, ls.readyt_grid(c))
+    keep = []
+    h = ctypes.c_void_p()
+    G = lb.make_grid(c.imt, c.jmt, 1, g, keep)
+    assert L.licom_bclinc_create(ctypes.byref(G), ctypes.byref(h)) == -1
+    assert b"invalid extents" in L.licom_last_error()
+    bad = dict(g, kmu=np.full((c.jmt, c.imt), c.km + 1, np.int32))
+    G = lb.make_grid(c.imt, c.jmt, c.km, bad, keep)
+    assert L.licom_bclinc_create(ctypes.byref(G), ctypes.byref(h)) == -1
+    assert b"kmu outside" in L.licom_last_error()
+    G = lb.make_grid(c.imt, c.jmt, c.km, g, keep)
+    G.epla = None
+    assert L.licom_bclinc_create(ctypes.byref(G), ctypes.byref(h)) == -1
+    assert b"NULL" in L.licom_last_error()
+    assert not h.value
+    assert L.licom_bclinc_stage(None, None, 0, 0) == -1
