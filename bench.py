@@ -439,6 +439,67 @@ def run_readyc(args, torch, case, rgrid, rt_fields, st, model):
     return out
 
 
+def bclinc_algorithmic_bytes(case):
+    """bclinc_'s compulsory HBM bytes for a leapfrog step (its three stages):
+    stage 0 reads vit, gg, up, vp, dlu, dlv, viv, akmu and writes dlu, dlv,
+    wka; stage 1 reads wka, v, vp, viv, up, dlu, akmu and writes vp, v, wka;
+    stage 2 reads wka, u, up, viv, utf, vtf, v and writes up, u, utf, vtf
+    (8 bytes each); the 2-D fields."""
+    n2, km = case.imt * case.jmt, case.km
+    return n2 * km * 8 * (22 + 10) + n2 * 8 * (10 + 6)
+
+
+def run_bclinc(args, torch, case, rgrid, rt_fields, st, model):
+    """bclinc_ at the workload size: one leapfrog step (stages 0, 1, 2 with
+    no halo exchange between, a single block) and one first step (stages 0,
+    1), on the FP64 model's velocities and synthetic buoyancy, viscosity and
+    forcing built on the device: ms per call and the HBM roofline."""
+    import licom_bclinc as lb
+    import licom_synth as ls
+    km, jmt, imt = case.km, case.jmt, case.imt
+    dev = model.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    gen = torch.Generator(device=dev).manual_seed(5)
+    rnd = lambda shape, scale: torch.randn(shape, generator=gen, **f64) * scale  # noqa: E731
+    s2, s3 = (jmt, imt), (km, jmt, imt)
+    viv, vit = rt_fields["viv"], rt_fields["vit"]
+    fields = dict(vit=vit, viv=viv, gg=(rnd(s3, 2.0e-3) - 1.0e-2) * vit,
+                  akmu=torch.rand(s3, generator=gen, **f64) * 5.0e-2 * viv,
+                  su=rnd(s2, 0.1), sv=rnd(s2, 0.1), psa=rnd(s2, 5.0) + 1.0e3, h0bl=rnd(s2, 0.3),
+                  ub=rnd(s2, 0.05), vb=rnd(s2, 0.05), h0bf=rnd(s2, 1.0), u=st["utf"] * viv,
+                  v=st["vtf"] * viv, up=st["utf"] * viv, vp=st["vtf"] * viv, dlu=rnd(s3, 1.0e-6) * viv,
+                  dlv=rnd(s3, 1.0e-6) * viv)
+    b = lb.Bclinc(imt, jmt, km, ls.bclinc_grid(case, rgrid), fields, device=dev, stream=model.stream)
+    b.step(2)
+    b.sync()
+    out = {}
+    for name, isc, stages in (("leapfrog", 2, (0, 1, 2)), ("first_step", 0, (0, 1))):
+        ms, per = [], [[] for _ in stages]
+        for _ in range(max(3, args.steps)):
+            tot = 0.0
+            for n, s in enumerate(stages):
+                b.stage(isc, s)
+                t = b.last_ms()
+                per[n].append(t)
+                tot += t
+            ms.append(tot)
+        avg = sum(ms) / len(ms)
+        entry = {"ms_per_call": round(avg, 4), "calls": len(ms),
+                 "stage_ms": [round(sum(p) / len(p), 4) for p in per]}
+        if name == "leapfrog":
+            alg = bclinc_algorithmic_bytes(case)
+            ach = alg / (avg * 1e-3) / 1e9
+            entry["roofline"] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes": alg}
+        print(f"[bench] bclinc {name}: {avg:.3f} ms per call (stages {entry['stage_ms']})", file=sys.stderr,
+              flush=True)
+        out[name] = entry
+    b.close()
+    del b, fields
+    torch.cuda.empty_cache()
+    return out
+
+
 def readyc_momentum_algorithmic_bytes(case):
     """licom_readyc_momentum's compulsory HBM bytes: reads u, v, up, vp,
     vit, viv, akmu and ws level 0; writes ws (levels 1..km-1), dlu, dlv,
@@ -486,6 +547,7 @@ def run_readyt(args, torch, lt, case, model):
     part = [sum(p[i] for p in parts) / len(parts) for i in range(3)]
     alg = readyt_algorithmic_bytes(case)
     shear = run_readyc(args, torch, case, g, keep, st, model)
+    bcl = run_bclinc(args, torch, case, g, keep, st, model)
     ach = alg / (avg * 1e-3) / 1e9
     print(f"[bench] readyt: {avg:.3f} ms per call ({ach:.0f} GB/s algorithmic)", file=sys.stderr,
           flush=True)
@@ -497,7 +559,7 @@ def run_readyt(args, torch, lt, case, model):
                         "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                         "traffic": traffic_from_profiles("k_readyt_col", args.workload),
                         "algorithmic_bytes": alg},
-           "readyc_shear": shear}
+           "readyc_shear": shear, "bclinc": bcl}
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import orc
