@@ -580,3 +580,56 @@ def bclinc_fields(case: Case, rfields: dict, seed: int = 18) -> dict:
              wka=rng.normal(0.0, 1.0, s3), utf=rng.normal(0.0, 0.1, s3), vtf=rng.normal(0.0, 0.1, s3),
              sbcx=np.full(s2, 7.0), sbcy=np.full(s2, 7.0), bbcx=np.full(s2, 7.0), bbcy=np.full(s2, 7.0))
     return {k: np.ascontiguousarray(val, np.float64) for k, val in f.items()}
+
+
+# ------------------------------------------------------------------ barotr_
+def barotr_grid(case: Case, rgrid: dict, del4: bool = True, dtb: float = 20.0, seed: int = 19) -> dict:
+    """barotr_'s grid over a tracer case's block: the readyc_ momentum
+    grid's metrics and block bounds, kmt's neighbour levels kmtn..kmtw, the
+    tracer del2 coefficients dtn..dtw (the case's own, Cartesian-like when
+    it has none), the barotropic Coriolis factors ebea/ebeb (grid.F90's
+    formula with eps = f dtb / 2), am and ah for the scheme."""
+    rng = np.random.default_rng(seed)
+    jmt, imt = case.jmt, case.imt
+    mg = readyc_mgrid(case, rgrid, del4=del4, seed=seed)
+    kmt = case.grid["kmt"].astype(np.int32)
+    pad = np.zeros_like(kmt)
+    kmtn, kmts, kmte, kmtw = pad.copy(), pad.copy(), pad.copy(), pad.copy()
+    kmtn[1:] = kmt[:-1]
+    kmts[:-1] = kmt[1:]
+    kmte[:, :-1] = kmt[:, 1:]
+    kmtw[:, 1:] = kmt[:, :-1]
+    dx = case.grid["dxu"]
+    tl = lambda: (1.0 + rng.uniform(-0.1, 0.1, (jmt, imt))) / (dx * dx)  # noqa: E731
+    lat = np.deg2rad(case.grid["lat_t"])[:, None] * np.ones((1, imt))
+    fcor = 2.0 * 7.292e-5 * np.sin(np.where(np.arange(jmt)[:, None] < jmt // 3, -lat, lat))
+    eps = 0.5 * fcor * dtb
+    g = {k: mg[k] for k in ("ib", "ie", "jb", "je", "del4", "kmu", "htw", "hts", "tarea_r", "uarea_r", "dxu",
+                            "dyu", "dxur", "dyur", "amf", "duc", "dum", "dun", "dus", "due", "duw", "dmc",
+                            "dmn", "dms", "dme", "dmw")}
+    g.update(dtb=dtb, am=mg["am"], ah=-2.0e13 if del4 else 1.0e3, kmt=kmt, kmtn=kmtn, kmts=kmts, kmte=kmte,
+             kmtw=kmtw, au0=rgrid["au0"], aus=rgrid["aus"], auw=rgrid["auw"], ausw=rgrid["ausw"], fcor=fcor,
+             ebea=1.0 / (1.0 + eps * eps), ebeb=eps / (1.0 + eps * eps),
+             ahf=10.0 ** rng.uniform(-1.0, 1.0, (jmt, imt)), dtn=tl(), dts=tl(), dte=tl(), dtw=tl())
+    return g
+
+
+def barotr_fields(case: Case, rfields: dict, seed: int = 20) -> dict:
+    """barotr_'s fields over a tracer case: level 0 of the wet masks, layer
+    depth dzph with the surface height, barotropic velocities at two levels,
+    the forcing terms bclinc_/readyt_ hand over (dlub, pax, pxb, whx, wgp
+    ...), running sums h0f/h0bf; work and wka start at a sentinel."""
+    rng = np.random.default_rng(seed)
+    jmt, imt = case.jmt, case.imt
+    s2 = (jmt, imt)
+    viv, vit = rfields["viv"][0], rfields["vit"][0]
+    h0p = rng.normal(0.0, 0.5, s2) * vit
+    ubp, vbp = rng.normal(0.0, 0.1, s2) * viv, rng.normal(0.0, 0.1, s2) * viv
+    f = dict(vit=vit, viv=viv, dzph=(3000.0 + rng.uniform(-500.0, 500.0, s2)) * viv,
+             dlub=rng.normal(0.0, 1.0e-6, s2), dlvb=rng.normal(0.0, 1.0e-6, s2), wgp=rng.uniform(0.9, 1.0, s2),
+             pax=rng.normal(0.0, 1.0e-7, s2), pay=rng.normal(0.0, 1.0e-7, s2), pxb=rng.normal(0.0, 1.0e-6, s2),
+             pyb=rng.normal(0.0, 1.0e-6, s2), whx=rng.normal(0.0, 1.0e-5, s2), why=rng.normal(0.0, 1.0e-5, s2),
+             h0=h0p + rng.normal(0.0, 0.01, s2), h0p=h0p, ub=ubp + rng.normal(0.0, 0.01, s2), vb=vbp.copy(),
+             ubp=ubp, vbp=vbp, h0f=rng.normal(0.0, 1.0, s2), h0bf=rng.normal(0.0, 1.0, s2),
+             work=np.full(s2, 7.0), wka=np.full((6, jmt, imt), 7.0))
+    return {k: np.ascontiguousarray(v, np.float64) for k, v in f.items()}

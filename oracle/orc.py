@@ -617,3 +617,59 @@ def bclinc_stage(imt, jmt, km, grid, fields, isc, stages=(0, 1, 2)):
     for s in stages:
         L.orc_bclinc_stage(ctypes.byref(G), ctypes.byref(F), int(isc), int(s))
     return out
+
+
+# ------------------------------------------------------------------ barotr_
+BAROTR_INTS = ("ib", "ie", "jb", "je", "del4")
+BAROTR_SCALARS = ("dtb", "am", "ah")
+BAROTR_IGRID = ("kmt", "kmu", "kmtn", "kmts", "kmte", "kmtw")
+BAROTR_DGRID = ("au0", "aus", "auw", "ausw", "htw", "hts", "tarea_r", "uarea_r", "dxu", "dyu", "dxur",
+                "dyur", "fcor", "ebea", "ebeb", "amf", "ahf", "dtn", "dts", "dte", "dtw", "duc", "dum",
+                "dun", "dus", "due", "duw", "dmc", "dmn", "dms", "dme", "dmw")
+BAROTR_IN = ("vit", "viv", "dzph", "dlub", "dlvb", "wgp", "pax", "pay", "pxb", "pyb", "whx", "why")
+BAROTR_INOUT = ("h0", "h0p", "ub", "vb", "ubp", "vbp", "h0f", "h0bf", "work", "wka")
+
+
+class OrcBarotrGrid(ctypes.Structure):
+    _fields_ = ([(k, ctypes.c_int) for k in ("imt", "jmt") + BAROTR_INTS] +
+                [(k, ctypes.c_double) for k in BAROTR_SCALARS] + [(k, _IP) for k in BAROTR_IGRID] +
+                [(k, _DP) for k in BAROTR_DGRID])
+
+
+class OrcBarotrFields(ctypes.Structure):
+    _fields_ = [(k, _DP) for k in BAROTR_IN + BAROTR_INOUT]
+
+
+def barotr_stage(imt, jmt, grid, fields, nc, stages=(0, 1, 2, 3)):
+    """One subcycle nc of barotr_ (barotr.cpp:901-1024) on numpy arrays, the
+    given stages in order with no halo exchange between them.  Returns a dict
+    with a copy of every in/out field (wka: 6 planes)."""
+    L = lib()
+    if not getattr(L, "_barotr_bound", False):
+        L.orc_barotr_stage.argtypes = [ctypes.POINTER(OrcBarotrGrid), ctypes.POINTER(OrcBarotrFields),
+                                       ctypes.c_int, ctypes.c_int]
+        L._barotr_bound = True
+    keep = []
+    G = OrcBarotrGrid()
+    G.imt, G.jmt = int(imt), int(jmt)
+    for k in BAROTR_INTS:
+        setattr(G, k, int(grid[k]))
+    for k in BAROTR_SCALARS:
+        setattr(G, k, float(grid[k]))
+    for k in BAROTR_IGRID:
+        a = np.ascontiguousarray(grid[k], np.int32)
+        keep.append(a)
+        setattr(G, k, _i(a))
+    for k in BAROTR_DGRID:
+        a = np.ascontiguousarray(grid[k], np.float64)
+        keep.append(a)
+        setattr(G, k, _d(a))
+    ins = [np.ascontiguousarray(fields[k], np.float64) for k in BAROTR_IN]
+    keep += ins
+    out = {k: np.array(fields[k], np.float64, order="C") for k in BAROTR_INOUT}
+    if out["wka"].shape != (6, jmt, imt):
+        raise ValueError(f"barotr_stage: wka has shape {out['wka'].shape}")
+    F = OrcBarotrFields(*([_d(a) for a in ins] + [_d(out[k]) for k in BAROTR_INOUT]))
+    for s in stages:
+        L.orc_barotr_stage(ctypes.byref(G), ctypes.byref(F), int(nc), int(s))
+    return out
