@@ -187,3 +187,32 @@ def test_bclinc_create_and_stage_validate_before_touching_a_device():
     assert b"NULL" in L.licom_last_error()
     assert not h.value
     assert L.licom_bclinc_stage(None, None, 0, 0) == -1
+
+
+def test_barotr_declarations_are_exported():
+    import licom_barotr
+    names = declared("licom_barotr.h")
+    assert names
+    L = ctypes.CDLL(licom_barotr.lt.lib_path())
+    assert [n for n in names if not hasattr(L, n)] == []
+    assert sorted(licom_barotr.EXPORTED) == names
+
+
+def test_barotr_create_validates_before_touching_a_device():
+    import licom_barotr as lbt
+    import licom_synth as ls
+    L = lbt.lib()
+    c = ls.make_case("C1", nx=16, ny=8, km=6, seed=2)
+    g = ls.barotr_grid(c, ls.readyt_grid(c), del4=True)
+    keep = []
+    h = ctypes.c_void_p()
+    assert L.licom_barotr_create(ctypes.byref(lbt.make_grid(4, c.jmt, g, keep)), ctypes.byref(h)) == -1
+    assert b"invalid extents" in L.licom_last_error()
+    G = lbt.make_grid(c.imt, c.jmt, dict(g, ie=c.imt - 1), keep)
+    assert L.licom_barotr_create(ctypes.byref(G), ctypes.byref(h)) == -1
+    assert b"block bounds" in L.licom_last_error()
+    G = lbt.make_grid(c.imt, c.jmt, g, keep)
+    G.dtw = None
+    assert L.licom_barotr_create(ctypes.byref(G), ctypes.byref(h)) == -1
+    assert b"NULL" in L.licom_last_error()
+    assert not h.value
