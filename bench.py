@@ -500,6 +500,62 @@ def run_bclinc(args, torch, case, rgrid, rt_fields, st, model):
     return out
 
 
+def barotr_algorithmic_bytes(case):
+    """One barotr_ subcycle's compulsory HBM bytes over its 2-D fields: 34
+    reads and 16 writes of 8 bytes per cell (stage 0: h0, ub, vb, dzph, vit
+    -> work, wka2, wka3; stage 1: h0p, work -> h0, then h0, ubp, vbp, viv and
+    the nine forcing fields -> work, wka2, wka3; stage 2: ubp, vbp, wka2,
+    wka3, h0, dzph, h0p, vit -> ub, vb, work; stage 3: h0p, work, ub, vb, h0f,
+    h0bf -> h0p, ubp, vbp, h0f, h0bf, h0).  The grid metrics are not
+    counted."""
+    return case.imt * case.jmt * 8 * (34 + 16)
+
+
+def run_barotr(args, torch, case, rgrid, rt_fields, model):
+    """barotr_'s subcycle at the workload size with the biharmonic
+    viscosity: stages 0-3 of subcycles 0..K-1 (every fourth with the surface
+    diffusion), no halo exchange between (a single block), synthetic 2-D
+    fields on the device: ms per subcycle and per stage."""
+    import licom_barotr as lbt
+    import licom_synth as ls
+    jmt, imt = case.jmt, case.imt
+    dev = model.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    gen = torch.Generator(device=dev).manual_seed(7)
+    rnd = lambda scale: torch.randn((jmt, imt), generator=gen, **f64) * scale  # noqa: E731
+    vit, viv = rt_fields["vit"][0].contiguous(), rt_fields["viv"][0].contiguous()
+    fields = dict(vit=vit, viv=viv, dzph=(3000.0 + rnd(100.0)) * viv, dlub=rnd(1e-6), dlvb=rnd(1e-6),
+                  wgp=torch.full((jmt, imt), 0.95, **f64), pax=rnd(1e-7), pay=rnd(1e-7), pxb=rnd(1e-6),
+                  pyb=rnd(1e-6), whx=rnd(1e-5), why=rnd(1e-5), h0=rnd(0.5) * vit, h0p=rnd(0.5) * vit,
+                  ub=rnd(0.1) * viv, vb=rnd(0.1) * viv, ubp=rnd(0.1) * viv, vbp=rnd(0.1) * viv,
+                  h0f=torch.zeros((jmt, imt), **f64), h0bf=torch.zeros((jmt, imt), **f64))
+    b = lbt.Barotr(imt, jmt, ls.barotr_grid(case, rgrid, del4=True), fields, device=dev, stream=model.stream)
+    b.run(1)
+    b.sync()
+    reps = max(4, args.steps)
+    per = [[] for _ in range(4)]
+    tot = []
+    for nc in range(reps):
+        t = 0.0
+        for s in range(4):
+            b.stage(nc, s)
+            ms = b.last_ms()
+            per[s].append(ms)
+            t += ms
+        tot.append(t)
+    b.close()
+    del b, fields
+    torch.cuda.empty_cache()
+    avg = sum(tot) / len(tot)
+    alg = barotr_algorithmic_bytes(case)
+    ach = alg / (avg * 1e-3) / 1e9
+    stage_ms = [round(sum(p) / len(p), 4) for p in per]
+    print(f"[bench] barotr subcycle: {avg:.3f} ms (stages {stage_ms})", file=sys.stderr, flush=True)
+    return {"ms_per_subcycle": round(avg, 4), "subcycles": reps, "stage_ms": stage_ms,
+            "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "algorithmic_bytes": alg}}
+
+
 def readyc_momentum_algorithmic_bytes(case):
     """licom_readyc_momentum's compulsory HBM bytes: reads u, v, up, vp,
     vit, viv, akmu and ws level 0; writes ws (levels 1..km-1), dlu, dlv,
@@ -548,6 +604,7 @@ def run_readyt(args, torch, lt, case, model):
     alg = readyt_algorithmic_bytes(case)
     shear = run_readyc(args, torch, case, g, keep, st, model)
     bcl = run_bclinc(args, torch, case, g, keep, st, model)
+    bcl["barotr"] = run_barotr(args, torch, case, g, keep, model)
     ach = alg / (avg * 1e-3) / 1e9
     print(f"[bench] readyt: {avg:.3f} ms per call ({ach:.0f} GB/s algorithmic)", file=sys.stderr,
           flush=True)
