@@ -626,15 +626,45 @@ def run_readyt(args, torch, lt, case, model):
         times = []
         for _ in range(3):
             t0 = time.perf_counter()
-            orc.readyt(band.imt, band.jmt, band.km, gb, fb, ist=1)
+            ob = orc.readyt(band.imt, band.jmt, band.km, gb, fb, ist=1)
             times.append(time.perf_counter() - t0)
         cells = band.imt * band.jmt * band.km
-        out["cpu_baseline"] = {"value": cells / sorted(times)[1], "unit": "cells/s",
-                               "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        shape = f"{band.imt}x{band.jmt}x{band.km}"
+        out["cpu_baseline"] = {"value": cells / sorted(times)[1], "unit": "cells/s", "cores": cores,
                                "kind": "port",
-                               "sample": f"oracle readyt_ restatement on a {band.imt}x{band.jmt}x{band.km} "
-                                         "block, median of 3 calls"}
+                               "sample": f"oracle readyt_ restatement on a {shape} block, median of 3 calls"}
+        out["cpu_baseline_rest"] = stepon_cpu_baselines(orc, ls, band, gb, fb, ob, cores)
     return out
+
+
+def stepon_cpu_baselines(orc, ls, band, gb, fb, ob, cores):
+    """The oracle's readyc_ (shear, momentum with del4), bclinc_ (a
+    leapfrog step) and barotr_ (one subcycle, del4) on the same band as
+    readyt_'s CPU baseline, median of 3 calls each, in cells (columns x
+    levels; barotr_: columns) per second.  Their loops are partly OpenMP."""
+    imt, jmt, km = band.imt, band.jmt, band.km
+    cg, cf = ls.readyc_inputs(band, gb, ob)
+    akmu = orc.readyc_akmu(imt, jmt, km, cg, np.full((km, jmt, imt), 1.0e-3), cf["viv"])
+    mg = ls.readyc_mgrid(band, gb, del4=1)
+    mf = ls.readyc_minputs(band, ob, cf, akmu)
+    bg, bfl = ls.bclinc_grid(band, gb), ls.bclinc_fields(band, fb)
+    tg, tf = ls.barotr_grid(band, gb, del4=True), ls.barotr_fields(band, fb)
+    calls = {"readyc_shear": (lambda: orc.readyc_shear(imt, jmt, km, cg, cf), imt * jmt * km),
+             "readyc_momentum_del4": (lambda: orc.readyc_momentum(imt, jmt, km, cg, mg, mf), imt * jmt * km),
+             "bclinc_leapfrog": (lambda: orc.bclinc_stage(imt, jmt, km, bg, bfl, 2, (0, 1, 2)), imt * jmt * km),
+             "barotr_subcycle": (lambda: orc.barotr_stage(imt, jmt, tg, tf, 0, (0, 1, 2, 3)), imt * jmt)}
+    res = {}
+    for name, (fn, cells) in calls.items():
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+        res[name] = {"value": cells / sorted(times)[1], "unit": "cells/s" if name != "barotr_subcycle"
+                     else "columns/s", "ms": round(sorted(times)[1] * 1e3, 2)}
+    return {"kind": "port", "cores": cores, "sample": f"the oracle on a {imt}x{jmt}x{km} band, median of 3",
+            "calls": res}
 
 
 def run_variant(model, args, torch, lt, case, updates):
