@@ -212,6 +212,43 @@ __global__ __launch_bounds__(256) void k_bcl_a(BGrid g, licom_bclinc_fields f, i
     }
 }
 
+// A column of x in registers with its depth sum in bclinc4/5's product order
+// (dzp ohbu x viv), x and viv read once, not again for the update; viv as a
+// bit mask when every level's is +0.0 or 1.0.
+struct Column {
+    double x[LICOM_MAX_KM];
+    uint64_t vone;
+    bool v01;
+    double work;
+};
+
+__device__ __forceinline__ void load_column(const BGrid &g, const double *__restrict__ x,
+                                            const double *__restrict__ viv, size_t q, double ohbu, Column &C) {
+    const size_t n2 = g.n2;
+    const int km = g.km;
+    C.vone = 0;
+    C.v01 = true;
+    C.work = 0.0;
+#pragma unroll
+    for (int k = 0; k < LICOM_MAX_KM; ++k) {
+        if (k < km) {
+            const size_t c = (size_t)k * n2 + q;
+            const double xv = x[c], vv = viv[c];
+            C.x[k] = xv;
+            C.work += g.dzp[k] * ohbu * xv * vv;
+            const uint64_t bits = __builtin_bit_cast(uint64_t, vv);
+            if (bits == 0x3FF0000000000000ull)
+                C.vone |= 1ull << k;
+            else if (bits != 0)
+                C.v01 = false;
+        }
+    }
+}
+
+__device__ __forceinline__ double viv_of(const Column &C, const double *__restrict__ viv, size_t c, int k) {
+    return C.v01 ? (((C.vone >> k) & 1ull) ? 1.0 : 0.0) : viv[c];
+}
+
 // bclinc4_cu (bclinc.cpp:266-379) or bclinc81_cu + bclinc82_cu (:670-726)
 template <bool EULER>
 __global__ __launch_bounds__(256) void k_bcl_b(BGrid g, licom_bclinc_fields f) {
@@ -221,7 +258,8 @@ __global__ __launch_bounds__(256) void k_bcl_b(BGrid g, licom_bclinc_fields f) {
     const size_t n2 = g.n2, q = (size_t)j * imt + i;
     const double ohbu = g.ohbu[q];
     if (EULER) {
-        // u and v as the caller's halo exchange left them
+        // u and v as the caller's halo exchange left them; two plain passes
+        // (the register column of both velocities would take 256 VGPRs)
         for (int pass = 0; pass < 2; ++pass) {
             double *x = pass ? f.v : f.u, *tf = pass ? f.vtf : f.utf;
             const double b = pass ? f.vb[q] : f.ub[q];
@@ -239,19 +277,19 @@ __global__ __launch_bounds__(256) void k_bcl_b(BGrid g, licom_bclinc_fields f) {
         }
         return;
     }
-    double work = 0;
-    for (int k = 0; k < km; ++k) {
-        const size_t c = (size_t)k * n2 + q;
-        work += g.dzp[k] * ohbu * f.wka[c] * f.viv[c];
-    }
+    Column C;
+    load_column(g, f.wka, f.viv, q, ohbu, C);
     const bool in2 = interior2(g, j, i);
     const double vb = f.vb[q];
-    for (int k = 0; k < km; ++k) {
-        const size_t c = (size_t)k * n2 + q;
-        const double w = (f.wka[c] - work + vb) * f.viv[c];
-        f.vp[c] = g.afc2 * f.v[c] + g.afc1 * (f.vp[c] + w);
-        f.v[c] = w;
-        f.wka[c] = in2 ? f.up[c] + f.dlu[c] * g.dtc2 : w;
+#pragma unroll
+    for (int k = 0; k < LICOM_MAX_KM; ++k) {
+        if (k < km) {
+            const size_t c = (size_t)k * n2 + q;
+            const double w = (C.x[k] - C.work + vb) * viv_of(C, f.viv, c, k);
+            f.vp[c] = g.afc2 * f.v[c] + g.afc1 * (f.vp[c] + w);
+            f.v[c] = w;
+            f.wka[c] = in2 ? f.up[c] + f.dlu[c] * g.dtc2 : w;
+        }
     }
     if (!in2) return;
     const int kz = g.kmu[q];
@@ -265,18 +303,18 @@ __global__ __launch_bounds__(256) void k_bcl_c(BGrid g, licom_bclinc_fields f) {
     if (i >= imt || j >= jmt) return;
     const size_t n2 = g.n2, q = (size_t)j * imt + i;
     const double ohbu = g.ohbu[q], ub = f.ub[q];
-    double work = 0.0e0;
-    for (int k = 0; k < km; ++k) {
-        const size_t c = (size_t)k * n2 + q;
-        work += g.dzp[k] * ohbu * f.wka[c] * f.viv[c];
-    }
-    for (int k = 0; k < km; ++k) {
-        const size_t c = (size_t)k * n2 + q;
-        const double w = (f.wka[c] - work + ub) * f.viv[c];
-        f.up[c] = g.afc2 * f.u[c] + g.afc1 * (f.up[c] + w);
-        f.u[c] = w;
-        f.utf[c] += w;
-        f.vtf[c] += f.v[c];
+    Column C;
+    load_column(g, f.wka, f.viv, q, ohbu, C);
+#pragma unroll
+    for (int k = 0; k < LICOM_MAX_KM; ++k) {
+        if (k < km) {
+            const size_t c = (size_t)k * n2 + q;
+            const double w = (C.x[k] - C.work + ub) * viv_of(C, f.viv, c, k);
+            f.up[c] = g.afc2 * f.u[c] + g.afc1 * (f.up[c] + w);
+            f.u[c] = w;
+            f.utf[c] += w;
+            f.vtf[c] += f.v[c];
+        }
     }
 }
 
