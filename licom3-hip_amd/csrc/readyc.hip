@@ -163,9 +163,10 @@ __global__ __launch_bounds__(256) void k_readyc_akmu(CGrid g, const double *__re
 //                   U cells around it, their divergence level by level (kept
 //                   in registers), its column integral, then the recursion of
 //                   ws (upwell_cu_0 + upwell_cu);
-//   k_rmom_lap1     (del4) one lane per U column: divergence and vorticity of
-//                   (up, vp) at the four T points around it (hdel4_div,
-//                   hdel4_zcurl), the viscosity factor from their gradients
+//   k_rmom_lap1     (del4) a 64x4 block of U columns marching k with an
+//                   LDS window of (up, vp): divergence and vorticity once per
+//                   T point (hdel4_div, hdel4_zcurl), then per U column the
+//                   viscosity factor from those of its four T points
 //                   (hdel4_grad twice) and the first Laplacian scaled by it
 //                   (hdiffu_del4_0) into scratch;
 //   k_rmom_main     one lane per U column marching k once: w and the face
@@ -299,48 +300,87 @@ __device__ __forceinline__ Lap load_lap(const MGrid &m, size_t q) {
     return L;
 }
 
-// hdel4_div and hdel4_zcurl at the four T points around the U point,
-// hdel4_grad of both (readyc.cpp:343-441) and hdiffu_del4_0 (:444-507).  The
-// lane reads up and vp over rows j-1..j+1 and columns i-2..i+1.
-__global__ __launch_bounds__(256, 2) void k_rmom_lap1(CGrid g, MGrid m, licom_readyc_mfields f) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+// hdel4_div and hdel4_zcurl at the T points of a 64x4 block of U columns,
+// hdel4_grad of both (readyc.cpp:343-441) and hdiffu_del4_0 (:444-507).
+// Per level the block stages up and vp over rows j0-1..j0+4 and columns
+// i0-2..i0+64 (flat offsets from the block's corner, so a column outside
+// 0..imt-1 reads the neighbouring row's cell, as the flat stencil offsets of
+// the reference do; offsets outside the plane read 0) in LDS, computes the
+// divergence and the vorticity once per T point (rows j0..j0+4, columns
+// i0-1..i0+63; each is shared by four U columns), then every U column forms
+// its viscosity factor and first Laplacian from LDS.  A T point whose
+// stencil leaves the arrays is used by no U column inside the block bounds
+// and gets 0.
+constexpr int L1WX = 67, L1WN = L1WX * 6;   // up/vp window
+constexpr int L1TX = 65, L1TN = L1TX * 5;   // T points
+struct TPoint {
+    double htwe, htwc, htsc, htsn, tr, dyc, dyn, dyw, dynw, dxc, dxn, dxw, dxnw;
+    int kt, e;       // kmt (-1: no T point here), LDS index
+    bool cu;         // i >= 1 (hdel4_zcurl)
+};
+__device__ __forceinline__ void tpoint_load(const CGrid &g, const MGrid &m, int i0, int j0, int e, TPoint &T) {
+    T.kt = -1;
+    T.e = e;
+    if (e >= L1TN) return;
+    const int tr = e / L1TX, tc = e - tr * L1TX;
+    const int it = i0 - 1 + tc, jt = j0 + tr;
+    const long long p = (long long)jt * g.imt + it;
+    T.cu = it >= 1;
+    if (p - g.imt - (T.cu ? 1 : 0) < 0 || p + 1 >= (long long)g.n2) return;
+    T.htwe = m.htw[p + 1];
+    T.htwc = m.htw[p];
+    T.htsc = m.hts[p];
+    T.htsn = m.hts[p - g.imt];
+    T.tr = m.tarea_r[p];
+    T.kt = m.kmt[p];
+    if (T.cu) {
+        T.dyc = m.dyu[p];
+        T.dyn = m.dyu[p - g.imt];
+        T.dyw = m.dyu[p - 1];
+        T.dynw = m.dyu[p - g.imt - 1];
+        T.dxc = m.dxu[p];
+        T.dxn = m.dxu[p - g.imt];
+        T.dxw = m.dxu[p - 1];
+        T.dxnw = m.dxu[p - g.imt - 1];
+    }
+}
+// divergence and vorticity of level k at one T point from the window; the
+// operands are named as the U column (j, i) with T point (j, i) names them
+__device__ __forceinline__ void tpoint_eval(const TPoint &T, int k, const double *U, const double *V,
+                                            double *sDV, double *sCZ) {
+    if (T.e >= L1TN) return;
+    double dv = 0.0e0, cz = 0.0e0;
+    if (k <= T.kt - 1) {
+        const int tr = T.e / L1TX, tc = T.e - tr * L1TX;
+        const int a = (tr + 1) * L1WX + tc + 1, b = tr * L1WX + tc + 1;   // rows j, j-1; column i
+        dv = 0.5 * ((U[a + 1] + U[b + 1]) * T.htwe - (U[a] + U[b]) * T.htwc + (V[a + 1] + V[a]) * T.htsc -
+                    (V[b + 1] + V[b]) * T.htsn) *
+             T.tr;
+        if (T.cu)
+            cz = 0.5 * ((V[a] * T.dyc + V[b] * T.dyn - V[a - 1] * T.dyw - V[b - 1] * T.dynw - U[a] * T.dxc -
+                         U[a - 1] * T.dxw + U[b] * T.dxn + U[b - 1] * T.dxnw));
+    }
+    sDV[T.e] = dv;
+    sCZ[T.e] = cz;
+}
+
+__global__ __launch_bounds__(256) void k_rmom_lap1(CGrid g, MGrid m, licom_readyc_mfields f) {
+    __shared__ double sU[2][L1WN], sV[2][L1WN];
+    __shared__ double sDV[L1TN], sCZ[L1TN];
+    const int tx = threadIdx.x, ty = threadIdx.y, t = tx + 64 * ty;
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
+    const int i = i0 + tx, j = j0 + ty;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
-    if (i >= imt || j >= jmt) return;
-    const size_t n2 = g.n2, q = (size_t)j * imt + i;
+    const size_t n2 = g.n2;
+    const bool in = i < imt && j < jmt;
+    const size_t q = in ? (size_t)j * imt + i : 0;
     // the reference compares 0-based j with POP's 1-based bounds as given
-    const bool reg = j >= m.jb - 2 && j <= m.je && i >= m.ib - 2 && i <= m.ie;
-    const double amf = m.amf[q];
-    const int kmu = m.kmu[q];
-    // T points t: 0 (j, i), 1 (j+1, i), 2 (j, i-1), 3 (j+1, i-1)
-    constexpr int DR[4] = {0, 1, 0, 1}, DC[4] = {0, 0, -1, -1};
-    double htwe[4] = {}, htwc[4] = {}, htsc[4] = {}, htsn[4] = {}, tr[4] = {}, dyc[4] = {}, dyn[4] = {},
-           dyw[4] = {}, dynw[4] = {}, dxc[4] = {}, dxn[4] = {}, dxw[4] = {}, dxnw[4] = {};
-    int kt[4] = {};
-    bool cu[4] = {};
+    const bool reg = in && j >= m.jb - 2 && j <= m.je && i >= m.ib - 2 && i <= m.ie;
+    const double amf = in ? m.amf[q] : 0.0;
+    const int kmu = in ? m.kmu[q] : 0;
     Lap L{};
     double gxf = 0, gyf = 0, dxdy = 0, amd = 1.0;
     if (reg) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const size_t p = q + DR[t] * imt + DC[t];
-            htwe[t] = m.htw[p + 1];
-            htwc[t] = m.htw[p];
-            htsc[t] = m.hts[p];
-            htsn[t] = m.hts[p - imt];
-            tr[t] = m.tarea_r[p];
-            kt[t] = m.kmt[p];
-            cu[t] = i + DC[t] >= 1;   // hdel4_zcurl needs i >= 1
-            if (cu[t]) {
-                dyc[t] = m.dyu[p];
-                dyn[t] = m.dyu[p - imt];
-                dyw[t] = m.dyu[p - 1];
-                dynw[t] = m.dyu[p - imt - 1];
-                dxc[t] = m.dxu[p];
-                dxn[t] = m.dxu[p - imt];
-                dxw[t] = m.dxu[p - 1];
-                dxnw[t] = m.dxu[p - imt - 1];
-            }
-        }
         L = load_lap(m, q);
         gxf = m.dxur[q] * 0.5;
         gyf = m.dyur[q] * 0.5;
@@ -348,50 +388,53 @@ __global__ __launch_bounds__(256, 2) void k_rmom_lap1(CGrid g, MGrid m, licom_re
         dxdy = t_area * t_area * t_area * t_area * t_area * 45.0e0;
         amd = fabs(m.am * amf);
     }
+    TPoint T0, T1;
+    tpoint_load(g, m, i0, j0, t, T0);
+    tpoint_load(g, m, i0, j0, t + 256, T1);
+    // this lane's window cells e = t, t + 256
     const size_t pb = n2 * sizeof(double);
-    uint32_t off[3][4];
+    uint32_t wo[2];
+    bool wv[2];
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-            off[r][c] = boff(reg && (c > 0 || i >= 2), q + (size_t)((r - 1) * imt + (c - 2)));
-    struct Lv {
-        double u[3][4], v[3][4];   // rows j-1..j+1, columns i-2..i+1
-    };
-    auto load = [&](int k, Lv &V) {
+    for (int u = 0; u < 2; ++u) {
+        const int e = t + 256 * u, r = e / L1WX, c = e - r * L1WX;
+        const long long fl = (long long)(j0 - 1 + r) * imt + (i0 - 2 + c);
+        wv[u] = e < L1WN;
+        wo[u] = boff(wv[u] && fl >= 0 && fl < (long long)n2, (size_t)fl);
+    }
+    double lu[2], lv[2];
+    auto load = [&](int k) {
         const size_t o = (size_t)k * n2;
         const auto ru = plane_rsrc(f.up + o, pb), rv = plane_rsrc(f.vp + o, pb);
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                V.u[r][c] = buf_load(ru, off[r][c], 0.0);
-                V.v[r][c] = buf_load(rv, off[r][c], 0.0);
-            }
+        for (int u = 0; u < 2; ++u) {
+            lu[u] = buf_load(ru, wo[u], 0.0);
+            lv[u] = buf_load(rv, wo[u], 0.0);
+        }
     };
-    Lv cur, nxt;
-    load(0, cur);
+    constexpr int DR[4] = {0, 1, 0, 1}, DC[4] = {0, 0, -1, -1};
+    load(0);
     for (int k = 0; k < km; ++k) {
-        if (k + 1 < km) load(k + 1, nxt);
+        double *U = sU[k & 1], *V = sV[k & 1];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (wv[u]) {
+                U[t + 256 * u] = lu[u];
+                V[t + 256 * u] = lv[u];
+            }
+        if (k + 1 < km) load(k + 1);
+        __syncthreads();
+        tpoint_eval(T0, k, U, V, sDV, sCZ);
+        tpoint_eval(T1, k, U, V, sDV, sCZ);
+        __syncthreads();
         double am_f = 1.0e0, hu = 0.0, hv = 0.0;
         if (reg && k <= kmu - 1) {
             double dv[4], cz[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int r0 = 1 + DR[t], c0 = 2 + DC[t];
-                const auto &U = cur.u, &V = cur.v;
-                dv[t] = 0.0e0;
-                cz[t] = 0.0e0;
-                if (k <= kt[t] - 1) {
-                    dv[t] = 0.5 *
-                            ((U[r0][c0 + 1] + U[r0 - 1][c0 + 1]) * htwe[t] - (U[r0][c0] + U[r0 - 1][c0]) * htwc[t] +
-                             (V[r0][c0 + 1] + V[r0][c0]) * htsc[t] - (V[r0 - 1][c0 + 1] + V[r0 - 1][c0]) * htsn[t]) *
-                            tr[t];
-                    if (cu[t])
-                        cz[t] = 0.5 * ((V[r0][c0] * dyc[t] + V[r0 - 1][c0] * dyn[t] - V[r0][c0 - 1] * dyw[t] -
-                                        V[r0 - 1][c0 - 1] * dynw[t] - U[r0][c0] * dxc[t] - U[r0][c0 - 1] * dxw[t] +
-                                        U[r0 - 1][c0] * dxn[t] + U[r0 - 1][c0 - 1] * dxnw[t]));
-                }
+            for (int s = 0; s < 4; ++s) {
+                const int e = (ty + DR[s]) * L1TX + tx + 1 + DC[s];
+                dv[s] = sDV[e];
+                cz[s] = sCZ[e];
             }
             const double gx = gxf * (dv[1] - dv[2] - dv[3] + dv[0]);
             const double gy = gyf * (dv[1] - dv[2] + dv[3] - dv[0]);
@@ -402,16 +445,18 @@ __global__ __launch_bounds__(256, 2) void k_rmom_lap1(CGrid g, MGrid m, licom_re
             am_f = sqrt(af) * dxdy / amd;
             am_f = fmin(40.0e0, am_f);
             am_f = fmax(1.0e0, am_f);
-            const auto &U = cur.u, &V = cur.v;
-            hu = lap_du(L, U[1][2], U[0][2], U[2][2], U[1][3], U[1][1]) +
-                 lap_dm(L, V[1][2], V[0][2], V[2][2], V[1][3], V[1][1]);
-            hv = lap_du(L, V[1][2], V[0][2], V[2][2], V[1][3], V[1][1]) -
-                 lap_dm(L, U[1][2], U[0][2], U[2][2], U[1][3], U[1][1]);
+            // window cell of (j + r - 1, i + c - 2) is (ty + r) * L1WX + tx + c
+            const int w = (ty + 1) * L1WX + tx + 2;
+            hu = lap_du(L, U[w], U[w - L1WX], U[w + L1WX], U[w + 1], U[w - 1]) +
+                 lap_dm(L, V[w], V[w - L1WX], V[w + L1WX], V[w + 1], V[w - 1]);
+            hv = lap_du(L, V[w], V[w - L1WX], V[w + L1WX], V[w + 1], V[w - 1]) -
+                 lap_dm(L, U[w], U[w - L1WX], U[w + L1WX], U[w + 1], U[w - 1]);
         }
-        const size_t c = (size_t)k * n2 + q;
-        m.d2u[c] = am_f * amf * hu;
-        m.d2v[c] = am_f * amf * hv;
-        cur = nxt;
+        if (in) {
+            const size_t c = (size_t)k * n2 + q;
+            m.d2u[c] = am_f * amf * hu;
+            m.d2v[c] = am_f * amf * hv;
+        }
     }
 }
 
