@@ -123,65 +123,110 @@ __global__ __launch_bounds__(256) void k_bcl_prep(BGrid g, licom_bclinc_fields f
     }
 }
 
-// the T-column state of bclinc11 that k_bcl_a carries for each of the four
-// T points around a U point
+// bclinc11's T-column state for one T point of a 64x4 block of U columns
+// (rows j0..j0+4, columns i0-1..i0+63; each T point serves four U columns),
+// carried down the column by the lane that owns the T point, with its vit and
+// gg of the next level loaded a level ahead
+constexpr int BTX = 65, BTN = BTX * 5;
 struct TCol {
-    double wkk, work, ohbt;
+    double wkk, work, ohbt, vt, gk;
     size_t q;
+    int e;          // index among the block's T points (BTN: the lane owns none)
+    bool ok;        // inside the array
 };
 
-__device__ __forceinline__ TCol tcol(const BGrid &g, const licom_bclinc_fields &f, size_t q, double aa) {
+__device__ __forceinline__ TCol tcol(const BGrid &g, const licom_bclinc_fields &f, int i0, int j0, int e,
+                                     double aa) {
     TCol t;
+    t.e = e < BTN ? e : BTN;
+    t.ok = false;
+    t.q = 0;
+    t.wkk = t.work = t.ohbt = t.vt = t.gk = 0.0;
+    if (e >= BTN) return t;
+    const int tr = e / BTX, it = i0 - 1 + (e - tr * BTX), jt = j0 + tr;
+    if (it < 0 || it >= g.imt || jt >= g.jmt) return t;
+    const size_t q = (size_t)jt * g.imt + it;
+    t.ok = true;
     t.q = q;
     t.work = aa * f.h0bf[q] + (1.0 - aa) * f.h0bl[q];   // h0bf already scaled by k_bcl_prep
     t.wkk = (f.psa[q] * g.od0 + t.work * 9.806e0) * f.vit[q];
     t.ohbt = g.ohbt[q];
+    t.vt = f.vit[q];
+    t.gk = f.gg[q];
     return t;
 }
 
+// level k of the recursion (bclinc11_cu) and the wka1/wka2 values there
+__device__ __forceinline__ void tcol_level(const BGrid &g, const licom_bclinc_fields &f, TCol &t, int k,
+                                           double dzk, double zk, double *W1, double *W2, double *G) {
+    if (t.e >= BTN) return;
+    double w1 = 0.0, w2 = 0.0, gk = 0.0;
+    if (t.ok) {
+        const double vt = t.vt;
+        gk = t.gk;
+        if (k + 1 < g.km) {
+            const size_t cn = (size_t)(k + 1) * g.n2 + t.q;
+            t.vt = f.vit[cn];
+            t.gk = f.gg[cn];
+        }
+        const double nxt = t.wkk - gk * dzk * vt;
+        w1 = 0.5 * (t.wkk + nxt);
+        w2 = (1.0 + t.ohbt * zk) * t.work * vt;
+        t.wkk = nxt;
+    }
+    W1[t.e] = w1;
+    W2[t.e] = w2;
+    G[t.e] = gk;
+}
+
 // bclinc2_cu / bclinc6_cu (bclinc.cpp:106-185, 418-486), then bclinc3_cu or
-// bclinc71_cu + bclinc72_cu
+// bclinc71_cu + bclinc72_cu.  Per level the block's T points publish their
+// wka1, wka2 and gg in LDS (double-buffered, one barrier per level); every
+// U column reads its four.
 template <bool EULER>
 __global__ __launch_bounds__(256) void k_bcl_a(BGrid g, licom_bclinc_fields f, int isc) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    __shared__ double sW1[2][BTN], sW2[2][BTN], sG[2][BTN];
+    const int tx = threadIdx.x, ty = threadIdx.y, tl = tx + 64 * ty;
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
+    const int i = i0 + tx, j = j0 + ty;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
-    if (i >= imt || j >= jmt) return;
-    const size_t n2 = g.n2, q = (size_t)j * imt + i;
-    const bool grd = i >= 1 && j < jmt - 1;
-    const bool in2 = interior2(g, j, i);
-    if (!grd) return;
+    const size_t n2 = g.n2;
+    const bool grd = i >= 1 && i < imt && j < jmt - 1;
+    const size_t q = grd ? (size_t)j * imt + i : 0;
+    const bool in2 = grd && interior2(g, j, i);
     double aa = 0;
     if (isc != 0) aa = 0.5;
-    // T points (j, i), (j, i-1), (j+1, i), (j+1, i-1)
-    TCol T[4] = {tcol(g, f, q, aa), tcol(g, f, q - 1, aa), tcol(g, f, q + imt, aa), tcol(g, f, q + imt - 1, aa)};
-    const int kmb = g.kmu[q] - 1;
-    const double fc = g.fcor[q], dxur = g.dxur[q], dyur = g.dyur[q];
-    const double ea = isc == 0 ? g.epea[q] : g.epla[q], eb = isc == 0 ? g.epeb[q] : g.eplb[q];
+    TCol T0 = tcol(g, f, i0, j0, tl, aa), T1 = tcol(g, f, i0, j0, tl + 256, aa);
+    int kmb = -1;
+    double fc = 0, dxur = 0, dyur = 0, ea = 0, eb = 0;
+    if (grd) {
+        kmb = g.kmu[q] - 1;
+        fc = g.fcor[q];
+        dxur = g.dxur[q];
+        dyur = g.dyur[q];
+        ea = isc == 0 ? g.epea[q] : g.epla[q];
+        eb = isc == 0 ? g.epeb[q] : g.eplb[q];
+    }
     const double dt = EULER ? g.dtc : g.dtc2;
+    // this U column's T points (j, i), (j, i-1), (j+1, i), (j+1, i-1)
+    const int e0 = ty * BTX + tx + 1, e1 = e0 - 1, e2 = e0 + BTX, e3 = e2 - 1;
     for (int k = 0; k < km; ++k) {
-        const size_t c = (size_t)k * n2 + q;
+        const int b = k & 1;
         const double dzk = g.dzp[k], zk = g.zkt[k];
-        // bclinc11's recursion and wka1/wka2 of the four T columns
-        double w1[4], w2[4], gk[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const size_t ct = (size_t)k * n2 + T[t].q;
-            const double vt = f.vit[ct];
-            gk[t] = f.gg[ct];
-            const double nxt = T[t].wkk - gk[t] * dzk * vt;
-            w1[t] = 0.5 * (T[t].wkk + nxt);
-            w2[t] = (1.0 + T[t].ohbt * zk) * T[t].work * vt;
-            T[t].wkk = nxt;
-        }
+        tcol_level(g, f, T0, k, dzk, zk, sW1[b], sW2[b], sG[b]);
+        tcol_level(g, f, T1, k, dzk, zk, sW1[b], sW2[b], sG[b]);
+        __syncthreads();
+        if (!grd) continue;
+        const size_t c = (size_t)k * n2 + q;
+        const double *W1 = sW1[b], *W2 = sW2[b], *G = sG[b];
         double gradx = fc * f.vp[c];
         double grady = -fc * f.up[c];
         if (kmb >= k) {
-            // w[0] (j, i), w[1] (j, i-1), w[2] (j+1, i), w[3] (j+1, i-1)
-            const double ggu = 0.25 * (gk[0] + gk[1] + gk[2] + gk[3]);
-            gradx += dxur * 0.5 * (w1[2] - w1[1] - w1[3] + w1[0]);
-            grady += dyur * 0.5 * (w1[2] - w1[1] + w1[3] - w1[0]);
-            gradx -= ggu * dxur * 0.5 * (w2[2] - w2[1] - w2[3] + w2[0]);
-            grady -= ggu * dyur * 0.5 * (w2[2] - w2[1] + w2[3] - w2[0]);
+            const double ggu = 0.25 * (G[e0] + G[e1] + G[e2] + G[e3]);
+            gradx += dxur * 0.5 * (W1[e2] - W1[e1] - W1[e3] + W1[e0]);
+            grady += dyur * 0.5 * (W1[e2] - W1[e1] + W1[e3] - W1[e0]);
+            gradx -= ggu * dxur * 0.5 * (W2[e2] - W2[e1] - W2[e3] + W2[e0]);
+            grady -= ggu * dyur * 0.5 * (W2[e2] - W2[e1] + W2[e3] - W2[e0]);
         }
         grady -= f.dlv[c];
         gradx -= f.dlu[c];
