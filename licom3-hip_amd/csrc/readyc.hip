@@ -160,9 +160,10 @@ __global__ __launch_bounds__(256) void k_readyc_akmu(CGrid g, const double *__re
 // launches over whole arrays, with eight 3-D scratch arrays between them.
 // Here:
 //   k_rmom_upwell   one lane per T column: the scaled velocities of the four
-//                   U cells around it, their divergence level by level (kept
-//                   in registers), its column integral, then the recursion of
-//                   ws (upwell_cu_0 + upwell_cu);
+//                   U cells around it (staged in LDS per level, each U cell
+//                   scaled once for the block), their divergence level by
+//                   level (kept in registers), its column integral, then the
+//                   recursion of ws (upwell_cu_0 + upwell_cu);
 //   k_rmom_lap1     (del4) a 64x4 block of U columns marching k with an
 //                   LDS window of (up, vp): divergence and vorticity once per
 //                   T point (hdel4_div, hdel4_zcurl), then per U column the
@@ -189,77 +190,112 @@ struct MGrid {
 
 __device__ __forceinline__ uint32_t boff(bool ok, size_t idx) { return ok ? (uint32_t)idx * 8u : kBufOff; }
 
+// The U cells of a 64x4 block of T columns: rows j0-1..j0+3, columns
+// i0..i0+64 (each serves up to four T columns); the lane that owns one holds
+// its upwell_cu_0 factor and stages (u, v) times it in LDS per level.
+constexpr int UWX = 65, UWN = UWX * 5;
+struct UCell {
+    double s;        // 1 + (h0 at the U point) / H there
+    uint32_t off;    // plane offset (kBufOff: no U cell the block uses)
+    int e;           // index among the block's U cells (UWN: the lane owns none)
+};
+__device__ __forceinline__ UCell ucell(const CGrid &g, const MGrid &m, const licom_readyc_mfields &f, int i0,
+                                       int j0, int e) {
+    UCell u;
+    u.e = e < UWN ? e : UWN;
+    u.s = 0.0;
+    u.off = kBufOff;
+    if (e >= UWN) return u;
+    const int r = e / UWX, iu = i0 + (e - r * UWX), ju = j0 - 1 + r;
+    // the T columns 1 <= j <= jmt-2, 1 <= i <= imt-2 use U cells
+    // 0 <= ju <= jmt-2, 1 <= iu <= imt-1
+    if (ju < 0 || ju > g.jmt - 2 || iu < 1 || iu > g.imt - 1) return u;
+    const size_t p = (size_t)ju * g.imt + iu, imt = g.imt;
+    // upwell_cu_0 (readyc.cpp:609-621)
+    const double w = g.au0[p] * f.h0[p] + g.aus[p] * f.h0[p + imt] + g.auw[p] * f.h0[p - 1] +
+                     g.ausw[p] * f.h0[p + imt - 1];
+    u.s = 1.0e0 + w * m.ohbu[p];
+    u.off = (uint32_t)p * 8u;
+    return u;
+}
+
 __global__ __launch_bounds__(256) void k_rmom_upwell(CGrid g, MGrid m, licom_readyc_mfields f) {
     __shared__ double sDZ[LICOM_MAX_KM];
-    {
-        const int t = threadIdx.x + 64 * threadIdx.y;
-        for (int r = t; r < g.km; r += 256) sDZ[r] = m.dzp[r];
-        __syncthreads();
-    }
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    __shared__ double sUK[2][UWN], sVK[2][UWN];
+    const int tx = threadIdx.x, ty = threadIdx.y, tl = tx + 64 * ty;
+    for (int r = tl; r < g.km; r += 256) sDZ[r] = m.dzp[r];
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
+    const int i = i0 + tx, j = j0 + ty;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
     // upwell_cu writes ws in 1 <= j < jmt-1, 1 <= i < imt-1 only
-    if (i < 1 || i > imt - 2 || j < 1 || j > jmt - 2) return;
-    const size_t n2 = g.n2, q = (size_t)j * imt + i;
-    // upwell_cu_0 (readyc.cpp:609-621): 1 + (h0 at the U point) / H there
-    auto scale = [&](size_t p) {
-        const double w = g.au0[p] * f.h0[p] + g.aus[p] * f.h0[p + imt] + g.auw[p] * f.h0[p - 1] +
-                         g.ausw[p] * f.h0[p + imt - 1];
-        return 1.0e0 + w * m.ohbu[p];
-    };
-    // the U cells (j, i+1), (j-1, i+1), (j, i), (j-1, i)
-    const size_t qa = q + 1, qb = q - imt + 1, qd = q - imt;
-    const double sa = scale(qa), sb = scale(qb), sc = scale(q), sd = scale(qd);
-    const double htwe = m.htw[q + 1], htwc = m.htw[q], htsc = m.hts[q], htsn = m.hts[q - imt];
-    const double tr = m.tarea_r[q], ohbt = m.ohbt[q], h0 = f.h0[q];
-    const int kmt = m.kmt[q];
+    const bool col = i >= 1 && i <= imt - 2 && j >= 1 && j <= jmt - 2;
+    const size_t n2 = g.n2, q = col ? (size_t)j * imt + i : 0;
+    const UCell C0 = ucell(g, m, f, i0, j0, tl), C1 = ucell(g, m, f, i0, j0, tl + 256);
+    double htwe = 0, htwc = 0, htsc = 0, htsn = 0, tr = 0, ohbt = 0, h0 = 0;
+    int kmt = 0;
+    if (col) {
+        htwe = m.htw[q + 1];
+        htwc = m.htw[q];
+        htsc = m.hts[q];
+        htsn = m.hts[q - imt];
+        tr = m.tarea_r[q];
+        ohbt = m.ohbt[q];
+        h0 = f.h0[q];
+        kmt = m.kmt[q];
+    }
     const size_t pb = n2 * sizeof(double);
-    const uint32_t oa = (uint32_t)qa * 8u, ob = (uint32_t)qb * 8u, oc = (uint32_t)q * 8u,
-                   od = (uint32_t)qd * 8u;
-    struct Lv {
-        double ua, ub, uc, ud, va, vb, vc, vd, vt;
-    };
-    auto load = [&](int k, Lv &L) {
+    const uint32_t oc = col ? (uint32_t)q * 8u : kBufOff;
+    double lu0, lv0, lu1, lv1, vt, vtn;
+    auto load = [&](int k) {
         const size_t o = (size_t)k * n2;
         const auto ru = plane_rsrc(f.u + o, pb), rv = plane_rsrc(f.v + o, pb);
-        L.ua = buf_load(ru, oa, 0.0);
-        L.ub = buf_load(ru, ob, 0.0);
-        L.uc = buf_load(ru, oc, 0.0);
-        L.ud = buf_load(ru, od, 0.0);
-        L.va = buf_load(rv, oa, 0.0);
-        L.vb = buf_load(rv, ob, 0.0);
-        L.vc = buf_load(rv, oc, 0.0);
-        L.vd = buf_load(rv, od, 0.0);
-        L.vt = buf_load(plane_rsrc(f.vit + o, pb), oc, 0.0);
+        lu0 = buf_load(ru, C0.off, 0.0);
+        lv0 = buf_load(rv, C0.off, 0.0);
+        lu1 = buf_load(ru, C1.off, 0.0);
+        lv1 = buf_load(rv, C1.off, 0.0);
+        vtn = buf_load(plane_rsrc(f.vit + o, pb), oc, 0.0);
     };
+    // this T column's U cells (j, i+1), (j-1, i+1), (j, i), (j-1, i)
+    const int ec = (ty + 1) * UWX + tx, ea = ec + 1, ed = ty * UWX + tx, eb = ed + 1;
     // upwell_cu (readyc.cpp:636-671): divergence per level, its integral
     double wk[LICOM_MAX_KM];
     double work = 0.0e0;
     uint64_t vone = 0;      // levels whose vit is exactly 1.0
     bool v01 = true;        // every vit of the column is +0.0 or 1.0
-    Lv cur, nxt;
-    load(0, cur);
+    load(0);
 #pragma unroll
     for (int k = 0; k < LICOM_MAX_KM; ++k) {
         if (k < km) {
-            if (k + 1 < km) load(k + 1, nxt);
+            const int b = k & 1;
+            if (C0.e < UWN) {
+                sUK[b][C0.e] = C0.s * lu0;
+                sVK[b][C0.e] = C0.s * lv0;
+            }
+            if (C1.e < UWN) {
+                sUK[b][C1.e] = C1.s * lu1;
+                sVK[b][C1.e] = C1.s * lv1;
+            }
+            vt = vtn;
+            if (k + 1 < km) load(k + 1);
+            __syncthreads();
             double d = 0.0e0;
             if (k <= kmt - 1) {
-                const double uka = sa * cur.ua, ukb = sb * cur.ub, ukc = sc * cur.uc, ukd = sd * cur.ud;
-                const double vka = sa * cur.va, vkb = sb * cur.vb, vkc = sc * cur.vc, vkd = sd * cur.vd;
+                const double *UK = sUK[b], *VK = sVK[b];
+                const double uka = UK[ea], ukb = UK[eb], ukc = UK[ec], ukd = UK[ed];
+                const double vka = VK[ea], vkb = VK[eb], vkc = VK[ec], vkd = VK[ed];
                 d = 0.5 * ((uka + ukb) * htwe - (ukc + ukd) * htwc + (vka + vkc) * htsc - (vkb + vkd) * htsn) *
                     tr;
             }
             wk[k] = d;
-            work -= sDZ[k] * d * cur.vt;
-            const uint64_t bits = __builtin_bit_cast(uint64_t, cur.vt);
+            work -= sDZ[k] * d * vt;
+            const uint64_t bits = __builtin_bit_cast(uint64_t, vt);
             if (bits == 0x3FF0000000000000ull)
                 vone |= 1ull << k;
             else if (bits != 0)
                 v01 = false;
-            cur = nxt;
         }
     }
+    if (!col) return;
     const double wo = work * ohbt;
     const double r = 1.0e0 / (1.0e0 + h0 * ohbt);
     double prev = f.ws[q];
