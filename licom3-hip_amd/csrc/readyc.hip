@@ -175,7 +175,9 @@ __global__ __launch_bounds__(256) void k_readyc_akmu(CGrid g, const double *__re
 //                   advection, the horizontal viscosity (hdiffu_del2_cu or
 //                   hdiffu_del4_1), the depth sums and the surface and bottom
 //                   stress (readyc4) and the explicit vertical viscosity
-//                   (readyc4_2), every output cell written once.
+//                   (readyc4_2), every output cell written once; the
+//                   neighbours' u, v, ws and viscosity operand come from a
+//                   per-level LDS window of the block.
 // Each level's inputs are loaded a level ahead through plane buffers
 // (out-of-range offsets return zero for the lanes that do not use them).
 // Operation order follows the reference statements (-ffp-contract=off).
@@ -496,27 +498,31 @@ __global__ __launch_bounds__(256) void k_rmom_lap1(CGrid g, MGrid m, licom_ready
     }
 }
 
+// Window of the U points around a 64x4 block: rows j0-1..j0+4, columns
+// i0-1..i0+64, at flat offsets from the block's corner (0 outside the plane).
+constexpr int MWX = 66, MWN = MWX * 6;
+
 template <bool DEL4>
 __global__ __launch_bounds__(256, 2) void k_rmom_main(CGrid g, MGrid m, licom_readyc_mfields f) {
     __shared__ double sDZ[LICOM_MAX_KM], sOZP[LICOM_MAX_KM], sOZT[LICOM_MAX_KM];
-    {
-        const int t = threadIdx.x + 64 * threadIdx.y;
-        for (int r = t; r < g.km; r += 256) {
-            sDZ[r] = m.dzp[r];
-            sOZP[r] = m.odzp[r];
-            sOZT[r] = g.odzt[r];
-        }
-        __syncthreads();
+    // per level: u, v and the viscosity operand of level k, ws of level k+1
+    __shared__ double sWU[2][MWN], sWV[2][MWN], sWA[2][MWN], sWB[2][MWN], sWW[2][MWN];
+    const int tx = threadIdx.x, ty = threadIdx.y, tl = tx + 64 * ty;
+    for (int r = tl; r < g.km; r += 256) {
+        sDZ[r] = m.dzp[r];
+        sOZP[r] = m.odzp[r];
+        sOZT[r] = g.odzt[r];
     }
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
+    const int i = i0 + tx, j = j0 + ty;
     const int imt = g.imt, jmt = g.jmt, km = g.km;
-    if (i >= imt || j >= jmt) return;
-    const size_t n2 = g.n2, q = (size_t)j * imt + i;
-    const bool in = j >= 1 && j < jmt - 1 && i >= 1 && i < imt - 1;
+    const bool act = i < imt && j < jmt;
+    const size_t n2 = g.n2, q = act ? (size_t)j * imt + i : 0;
+    const bool in = act && j >= 1 && j < jmt - 1 && i >= 1 && i < imt - 1;
     // hdiffu_del2_cu / hdiffu_del4_1: 0-based j against POP's 1-based bounds
-    const bool hreg = j >= m.jb - 1 && j < m.je && i >= m.ib - 1 && i < m.ie;
-    const double ohbu = m.ohbu[q], c0f = m.c0f, cag = m.cag, sag = m.sag, am = m.am;
-    const int kmu = m.kmu[q];
+    const bool hreg = act && j >= m.jb - 1 && j < m.je && i >= m.ib - 1 && i < m.ie;
+    const double ohbu = act ? m.ohbu[q] : 0.0, c0f = m.c0f, cag = m.cag, sag = m.sag, am = m.am;
+    const int kmu = act ? m.kmu[q] : 0;
     double a0 = 0, as = 0, aw = 0, asw = 0, huew = 0, huec = 0, huns = 0, hunc = 0, ua = 0, sn = 0, su = 0,
            sv = 0;
     Lap L{};
@@ -536,117 +542,127 @@ __global__ __launch_bounds__(256, 2) void k_rmom_main(CGrid g, MGrid m, licom_re
     }
     if (hreg) L = load_lap(m, q);
     const size_t pb = n2 * sizeof(double);
-    const uint32_t oc = (uint32_t)q * 8u, ow = boff(in, q - 1), oe = boff(in, q + 1), on = boff(in, q - imt),
-                   os = boff(in, q + imt), osw = boff(in, q + imt - 1), ocin = in ? oc : kBufOff;
-    const uint32_t hw = boff(hreg, q - 1), he = boff(hreg, q + 1), hn = boff(hreg, q - imt),
-                   hs = boff(hreg, q + imt), hc = hreg ? oc : kBufOff;
+    const uint32_t oc = act ? (uint32_t)q * 8u : kBufOff, ocin = in ? oc : kBufOff;
     const double *ha = DEL4 ? m.d2u : f.up, *hb = DEL4 ? m.d2v : f.vp;
+    // this lane's window cells e = tl, tl + 256
+    uint32_t wo[2];
+    bool wv[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int e = tl + 256 * u, r = e / MWX, c = e - r * MWX;
+        const long long fl = (long long)(j0 - 1 + r) * imt + (i0 - 1 + c);
+        wv[u] = e < MWN;
+        wo[u] = boff(wv[u] && fl >= 0 && fl < (long long)n2, (size_t)fl);
+    }
+    // window values of the next level to stage: u, v, a, b of level k, ws of k+1
+    double lu[2], lv[2], la[2], lb[2], lw[2];
+    auto load_win = [&](int k) {
+        const size_t o = (size_t)k * n2;
+        const auto ru = plane_rsrc(f.u + o, pb), rv = plane_rsrc(f.v + o, pb);
+        const auto ra = plane_rsrc(ha + o, pb), rb = plane_rsrc(hb + o, pb);
+        const auto rw = plane_rsrc(f.ws + (k + 1 < km ? o + n2 : o), pb);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            lu[u] = buf_load(ru, wo[u], 0.0);
+            lv[u] = buf_load(rv, wo[u], 0.0);
+            la[u] = buf_load(ra, wo[u], 0.0);
+            lb[u] = buf_load(rb, wo[u], 0.0);
+            lw[u] = buf_load(rw, wo[u], 0.0);
+        }
+    };
+    // the lane's own cell: u, v (for level k+1's vertical advection), up, vp, viv, akmu
     struct Lv {
-        double u, uw, ue, un, us, v, vw, ve, vn, vs;   // u, v around the U point
-        double wc, ws, ww, wsw;                         // ws of the four T points around it
-        double up, vp, vv, ak;
+        double u, v, up, vp, vv, ak;
     };
     auto load = [&](int k, Lv &V) {
         const size_t o = (size_t)k * n2;
-        const auto ru = plane_rsrc(f.u + o, pb), rv = plane_rsrc(f.v + o, pb), rw = plane_rsrc(f.ws + o, pb);
-        V.u = buf_load(ru, ocin, 0.0);
-        V.uw = buf_load(ru, ow, 0.0);
-        V.ue = buf_load(ru, oe, 0.0);
-        V.un = buf_load(ru, on, 0.0);
-        V.us = buf_load(ru, os, 0.0);
-        V.v = buf_load(rv, ocin, 0.0);
-        V.vw = buf_load(rv, ow, 0.0);
-        V.ve = buf_load(rv, oe, 0.0);
-        V.vn = buf_load(rv, on, 0.0);
-        V.vs = buf_load(rv, os, 0.0);
-        V.wc = buf_load(rw, ocin, 0.0);
-        V.ws = buf_load(rw, os, 0.0);
-        V.ww = buf_load(rw, ow, 0.0);
-        V.wsw = buf_load(rw, osw, 0.0);
+        V.u = buf_load(plane_rsrc(f.u + o, pb), ocin, 0.0);
+        V.v = buf_load(plane_rsrc(f.v + o, pb), ocin, 0.0);
         V.up = buf_load(plane_rsrc(f.up + o, pb), oc, 0.0);
         V.vp = buf_load(plane_rsrc(f.vp + o, pb), oc, 0.0);
         V.vv = buf_load(plane_rsrc(f.viv + o, pb), oc, 0.0);
         V.ak = buf_load(plane_rsrc(f.akmu + o, pb), ocin, 0.0);
     };
-    // the viscosity operand (up, vp or d2u, d2v) around the U point, loaded
-    // for the current level only
-    struct Hv {
-        double a, an, as, ae, aw, b, bn, bs, be, bw;
-    };
-    auto load_h = [&](int k, const Lv &C, Hv &H) {
-        const size_t o = (size_t)k * n2;
-        const auto ra = plane_rsrc(ha + o, pb), rb = plane_rsrc(hb + o, pb);
-        H.an = buf_load(ra, hn, 0.0);
-        H.as = buf_load(ra, hs, 0.0);
-        H.ae = buf_load(ra, he, 0.0);
-        H.aw = buf_load(ra, hw, 0.0);
-        H.bn = buf_load(rb, hn, 0.0);
-        H.bs = buf_load(rb, hs, 0.0);
-        H.be = buf_load(rb, he, 0.0);
-        H.bw = buf_load(rb, hw, 0.0);
-        if (DEL4) {
-            H.a = buf_load(ra, hc, 0.0);
-            H.b = buf_load(rb, hc, 0.0);
-        } else {
-            H.a = C.up;
-            H.b = C.vp;
-        }
-    };
+    // this U point and its neighbours in the window
+    const int w0 = (ty + 1) * MWX + tx + 1, wW = w0 - 1, wE = w0 + 1, wN = w0 - MWX, wS = w0 + MWX,
+              wSW = wS - 1;
     constexpr double p25 = 0.25, p5 = 0.5, aid = 1 - 0.5;   // (1 - aidif)
     Lv cur, nxt{};
     load(0, cur);
+    load_win(0);
     // w at the U point (readyc3), level 0
-    double wu = a0 * (cur.wc) + as * (cur.ws) + aw * (cur.ww) + asw * (cur.wsw);
+    double wu = 0.0;
+    {
+        const auto rw = plane_rsrc(f.ws, pb);
+        const uint32_t os = in ? (uint32_t)(q + imt) * 8u : kBufOff, ow = in ? (uint32_t)(q - 1) * 8u : kBufOff,
+                       osw = in ? (uint32_t)(q + imt - 1) * 8u : kBufOff;
+        wu = a0 * (buf_load(rw, ocin, 0.0)) + as * (buf_load(rw, os, 0.0)) + aw * (buf_load(rw, ow, 0.0)) +
+             asw * (buf_load(rw, osw, 0.0));
+    }
     double uprv = 0, vprv = 0, upprv = 0, vpprv = 0, akprv = 0, wkprv = 0;
     double dlub = 0.0, dlvb = 0.0, wkb = 0.0, ub = 0.0, vb = 0.0;
     for (int k = 0; k < km; ++k) {
         const bool more = k + 1 < km;
-        Hv hh;
-        load_h(k, cur, hh);
-        if (more) load(k + 1, nxt);
+        const int bf = k & 1;
+        double *U = sWU[bf], *V = sWV[bf], *A = sWA[bf], *B = sWB[bf], *W = sWW[bf];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (wv[u]) {
+                const int e = tl + 256 * u;
+                U[e] = lu[u];
+                V[e] = lv[u];
+                A[e] = la[u];
+                B[e] = lb[u];
+                W[e] = lw[u];
+            }
+        if (more) {
+            load(k + 1, nxt);
+            load_win(k + 1);
+        }
+        __syncthreads();
         const size_t c = (size_t)k * n2 + q;
         const double wkc = c0f * sqrt(cur.up * cur.up + cur.vp * cur.vp);   // hdiffu_*'s wka
         double dlu = 0.0, dlv = 0.0;
         double wun = 0.0;
         if (in) {
+            const double cu = U[w0], cuw = U[wW], cue = U[wE], cun = U[wN], cus = U[wS];
+            const double cv = V[w0], cvw = V[wW], cve = V[wE], cvn = V[wN], cvs = V[wS];
             // readyc3 (readyc.cpp:258-269) and advection_momentum_cu (:694-716)
-            if (more) wun = a0 * (nxt.wc) + as * (nxt.ws) + aw * (nxt.ww) + asw * (nxt.wsw);
-            const double uwi = (cur.uw + cur.u) * p25 * huew, uwe = (cur.u + cur.ue) * p25 * huec;
-            const double vsj = (cur.v + cur.vs) * p25 * huns, vsn = (cur.vn + cur.v) * p25 * hunc;
-            dlu = (-uwi * (cur.u - cur.uw) - uwe * (cur.ue - cur.u) - vsj * (cur.us - cur.u) -
-                   vsn * (cur.u - cur.un)) *
-                  ua;
-            dlv = (-uwi * (cur.v - cur.vw) - uwe * (cur.ve - cur.v) - vsj * (cur.vs - cur.v) -
-                   vsn * (cur.v - cur.vn)) *
-                  ua;
+            if (more) wun = a0 * (W[w0]) + as * (W[wS]) + aw * (W[wW]) + asw * (W[wSW]);
+            const double uwi = (cuw + cu) * p25 * huew, uwe = (cu + cue) * p25 * huec;
+            const double vsj = (cv + cvs) * p25 * huns, vsn = (cvn + cv) * p25 * hunc;
+            dlu = (-uwi * (cu - cuw) - uwe * (cue - cu) - vsj * (cus - cu) - vsn * (cu - cun)) * ua;
+            dlv = (-uwi * (cv - cvw) - uwe * (cve - cv) - vsj * (cvs - cv) - vsn * (cv - cvn)) * ua;
             double z1 = 0.0e0, z2 = 0.0e0, z3 = 0.0e0, z4 = 0.0e0;
             if (k > 0) {
-                z1 = wu * (uprv - cur.u);
-                z3 = wu * (vprv - cur.v);
+                z1 = wu * (uprv - cu);
+                z3 = wu * (vprv - cv);
             }
             if (k < km - 1) {
-                z2 = wun * (cur.u - nxt.u);
-                z4 = wun * (cur.v - nxt.v);
+                z2 = wun * (cu - nxt.u);
+                z4 = wun * (cv - nxt.v);
             }
             dlu += -p5 * sOZP[k] * (z1 + z2);
             dlv += -p5 * sOZP[k] * (z3 + z4);
             // hdiffu_del2_cu (:285-335) / hdiffu_del4_1 (:521-559)
             double hu = 0.0, hv = 0.0;
             if (hreg) {
-                const Hv &H = hh;
+                const double Ha = DEL4 ? A[w0] : cur.up, Hb = DEL4 ? B[w0] : cur.vp;
+                const double an = A[wN], as_ = A[wS], ae = A[wE], aw_ = A[wW];
+                const double bn = B[wN], bs = B[wS], be = B[wE], bw = B[wW];
                 if (DEL4) {
-                    hu = am * (lap_du(L, H.a, H.an, H.as, H.ae, H.aw) + lap_dm(L, H.b, H.bn, H.bs, H.be, H.bw));
-                    hv = am * (lap_du(L, H.b, H.bn, H.bs, H.be, H.bw) - lap_dm(L, H.a, H.an, H.as, H.ae, H.aw));
+                    hu = am * (lap_du(L, Ha, an, as_, ae, aw_) + lap_dm(L, Hb, bn, bs, be, bw));
+                    hv = am * (lap_du(L, Hb, bn, bs, be, bw) - lap_dm(L, Ha, an, as_, ae, aw_));
                 } else {
-                    hu = am * (lap_du(L, H.a, H.an, H.as, H.ae, H.aw) + lap_dm(L, H.b, H.bn, H.bs, H.be, H.bw)) *
-                         cur.vv;
-                    hv = am * (lap_du(L, H.b, H.bn, H.bs, H.be, H.bw) - lap_dm(L, H.a, H.an, H.as, H.ae, H.aw)) *
-                         cur.vv;
+                    hu = am * (lap_du(L, Ha, an, as_, ae, aw_) + lap_dm(L, Hb, bn, bs, be, bw)) * cur.vv;
+                    hv = am * (lap_du(L, Hb, bn, bs, be, bw) - lap_dm(L, Ha, an, as_, ae, aw_)) * cur.vv;
                 }
             }
             if (k > kmu - 1) hu = hv = 0.0;
             dlv = dlv + hv;
             dlu = dlu + hu;
+            uprv = cu;
+            vprv = cv;
         }
         // readyc4's depth sums (:730-743) see the tendencies before readyc4_2
         dlub += sDZ[k] * ohbu * dlu * cur.vv;
@@ -680,11 +696,11 @@ __global__ __launch_bounds__(256, 2) void k_rmom_main(CGrid g, MGrid m, licom_re
             dlv = dlv + sOZP[k] * (dv1 - dv2);
             dlu = dlu + sOZP[k] * (du1 - du2);
         }
-        f.dlu[c] = dlu;
-        f.dlv[c] = dlv;
-        f.wka[c] = wkc;
-        uprv = cur.u;
-        vprv = cur.v;
+        if (act) {
+            f.dlu[c] = dlu;
+            f.dlv[c] = dlv;
+            f.wka[c] = wkc;
+        }
         upprv = cur.up;
         vpprv = cur.vp;
         akprv = cur.ak;
@@ -692,6 +708,7 @@ __global__ __launch_bounds__(256, 2) void k_rmom_main(CGrid g, MGrid m, licom_re
         wu = wun;
         cur = nxt;
     }
+    if (!act) return;
     // readyc4 (:745-785): surface stress, bottom drag at the deepest wet level
     if (in) {
         double sbcx = 0.0e0, sbcy = 0.0e0, bbcx = 0.0e0, bbcy = 0.0e0;
