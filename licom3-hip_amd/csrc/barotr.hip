@@ -151,37 +151,53 @@ __global__ __launch_bounds__(256) void k_bt0(TGrid g, licom_barotr_fields f) {
 // (barotr.cpp:272-407): the first Laplacian of (ubp, vbp) scaled by the
 // viscosity factor
 __global__ __launch_bounds__(256) void k_bt1_lap1(TGrid g, licom_barotr_fields f) {
-    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+    // divergence and vorticity once per T point of the block (rows j0..j0+4,
+    // columns i0-1..i0+63, flat offsets from the block's corner; each T point
+    // serves four U points), shared through LDS
+    constexpr int TX = 65, TN = TX * 5;
+    __shared__ double sDV[TN], sCZ[TN];
+    const int tx = threadIdx.x, ty = threadIdx.y, tl = tx + 64 * ty;
+    const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 4;
     const int imt = g.imt, jmt = g.jmt;
+    const double *u = f.ubp, *v = f.vbp;
+    for (int e = tl; e < TN; e += 256) {
+        const int tr = e / TX, ii = i0 - 1 + (e - tr * TX), jj = j0 + tr;
+        double dv = 0.0e0, cz = 0.0e0;
+        // a T point outside the array is used by no U point inside the
+        // bounds (column -1 is the previous row's last cell, as the flat
+        // offsets of the reference give it)
+        if (jj >= 1 && jj < jmt && ii < imt) {
+            const size_t p = (size_t)((long long)jj * imt + ii);
+            const bool wet = 0 <= g.kmt[p] - 1;
+            if (ii < imt - 1 && wet)
+                dv = 0.5 *
+                     ((u[p + 1] + u[p - imt + 1]) * g.htw[p + 1] - (u[p] + u[p - imt]) * g.htw[p] +
+                      (v[p + 1] + v[p]) * g.hts[p] - (v[p - imt + 1] + v[p - imt]) * g.hts[p - imt]) *
+                     g.tarea_r[p];
+            if (ii >= 1 && wet)
+                cz = 0.5 * ((v[p] * g.dyu[p] + v[p - imt] * g.dyu[p - imt] - v[p - 1] * g.dyu[p - 1] -
+                             v[p - imt - 1] * g.dyu[p - imt - 1] - u[p] * g.dxu[p] - u[p - 1] * g.dxu[p - 1] +
+                             u[p - imt] * g.dxu[p - imt] + u[p - imt - 1] * g.dxu[p - imt - 1]));
+        }
+        sDV[e] = dv;
+        sCZ[e] = cz;
+    }
+    __syncthreads();
+    const int i = i0 + tx, j = j0 + ty;
     if (i >= imt || j >= jmt) return;
     const size_t q = (size_t)j * imt + i;
-    const double *u = f.ubp, *v = f.vbp;
     double am_f = 1.0e0, hu = 0, hv = 0;
     if (j >= g.jb - 2 && j <= g.je && i >= g.ib - 2 && i <= g.ie && g.kmu[q] - 1 >= 0) {
-        // divergence and vorticity at the T points (j, i), (j, i-1), (j+1, i), (j+1, i-1)
-        auto dv = [&](int jj, int ii) {
-            const size_t p = (size_t)jj * imt + ii;
-            if (!(ii < imt - 1 && jj >= 1 && 0 <= g.kmt[p] - 1)) return 0.0e0;
-            return 0.5 *
-                   ((u[p + 1] + u[p - imt + 1]) * g.htw[p + 1] - (u[p] + u[p - imt]) * g.htw[p] +
-                    (v[p + 1] + v[p]) * g.hts[p] - (v[p - imt + 1] + v[p - imt]) * g.hts[p - imt]) *
-                   g.tarea_r[p];
-        };
-        auto cz = [&](int jj, int ii) {
-            const size_t p = (size_t)jj * imt + ii;
-            if (!(ii >= 1 && jj >= 1 && 0 <= g.kmt[p] - 1)) return 0.0e0;
-            return 0.5 * ((v[p] * g.dyu[p] + v[p - imt] * g.dyu[p - imt] - v[p - 1] * g.dyu[p - 1] -
-                           v[p - imt - 1] * g.dyu[p - imt - 1] - u[p] * g.dxu[p] - u[p - 1] * g.dxu[p - 1] +
-                           u[p - imt] * g.dxu[p - imt] + u[p - imt - 1] * g.dxu[p - imt - 1]));
-        };
+        // the T points (j, i), (j+1, i), (j, i-1), (j+1, i-1)
+        const int ec = ty * TX + tx + 1, es = ec + TX, ew = ec - 1, esw = es - 1;
         const double gxf = g.dxur[q] * 0.5, gyf = g.dyur[q] * 0.5;
-        double wc = dv(j, i), ws = dv(j + 1, i), ww = dv(j, i - 1), wsw = dv(j + 1, i - 1);
+        double wc = sDV[ec], ws = sDV[es], ww = sDV[ew], wsw = sDV[esw];
         double gx = gxf * (ws - ww - wsw + wc), gy = gyf * (ws - ww + wsw - wc);
         double af = 0.0 + (gx * gx + gy * gy);
-        wc = cz(j, i);
-        ws = cz(j + 1, i);
-        ww = cz(j, i - 1);
-        wsw = cz(j + 1, i - 1);
+        wc = sCZ[ec];
+        ws = sCZ[es];
+        ww = sCZ[ew];
+        wsw = sCZ[esw];
         gx = gxf * (ws - ww - wsw + wc);
         gy = gyf * (ws - ww + wsw - wc);
         af += gx * gx + gy * gy;
